@@ -1,0 +1,319 @@
+"""astrosimgpu_cuda_amd -- MI355X-native astrosim_fused step pipeline.
+
+Python face of the C ABI in ``include/astrosim.h`` (``lib/libastrosim.so``).
+The objects mirror the reference program's own structure
+(src/astrosim_fused.cu): a :class:`Network` is what ``ScalingConfig::make``,
+``build_connectivity`` and the population draws produce from ``(--n, --seed)``;
+a :class:`Simulation` is the device state plus the time loop, configured by
+the reference's remaining flags (``--dt --substeps --block --w-n2a --w-a2n
+--no-stp``). Every call goes to the HIP library; there is no fallback, and a
+missing library raises :class:`LibraryNotBuilt`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "AstrosimError", "LibraryNotBuilt", "Network", "Simulation", "lib_path", "cli_path",
+    "device_info", "device_memory", "FIELDS", "FP64", "FP32C", "PLAIN", "BLOCKED", "load",
+]
+
+PKG = Path(__file__).resolve().parent
+FP64, FP32C = 0, 1
+PLAIN, BLOCKED = 0, 1
+FIELDS = {"Ca": 0, "IP3": 1, "h": 2, "V": 3, "w": 4, "g_ex": 5, "dg_ex": 6, "g_in": 7,
+          "dg_in": 8, "I_sic_hold": 9, "stp_x": 10, "stp_u": 11, "stp_t_last": 12}
+ASTRO_FIELDS = {"Ca", "IP3", "h"}
+_STATUS = {0: "AS_OK", 1: "AS_ERR_INVALID", 2: "AS_ERR_HIP", 3: "AS_ERR_NOMEM",
+           4: "AS_ERR_NODEVICE", 5: "AS_ERR_IO", 6: "AS_ERR_RANGE"}
+
+
+class AstrosimError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{_STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class LibraryNotBuilt(RuntimeError):
+    pass
+
+
+def lib_path() -> Path:
+    return PKG / "lib" / "libastrosim.so"
+
+
+def cli_path() -> Path:
+    return PKG / "bin" / "astrosim_fused"
+
+
+class _Config(C.Structure):
+    _fields_ = [("dt", C.c_double), ("substeps", C.c_int32), ("block", C.c_int32),
+                ("w_n2a", C.c_double), ("w_a2n", C.c_double), ("stp_enabled", C.c_int32),
+                ("precision", C.c_int32), ("method", C.c_int32)]
+
+
+class _NetInfo(C.Structure):
+    _fields_ = [("n_total", C.c_uint32), ("n_exc", C.c_uint32), ("n_inh", C.c_uint32),
+                ("n_astro", C.c_uint32), ("n_neurons", C.c_uint32), ("p_primary", C.c_double),
+                ("n_exc_edges", C.c_uint64), ("n_inh_edges", C.c_uint64),
+                ("n_e2a_edges", C.c_uint64), ("n_sic_edges", C.c_uint64),
+                ("build_seconds", C.c_double)]
+
+
+_lib = None
+
+# (name, restype, argtypes) for every symbol include/astrosim.h declares.
+SIGNATURES = [
+    ("as_config_default", None, [C.POINTER(_Config)]),
+    ("as_network_build", C.c_int, [C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("as_network_info_get", C.c_int, [C.c_void_p, C.POINTER(_NetInfo)]),
+    ("as_network_csr", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    ("as_network_params", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ("as_network_dump_edges", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("as_network_free", None, [C.c_void_p]),
+    ("as_sim_footprint", C.c_int, [C.c_void_p, C.POINTER(_Config), C.POINTER(C.c_uint64)]),
+    ("as_sim_create", C.c_int, [C.c_void_p, C.POINTER(_Config), C.POINTER(C.c_void_p)]),
+    ("as_sim_step", C.c_int, [C.c_void_p, C.c_int64]),
+    ("as_sim_sync", C.c_int, [C.c_void_p]),
+    ("as_sim_steps", C.c_int64, [C.c_void_p]),
+    ("as_sim_spike_count", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("as_sim_reset_ever_active", C.c_int, [C.c_void_p]),
+    ("as_sim_read_state", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ("as_sim_read_refractory", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("as_sim_read_ever_active", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("as_sim_read_raster", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
+    ("as_sim_bytes", C.c_uint64, [C.c_void_p]),
+    ("as_sim_set_timing", C.c_int, [C.c_void_p, C.c_int]),
+    ("as_sim_kernel_time", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    ("as_sim_destroy", None, [C.c_void_p]),
+    ("as_device_info", C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
+    ("as_device_memory", C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("as_last_error", C.c_char_p, []),
+    ("as_version", C.c_char_p, []),
+]
+
+
+def load():
+    """Loads lib/libastrosim.so (built by ``python -m astrosimgpu_cuda_amd.build``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not path.exists():
+        raise LibraryNotBuilt(f"{path} is missing: run `python -m astrosimgpu_cuda_amd.build` first")
+    lib = C.CDLL(str(path))
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise AstrosimError(status, load().as_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def device_info() -> tuple[str, str]:
+    lib = load()
+    name = C.create_string_buffer(256)
+    arch = C.create_string_buffer(64)
+    _check(lib.as_device_info(name, 256, arch, 64))
+    return name.value.decode(), arch.value.decode()
+
+
+def device_memory() -> tuple[int, int]:
+    lib = load()
+    f, t = C.c_uint64(), C.c_uint64()
+    _check(lib.as_device_memory(C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
+class Network:
+    """Connectivity and per-cell parameters for ``(n_total, seed)`` (host only).
+
+    Reference: ScalingConfig::make (src/astrosim_fused.cu:606-614),
+    build_connectivity (:631-735), population draws (:917-933).
+    """
+
+    CSR_NAMES = {"exc": 0, "inh": 1, "sic": 2, "e2a": 3}
+    PARAM_NAMES = {"Ca_tot": 0, "IP3_0": 1, "tau_IP3": 2, "delta_IP3": 3, "V_reset": 4, "b": 5}
+
+    def __init__(self, n_total: int, seed: int = 1):
+        lib = load()
+        h = C.c_void_p()
+        _check(lib.as_network_build(C.c_uint32(n_total), C.c_uint64(seed), C.byref(h)))
+        self._h = h
+        info = _NetInfo()
+        _check(lib.as_network_info_get(h, C.byref(info)))
+        self.n_total, self.seed = n_total, seed
+        self.n_exc, self.n_inh, self.n_astro = info.n_exc, info.n_inh, info.n_astro
+        self.n_neurons = info.n_neurons
+        self.p_primary = info.p_primary
+        self.n_edges = {"exc": info.n_exc_edges, "inh": info.n_inh_edges,
+                        "e2a": info.n_e2a_edges, "sic": info.n_sic_edges}
+        self.build_seconds = info.build_seconds
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("network was freed")
+        return self._h
+
+    def csr(self, which: str) -> tuple[np.ndarray, np.ndarray]:
+        idx = self.CSR_NAMES[which]
+        n_t = self.n_astro if which == "e2a" else self.n_neurons
+        rs = np.empty(n_t + 1, dtype=np.uint32)
+        src = np.empty(self.n_edges[which], dtype=np.uint32)
+        _check(load().as_network_csr(self.handle, idx, _ptr(rs), _ptr(src) if src.size else None))
+        return rs, src
+
+    def params(self, which: str) -> np.ndarray:
+        idx = self.PARAM_NAMES[which]
+        out = np.empty(self.n_astro if idx < 4 else self.n_neurons, dtype=np.float64)
+        _check(load().as_network_params(self.handle, idx, _ptr(out)))
+        return out
+
+    def dump_edges(self, path: str | os.PathLike) -> None:
+        _check(load().as_network_dump_edges(self.handle, str(path).encode()))
+
+    def free(self) -> None:
+        if self._h is not None:
+            load().as_network_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def make_config(dt=0.1, substeps=1, block=128, w_n2a=None, w_a2n=None, stp=True,
+                precision="fp64", method="plain") -> _Config:
+    cfg = _Config()
+    load().as_config_default(C.byref(cfg))
+    cfg.dt = dt
+    cfg.substeps = substeps
+    cfg.block = block
+    cfg.w_n2a = math.nan if w_n2a is None else w_n2a
+    cfg.w_a2n = math.nan if w_a2n is None else w_a2n
+    cfg.stp_enabled = 1 if stp else 0
+    cfg.precision = {"fp64": FP64, "fp32c": FP32C}[precision] if isinstance(precision, str) else precision
+    cfg.method = {"plain": PLAIN, "blocked": BLOCKED}[method] if isinstance(method, str) else method
+    return cfg
+
+
+class Simulation:
+    """Device state and time loop of one network on the GPU.
+
+    Reference: device bring-up and upload (src/astrosim_fused.cu:904-1018),
+    the step loop (:1042-1064), diagnostics (:1066-1125).
+    """
+
+    def __init__(self, network: Network, **kw):
+        lib = load()
+        self.cfg = make_config(**kw)
+        self.n_neurons, self.n_astro, self.n_exc = network.n_neurons, network.n_astro, network.n_exc
+        h = C.c_void_p()
+        _check(lib.as_sim_create(network.handle, C.byref(self.cfg), C.byref(h)))
+        self._h = h
+
+    @staticmethod
+    def footprint(network: Network, **kw) -> int:
+        cfg = make_config(**kw)
+        out = C.c_uint64()
+        _check(load().as_sim_footprint(network.handle, C.byref(cfg), C.byref(out)))
+        return out.value
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("simulation was destroyed")
+        return self._h
+
+    def step(self, k: int = 1) -> None:
+        _check(load().as_sim_step(self.handle, C.c_int64(k)))
+
+    def sync(self) -> None:
+        _check(load().as_sim_sync(self.handle))
+
+    @property
+    def steps(self) -> int:
+        return load().as_sim_steps(self.handle)
+
+    def spike_count(self) -> int:
+        out = C.c_uint64()
+        _check(load().as_sim_spike_count(self.handle, C.byref(out)))
+        return out.value
+
+    def reset_ever_active(self) -> None:
+        _check(load().as_sim_reset_ever_active(self.handle))
+
+    def state(self, field: str) -> np.ndarray:
+        n = self.n_astro if field in ASTRO_FIELDS else self.n_neurons
+        out = np.empty(n, dtype=np.float64)
+        _check(load().as_sim_read_state(self.handle, FIELDS[field], _ptr(out)))
+        return out
+
+    def refractory(self) -> np.ndarray:
+        out = np.empty(self.n_neurons, dtype=np.int32)
+        _check(load().as_sim_read_refractory(self.handle, _ptr(out)))
+        return out
+
+    def ever_active(self) -> np.ndarray:
+        out = np.empty(self.n_astro, dtype=np.uint8)
+        _check(load().as_sim_read_ever_active(self.handle, _ptr(out)))
+        return out
+
+    def raster(self, first_step: int, n_steps: int) -> np.ndarray:
+        out = np.empty((n_steps, self.n_neurons), dtype=np.uint8)
+        _check(load().as_sim_read_raster(self.handle, C.c_int64(first_step), C.c_int32(n_steps), _ptr(out)))
+        return out
+
+    def device_bytes(self) -> int:
+        return load().as_sim_bytes(self.handle)
+
+    def set_timing(self, on: bool) -> None:
+        _check(load().as_sim_set_timing(self.handle, 1 if on else 0))
+
+    def kernel_time(self) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_int64()
+        _check(load().as_sim_kernel_time(self.handle, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def close(self) -> None:
+        if self._h is not None:
+            load().as_sim_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_cli(args: list[str], timeout: float | None = None) -> subprocess.CompletedProcess:
+    """Runs the ``astrosim_fused`` drop-in binary with the reference's flags."""
+    exe = cli_path()
+    if not exe.exists():
+        raise LibraryNotBuilt(f"{exe} is missing: run `python -m astrosimgpu_cuda_amd.build` first")
+    return subprocess.run([str(exe), *args], capture_output=True, text=True, timeout=timeout)

@@ -1,0 +1,280 @@
+// as_capi.cpp -- the extern "C" boundary (include/astrosim.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/astrosim.h"
+#include "as_engine.h"
+#include "as_setup.h"
+
+struct as_network {
+    astrosim::Network net;
+};
+
+struct as_sim {
+    std::unique_ptr<astrosim::Engine> eng;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+as_status guarded(F&& f) {
+    try {
+        f();
+        return AS_OK;
+    } catch (const astrosim::Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return AS_ERR_NOMEM;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return AS_ERR_INVALID;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return AS_ERR_INVALID;
+    }
+}
+
+void require(bool ok, const char* msg) {
+    if (!ok) throw astrosim::Error(AS_ERR_INVALID, msg);
+}
+
+void require_device() {
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        throw astrosim::Error(AS_ERR_NODEVICE, "No GPU device visible.");
+    }
+}
+
+std::unique_ptr<astrosim::Engine> make_engine(const astrosim::Network& net, const astrosim::RunParams& rp,
+                                              hipStream_t s) {
+    if (rp.precision == AS_FP64 && rp.method == AS_METHOD_PLAIN) return astrosim::make_plain_engine(net, rp, s);
+    throw astrosim::Error(AS_ERR_INVALID, "this precision/method combination is not available");
+}
+
+}  // namespace
+
+extern "C" {
+
+void as_config_default(as_config* cfg) {
+    if (!cfg) return;
+    cfg->dt = 0.1;
+    cfg->substeps = 1;
+    cfg->block = 128;
+    cfg->w_n2a = std::nan("");
+    cfg->w_a2n = std::nan("");
+    cfg->stp_enabled = 1;
+    cfg->precision = AS_FP64;
+    cfg->method = AS_METHOD_PLAIN;
+}
+
+as_status as_network_build(uint32_t n_total, uint64_t seed, as_network** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        *out = nullptr;
+        std::unique_ptr<as_network> n(new as_network);
+        astrosim::build_network(n_total, seed, n->net);
+        *out = n.release();
+    });
+}
+
+as_status as_network_info_get(const as_network* net, as_network_info* info) {
+    return guarded([&] {
+        require(net && info, "NULL argument");
+        const auto& c = net->net.cfg;
+        info->n_total = c.n_total;
+        info->n_exc = c.n_exc;
+        info->n_inh = c.n_inh;
+        info->n_astro = c.n_astro;
+        info->n_neurons = c.n_neurons();
+        info->p_primary = c.p_primary;
+        info->n_exc_edges = net->net.inc_exc.n_edges();
+        info->n_inh_edges = net->net.inc_inh.n_edges();
+        info->n_e2a_edges = net->net.inc_e2a.n_edges();
+        info->n_sic_edges = net->net.inc_sic.n_edges();
+        info->build_seconds = net->net.build_seconds;
+    });
+}
+
+as_status as_network_csr(const as_network* net, int which, uint32_t* row_start, uint32_t* src) {
+    return guarded([&] {
+        require(net != nullptr, "net is NULL");
+        require(which >= 0 && which < 4, "which must be 0..3");
+        const astrosim::HostCSR* c[4] = {&net->net.inc_exc, &net->net.inc_inh, &net->net.inc_sic,
+                                         &net->net.inc_e2a};
+        const auto& csr = *c[which];
+        if (row_start) std::memcpy(row_start, csr.row_start.data(), csr.row_start.size() * sizeof(uint32_t));
+        if (src && !csr.src.empty()) std::memcpy(src, csr.src.data(), csr.src.size() * sizeof(uint32_t));
+    });
+}
+
+as_status as_network_params(const as_network* net, int which, double* out) {
+    return guarded([&] {
+        require(net && out, "NULL argument");
+        require(which >= 0 && which < 6, "which must be 0..5");
+        const std::vector<double>* v[6] = {&net->net.Ca_tot, &net->net.IP3_0, &net->net.tau_IP3,
+                                           &net->net.delta_IP3, &net->net.V_reset, &net->net.b};
+        std::memcpy(out, v[which]->data(), v[which]->size() * sizeof(double));
+    });
+}
+
+as_status as_network_dump_edges(const as_network* net, const char* path) {
+    return guarded([&] {
+        require(net && path, "NULL argument");
+        if (!astrosim::dump_edges_csv(net->net, path))
+            throw astrosim::Error(AS_ERR_IO, std::string("cannot open ") + path + " for --dump-edges");
+    });
+}
+
+void as_network_free(as_network* net) { delete net; }
+
+as_status as_sim_footprint(const as_network* net, const as_config* cfg, uint64_t* bytes) {
+    return guarded([&] {
+        require(net && cfg && bytes, "NULL argument");
+        const astrosim::RunParams rp = astrosim::resolve_params(*cfg);
+        *bytes = astrosim::plain_footprint(net->net, rp);
+    });
+}
+
+as_status as_sim_create(const as_network* net, const as_config* cfg, as_sim** out) {
+    return guarded([&] {
+        require(net && cfg && out, "NULL argument");
+        *out = nullptr;
+        const astrosim::RunParams rp = astrosim::resolve_params(*cfg);
+        require_device();
+        std::unique_ptr<as_sim> s(new as_sim);
+        AS_HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        try {
+            s->eng = make_engine(net->net, rp, s->stream);
+        } catch (...) {
+            (void)hipStreamDestroy(s->stream);
+            s->stream = nullptr;
+            throw;
+        }
+        *out = s.release();
+    });
+}
+
+as_status as_sim_step(as_sim* sim, int64_t k) {
+    return guarded([&] {
+        require(sim != nullptr, "sim is NULL");
+        require(k >= 0, "k must be >= 0");
+        sim->eng->enqueue_steps(k);
+    });
+}
+
+as_status as_sim_sync(as_sim* sim) {
+    return guarded([&] {
+        require(sim != nullptr, "sim is NULL");
+        AS_HIP_CHECK(hipStreamSynchronize(sim->stream));
+    });
+}
+
+int64_t as_sim_steps(const as_sim* sim) { return sim ? sim->eng->steps() : -1; }
+
+as_status as_sim_spike_count(as_sim* sim, uint64_t* out) {
+    return guarded([&] {
+        require(sim && out, "NULL argument");
+        *out = sim->eng->spike_count();
+    });
+}
+
+as_status as_sim_reset_ever_active(as_sim* sim) {
+    return guarded([&] {
+        require(sim != nullptr, "sim is NULL");
+        sim->eng->reset_ever_active();
+    });
+}
+
+as_status as_sim_read_state(as_sim* sim, int field, double* out) {
+    return guarded([&] {
+        require(sim && out, "NULL argument");
+        sim->eng->read_state(field, out);
+    });
+}
+
+as_status as_sim_read_refractory(as_sim* sim, int32_t* out) {
+    return guarded([&] {
+        require(sim && out, "NULL argument");
+        sim->eng->read_refractory(out);
+    });
+}
+
+as_status as_sim_read_ever_active(as_sim* sim, uint8_t* out) {
+    return guarded([&] {
+        require(sim && out, "NULL argument");
+        sim->eng->read_ever_active(out);
+    });
+}
+
+as_status as_sim_read_raster(as_sim* sim, int64_t first_step, int32_t n_steps, uint8_t* out) {
+    return guarded([&] {
+        require(sim && out, "NULL argument");
+        sim->eng->read_raster(first_step, n_steps, out);
+    });
+}
+
+uint64_t as_sim_bytes(const as_sim* sim) { return sim ? sim->eng->device_bytes() : 0; }
+
+as_status as_sim_set_timing(as_sim* sim, int enable) {
+    return guarded([&] {
+        require(sim != nullptr, "sim is NULL");
+        sim->eng->set_timing(enable != 0);
+    });
+}
+
+as_status as_sim_kernel_time(as_sim* sim, double* total_ms, int64_t* launches) {
+    return guarded([&] {
+        require(sim && total_ms && launches, "NULL argument");
+        sim->eng->kernel_time(total_ms, launches);
+    });
+}
+
+void as_sim_destroy(as_sim* sim) {
+    if (!sim) return;
+    sim->eng.reset();
+    if (sim->stream) (void)hipStreamDestroy(sim->stream);
+    delete sim;
+}
+
+as_status as_device_info(char* name, int name_len, char* arch, int arch_len) {
+    return guarded([&] {
+        require_device();
+        hipDeviceProp_t prop{};
+        AS_HIP_CHECK(hipGetDeviceProperties(&prop, 0));
+        if (name && name_len > 0) {
+            std::strncpy(name, prop.name, name_len - 1);
+            name[name_len - 1] = '\0';
+        }
+        if (arch && arch_len > 0) {
+            std::strncpy(arch, prop.gcnArchName, arch_len - 1);
+            arch[arch_len - 1] = '\0';
+        }
+    });
+}
+
+as_status as_device_memory(uint64_t* free_bytes, uint64_t* total_bytes) {
+    return guarded([&] {
+        require_device();
+        size_t f = 0, t = 0;
+        AS_HIP_CHECK(hipMemGetInfo(&f, &t));
+        if (free_bytes) *free_bytes = f;
+        if (total_bytes) *total_bytes = t;
+    });
+}
+
+const char* as_last_error(void) { return g_last_error.c_str(); }
+
+const char* as_version(void) { return "astrosim-mi355x 0.1.0 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,177 @@
+/* as_dmath.h -- deterministic double-precision exp, log and cos.
+ *
+ * The simulator's FP64 path must give the same bits on the host and on a
+ * gfx950 GPU. Vendor math libraries (glibc on the host, the ROCm device
+ * library on the GPU) each round their results a little differently, and in
+ * a spiking network a one-ulp difference in one exp() eventually moves a
+ * spike. This header therefore supplies the three transcendentals the
+ * per-step arithmetic needs, written only with IEEE-754 +, -, *, / (each
+ * correctly rounded on both sides), exact bit manipulation and floor(), so
+ * that any build with floating-point contraction disabled evaluates them to
+ * identical bits on every target.
+ *
+ * Methods (textbook, from the mathematics):
+ *   exp : Cody-Waite reduction x = k*ln2 + r, |r| <= ln2/2, with ln2 split
+ *         into a 32-bit head and a tail; exp(r) by its Taylor series to
+ *         degree 13 (truncation < 2^-57); 2^k applied as two exact scalings.
+ *   log : x = 2^k * m with m in [sqrt(2)/2, sqrt(2)], f = m - 1,
+ *         s = f/(2+f); log(m) = 2*atanh(s) = 2s + s*R(s^2) with the atanh
+ *         series to s^23 (truncation < 2^-60), evaluated in the
+ *         f - (f^2/2 - s*(f^2/2 + R)) arrangement that keeps the leading
+ *         term exact.
+ *   cos : Cody-Waite reduction by pi/2 with a three-part constant (33+33+53
+ *         bits), then the Taylor series of cos or sin on |r| <= pi/4
+ *         (truncation < 2^-57). Full accuracy for |x| < 2^20; the simulator
+ *         only calls it on [0, 2*pi).
+ * The split constants were computed with 80-digit decimal arithmetic.
+ * Accuracy against glibc is checked by tests/test_dmath.py (max error a
+ * small number of ulps); equality of host and device bits by the GPU tests.
+ *
+ * The header is plain C99 as well as HIP C++, so the host oracle and the
+ * device kernels include the very same definitions.
+ */
+#ifndef AS_DMATH_H
+#define AS_DMATH_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define AS_DM_FN static inline __host__ __device__
+#else
+#define AS_DM_FN static inline
+#endif
+
+AS_DM_FN uint64_t as_dm_bits(double x) {
+    uint64_t u;
+    __builtin_memcpy(&u, &x, sizeof u);
+    return u;
+}
+
+AS_DM_FN double as_dm_double(uint64_t u) {
+    double x;
+    __builtin_memcpy(&x, &u, sizeof x);
+    return x;
+}
+
+/* 2^k as a double for -1022 <= k <= 1023. */
+AS_DM_FN double as_dm_pow2(int k) {
+    return as_dm_double((uint64_t)(k + 1023) << 52);
+}
+
+#define AS_DM_LN2_HI 0x1.62e42fee00000p-1  /* ln2 truncated to 32 bits */
+#define AS_DM_LN2_LO 0x1.a39ef35793c76p-33 /* ln2 - LN2_HI, rounded */
+#define AS_DM_INV_LN2 0x1.71547652b82fep+0
+#define AS_DM_PIO2_1 0x1.921fb54400000p+0  /* pi/2, first 33 bits */
+#define AS_DM_PIO2_2 0x1.0b4611a600000p-34 /* next 33 bits */
+#define AS_DM_PIO2_3 0x1.3198a2e037073p-69 /* remainder, rounded */
+#define AS_DM_TWO_OVER_PI 0x1.45f306dc9c883p-1
+#define AS_DM_SQRT2 0x1.6a09e667f3bcdp+0
+
+AS_DM_FN double as_exp(double x) {
+    if (x != x) return x;
+    if (x > 709.782712893384) return as_dm_double(0x7ff0000000000000ULL);
+    if (x < -745.2) return 0.0;
+    const double kd = __builtin_floor(x * AS_DM_INV_LN2 + 0.5);
+    const double r = (x - kd * AS_DM_LN2_HI) - kd * AS_DM_LN2_LO;
+    /* exp(r) = 1 + r + r^2 * q(r), q = sum_{n>=2} r^(n-2)/n! */
+    double q = 1.0 / 6227020800.0;           /* 1/13! */
+    q = 1.0 / 479001600.0 + r * q;            /* 1/12! */
+    q = 1.0 / 39916800.0 + r * q;             /* 1/11! */
+    q = 1.0 / 3628800.0 + r * q;              /* 1/10! */
+    q = 1.0 / 362880.0 + r * q;               /* 1/9!  */
+    q = 1.0 / 40320.0 + r * q;                /* 1/8!  */
+    q = 1.0 / 5040.0 + r * q;                 /* 1/7!  */
+    q = 1.0 / 720.0 + r * q;                  /* 1/6!  */
+    q = 1.0 / 120.0 + r * q;                  /* 1/5!  */
+    q = 1.0 / 24.0 + r * q;                   /* 1/4!  */
+    q = 1.0 / 6.0 + r * q;                    /* 1/3!  */
+    q = 0.5 + r * q;                          /* 1/2!  */
+    const double p = 1.0 + (r + (r * r) * q);
+    const int k = (int)kd;
+    const int k1 = k / 2;
+    const int k2 = k - k1;
+    return (p * as_dm_pow2(k1)) * as_dm_pow2(k2);
+}
+
+AS_DM_FN double as_log(double x) {
+    if (x != x) return x;
+    if (x < 0.0) return as_dm_double(0x7ff8000000000000ULL);
+    if (x == 0.0) return as_dm_double(0xfff0000000000000ULL);
+    uint64_t u = as_dm_bits(x);
+    if (u >= 0x7ff0000000000000ULL) return x; /* +inf */
+    int k = 0;
+    if (u < 0x0010000000000000ULL) { /* subnormal: scale by 2^54 */
+        x *= 0x1p54;
+        u = as_dm_bits(x);
+        k = -54;
+    }
+    k += (int)(u >> 52) - 1023;
+    double m = as_dm_double((u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
+    if (m > AS_DM_SQRT2) {
+        m *= 0.5;
+        k += 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    /* R = sum_{n=1..11} 2 z^n / (2n+1) */
+    double R = 2.0 / 23.0;
+    R = 2.0 / 21.0 + z * R;
+    R = 2.0 / 19.0 + z * R;
+    R = 2.0 / 17.0 + z * R;
+    R = 2.0 / 15.0 + z * R;
+    R = 2.0 / 13.0 + z * R;
+    R = 2.0 / 11.0 + z * R;
+    R = 2.0 / 9.0 + z * R;
+    R = 2.0 / 7.0 + z * R;
+    R = 2.0 / 5.0 + z * R;
+    R = 2.0 / 3.0 + z * R;
+    R = z * R;
+    const double hfsq = 0.5 * f * f;
+    const double kd = (double)k;
+    return kd * AS_DM_LN2_HI - ((hfsq - (s * (hfsq + R) + kd * AS_DM_LN2_LO)) - f);
+}
+
+/* cos(r) on |r| <= pi/4 (+ a few ulps). */
+AS_DM_FN double as_dm_cos_kernel(double r) {
+    const double z = r * r;
+    double c = 1.0 / 20922789888000.0;       /* +1/16! */
+    c = -1.0 / 87178291200.0 + z * c;         /* -1/14! */
+    c = 1.0 / 479001600.0 + z * c;            /* +1/12! */
+    c = -1.0 / 3628800.0 + z * c;             /* -1/10! */
+    c = 1.0 / 40320.0 + z * c;                /* +1/8!  */
+    c = -1.0 / 720.0 + z * c;                 /* -1/6!  */
+    c = 1.0 / 24.0 + z * c;                   /* +1/4!  */
+    const double hz = 0.5 * z;
+    const double w = 1.0 - hz;
+    return w + (((1.0 - w) - hz) + z * (z * c));
+}
+
+/* sin(r) on |r| <= pi/4 (+ a few ulps). */
+AS_DM_FN double as_dm_sin_kernel(double r) {
+    const double z = r * r;
+    double s = 1.0 / 355687428096000.0;      /* +1/17! */
+    s = -1.0 / 1307674368000.0 + z * s;       /* -1/15! */
+    s = 1.0 / 6227020800.0 + z * s;           /* +1/13! */
+    s = -1.0 / 39916800.0 + z * s;            /* -1/11! */
+    s = 1.0 / 362880.0 + z * s;               /* +1/9!  */
+    s = -1.0 / 5040.0 + z * s;                /* -1/7!  */
+    s = 1.0 / 120.0 + z * s;                  /* +1/5!  */
+    s = -1.0 / 6.0 + z * s;                   /* -1/3!  */
+    return r + (r * z) * s;
+}
+
+AS_DM_FN double as_cos(double x) {
+    if (x != x) return x;
+    const double ax = x < 0.0 ? -x : x;
+    if (!(ax < 0x1p30)) return as_dm_double(0x7ff8000000000000ULL); /* outside supported range */
+    const double nd = __builtin_floor(x * AS_DM_TWO_OVER_PI + 0.5);
+    const double r = ((x - nd * AS_DM_PIO2_1) - nd * AS_DM_PIO2_2) - nd * AS_DM_PIO2_3;
+    const int n = ((int)nd) & 3;
+    if (n == 0) return as_dm_cos_kernel(r);
+    if (n == 1) return -as_dm_sin_kernel(r);
+    if (n == 2) return -as_dm_cos_kernel(r);
+    return as_dm_sin_kernel(r);
+}
+
+#endif /* AS_DMATH_H */

@@ -1,0 +1,86 @@
+// as_rng.h -- splitmix64 counter RNG shared by host setup and device kernels.
+//
+// Every random number in a run is a pure function of (seed, stream, counter):
+// no generator state lives on the device, so any cell can draw its numbers
+// for any step independently and in any order, and the host and the GPU draw
+// identical bits. Transcendentals come from as_dmath.h so the floating-point
+// draws (normal, Poisson) are also identical on host and device.
+#pragma once
+
+#include <cstdint>
+
+#include "as_dmath.h"
+
+#if defined(__HIPCC__)
+#define AS_HD __host__ __device__ __forceinline__
+#else
+#define AS_HD inline
+#endif
+
+namespace astrosim {
+
+// Reference: src/astrosim_fused.cu:127-135. Weyl increment of the stream and
+// counter into the seed, then the splitmix64 output finaliser.
+AS_HD std::uint64_t rng_bits(std::uint64_t seed, std::uint64_t stream, std::uint64_t counter) {
+    std::uint64_t z = seed + stream * 0x9E3779B97F4A7C15ULL + counter * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// Reference: src/astrosim_fused.cu:137-141. Top 53 bits scaled to [0, 1);
+// exact in double precision.
+AS_HD double rng_uniform(std::uint64_t seed, std::uint64_t stream, std::uint64_t counter) {
+    return static_cast<double>(rng_bits(seed, stream, counter) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Box-Muller, cosine branch, on counters 0 and 1 of the stream.
+// Reference: src/astrosim_fused.cu:143-149 (u1 clamped at 1e-300).
+AS_HD double rng_normal(std::uint64_t seed, std::uint64_t stream) {
+    double u1 = rng_uniform(seed, stream, 0);
+    if (u1 <= 1e-300) u1 = 1e-300;
+    const double u2 = rng_uniform(seed, stream, 1);
+    return __builtin_sqrt(-2.0 * as_log(u1)) * as_cos(6.283185307179586476925286766559 * u2);
+}
+
+// Poisson count for a fresh stream. Knuth's product-of-uniforms method for
+// lambda <= 30 (counters 0, 1, 2, ...), a rounded normal approximation above.
+// `limit` must equal as_exp(-lambda); callers hoist it because lambda is a
+// per-population constant. Reference: src/astrosim_fused.cu:154-175.
+AS_HD int rng_poisson(std::uint64_t seed, std::uint64_t stream, double lambda, double limit) {
+    if (lambda <= 0.0) return 0;
+    if (lambda > 30.0) {
+        double u1 = rng_uniform(seed, stream, 0);
+        if (u1 <= 1e-300) u1 = 1e-300;
+        const double u2 = rng_uniform(seed, stream, 1);
+        const double z =
+            __builtin_sqrt(-2.0 * as_log(u1)) * as_cos(6.283185307179586476925286766559 * u2);
+        const double v = lambda + __builtin_sqrt(lambda) * z;
+        return v < 0.0 ? 0 : static_cast<int>(v + 0.5);
+    }
+    std::uint64_t c = 0;
+    double product = rng_uniform(seed, stream, c++);
+    int count = 0;
+    while (product > limit) {
+        ++count;
+        product *= rng_uniform(seed, stream, c++);
+    }
+    return count;
+}
+
+// Stream-key constants XOR-ed into the run seed, one per draw site.
+// Reference: src/astrosim_fused.cu:395 (astro Poisson), :404 (astro noise),
+// :476 (neuron Poisson), :499 (neuron noise), :634/:917 (host setup stream).
+constexpr std::uint64_t kKeyAstroPoisson = 0xC2B2AE3D27D4EB4FULL;
+constexpr std::uint64_t kKeyAstroNoise = 0x9E3779B97F4A7C15ULL;
+constexpr std::uint64_t kKeyNeuronPoisson = 0x2545F4914F6CDD1DULL;
+constexpr std::uint64_t kKeyNeuronNoise = 0x9E3779B185EBCA87ULL;
+constexpr std::uint64_t kSetupStream = 0x5EEDULL;
+// Per-step stream ids are step*kStepStride + cell (Poisson and astrocyte
+// noise) and noise_idx*kNoiseStride + neuron (neuron noise). They wrap in
+// 64 bits and alias across steps for cell >= kStepStride; that is part of
+// the draw contract. Reference: src/astrosim_fused.cu:396, :405, :477, :500.
+constexpr std::uint64_t kStepStride = 1000003ULL;
+constexpr std::uint64_t kNoiseStride = 1000033ULL;
+
+}  // namespace astrosim

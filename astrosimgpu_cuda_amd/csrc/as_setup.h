@@ -1,0 +1,72 @@
+// as_setup.h -- host-side network construction: sequential setup RNG,
+// randomised per-cell parameters, tripartite connectivity and its inversion
+// into per-target CSR.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "as_model.h"
+
+namespace astrosim {
+
+// Stateful sequential draws on one (seed, stream), with a cached Box-Muller
+// spare. Uses the host C library's log/sqrt/sin/cos exactly as the reference
+// setup does. Reference: src/astrosim_fused.cu:180-208.
+class HostRng {
+public:
+    HostRng(std::uint64_t seed, std::uint64_t stream) : seed_(seed), stream_(stream) {}
+    double uniform() { return rng_uniform(seed_, stream_, counter_++); }
+    double normal();
+    double normal_redraw(double mean, double stddev, double lo, double hi, int max_tries = 1000);
+    std::uint64_t counter() const { return counter_; }
+
+private:
+    std::uint64_t seed_, stream_, counter_ = 0;
+    bool has_spare_ = false;
+    double spare_ = 0.0;
+};
+
+// mean * N(1, var) restricted to [lower, upper] * mean (bounds swapped for a
+// negative mean). Reference: src/astrosim_fused.cu:210-216.
+double draw_scaled(double mean, bool enabled, double lower, double upper, double var, HostRng& rng);
+
+// Gap to the next Bernoulli(p) success among `remaining` candidates.
+// Reference: src/astrosim_fused.cu:617-624. `log1m_p` = log1p(-p).
+index_t geometric_skip(double p, double log1m_p, index_t remaining, HostRng& rng);
+
+// Connectivity inverted by target: row t lists the sources of target t in
+// generation order. Reference: src/astrosim_fused.cu:296-315.
+struct HostCSR {
+    std::vector<index_t> row_start;  // n_targets + 1
+    std::vector<index_t> src;        // row_start[n_targets]
+    index_t n_targets() const { return row_start.empty() ? 0 : index_t(row_start.size() - 1); }
+    index_t n_edges() const { return index_t(src.size()); }
+};
+
+// Everything a simulation needs that depends only on (N_total, seed).
+struct Network {
+    ScalingConfig cfg;
+    std::uint64_t seed = 0;
+    HostCSR inc_exc;  // per neuron: excitatory sources (global ids < n_exc)
+    HostCSR inc_inh;  // per neuron: inhibitory sources (LOCAL ids in [0, n_inh))
+    HostCSR inc_sic;  // per neuron: astrocyte sources
+    HostCSR inc_e2a;  // per astrocyte: excitatory neuron sources
+    std::vector<double> Ca_tot, IP3_0, tau_IP3, delta_IP3;  // per astrocyte
+    std::vector<double> V_reset, b;                          // per neuron
+    double build_seconds = 0.0;  // connectivity generation + inversion
+};
+
+// Builds the network for (n_total, seed). Throws std::invalid_argument for
+// sizes the 32-bit edge ids cannot hold.
+void build_network(index_t n_total, std::uint64_t seed, Network& net);
+
+// Writes the edge list as CSV (type,source,target), astrocytes prefixed 'A',
+// inhibitory sources shifted to global ids. Reference: src/astrosim_fused.cu:873-902.
+bool dump_edges_csv(const Network& net, const std::string& path);
+
+// Largest N_total accepted: keeps every CSR's edge count far below 2^32.
+constexpr index_t kMaxNTotal = 40000000u;
+
+}  // namespace astrosim

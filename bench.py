@@ -1,0 +1,171 @@
+"""Headline benchmark: simulation steps per second of the astrosim_fused step
+pipeline on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--seed S]
+                    [--legs fp64:plain,...] [--no-cpu-baseline]
+
+Workload (default): the full-scale single network, N_total = 10,000,000
+(8M excitatory + 2M inhibitory neurons, 2M astrocytes, ~1.32G int32 in-edges),
+seed 42, dt 0.1 ms. A "step" is one simulated 0.1 ms step of every cell. The
+network is generated on the host and resident in HBM before timing starts;
+W untimed steps, then exactly K timed steps bracketed by device syncs.
+
+One JSON line is printed. `value` is the headline leg's steps/s; every leg's
+figures are in `legs`. `roofline` is for the headline leg's step kernel:
+algorithmic bytes per launch (the reference-layout count, with this network's
+exact edge counts) over the average launch time measured with HIP events on
+the simulation's stream. `cpu_baseline` times the FP64 host oracle's stepper
+on this same network on the host cores for a bounded number of steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_step(net, prec: str) -> float:
+    """Bytes one step must move in the reference data layout, each touched
+    once (dense FP state, one delayed ring slot read per population, every
+    CSR id read once). Neuron: 12s state r/w + 2s I_sic + s ring write + s
+    slot read + 8 refractory + 12 row_start + 4*ids; astrocyte: 6s state +
+    4s params + s ring write + s slot read + 4 row_start + 4*ids."""
+    s = 8 if prec == "fp64" else 4
+    n, na = net.n_neurons, net.n_astro
+    ids = sum(net.n_edges.values())
+    neuron = n * (12 * s + 2 * s + s + s + 8 + 12)
+    astro = na * (6 * s + 4 * s + s + s + 4)
+    return float(neuron + astro + 4 * ids)
+
+
+def run_leg(A, net, prec: str, method: str, steps: int, warmup: int) -> dict:
+    log(f"leg {prec}/{method}: create")
+    sim = A.Simulation(net, precision=prec, method=method)
+    sim.step(warmup)
+    sim.sync()
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    kms, launches = sim.kernel_time()
+    sim.set_timing(False)
+    spikes = sim.spike_count()
+    out = {
+        "precision": prec, "method": method, "steps": steps, "wall_s": wall,
+        "steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
+        "kernel_ms_total": kms, "launches": launches,
+        "kernel_ms_per_launch": kms / max(1, launches),
+        "steps_per_launch": steps / max(1, launches),
+        "device_bytes": sim.device_bytes(), "spikes_total": spikes,
+    }
+    sim.close()
+    log(f"leg {prec}/{method}: {out['steps_per_s']:.1f} steps/s, {out['kernel_ms_per_launch']:.3f} ms/launch")
+    return out
+
+
+def cpu_baseline(net, seed: int, n_steps: int) -> dict:
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    csrs = {w: net.csr(w) for w in ("exc", "inh", "sic", "e2a")}
+    params = {w: net.params(w) for w in ("Ca_tot", "IP3_0", "tau_IP3", "delta_IP3", "V_reset", "b")}
+    onet = O.OracleNet.from_arrays(net.n_total, seed, csrs, params)
+    del csrs, params
+    threads = min(os.cpu_count() or 1, 16)
+    osim = O.OracleSim(onet, threads=threads)
+    osim.step(1)  # first touch of the state arrays
+    t0 = time.perf_counter()
+    osim.step(n_steps)
+    wall = time.perf_counter() - t0
+    return {"value": n_steps / wall, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 host oracle stepper, {n_steps} steps of the same N_total={net.n_total} "
+                      f"network (after 1 warm step), OpenMP over cells"}
+
+
+def load_traffic(workload: str) -> float | None:
+    p = ROOT / "profiles" / f"traffic_{workload}.json"
+    if p.exists():
+        try:
+            return float(json.loads(p.read_text())["hbm_bytes_per_launch"])
+        except Exception:
+            return None
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--legs", default="fp64:plain")
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
+              "multi-GPU sharding is a design note, not built", file=sys.stderr)
+        return 2
+
+    import astrosimgpu_cuda_amd as A
+    name, arch = A.device_info()
+    log(f"device {name} ({arch}); building N_total={args.n} seed={args.seed}")
+    t0 = time.perf_counter()
+    net = A.Network(args.n, args.seed)
+    log(f"network built in {time.perf_counter() - t0:.1f} s (connectivity {net.build_seconds:.1f} s), "
+        f"edges {net.n_edges}")
+
+    legs = [tuple(x.split(":")) for x in args.legs.split(",")]
+    results = [run_leg(A, net, p, m, args.steps, args.warmup) for p, m in legs]
+    head = results[-1]
+    workload = f"n{args.n}_seed{args.seed}"
+    bpl = algorithmic_bytes_per_step(net, head["precision"]) * head["steps_per_launch"]
+    achieved = bpl / (head["kernel_ms_per_launch"] * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline: FP64 host oracle")
+        cpu = cpu_baseline(net, args.seed, args.cpu_steps)
+        log(f"cpu baseline: {cpu['value']:.3f} steps/s on {cpu['cores']} threads")
+    line = {
+        "metric": "steps/s at N=10M FP32 vs FP64 (1 GPU)",
+        "value": head["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head["precision"] == "fp64" else "f32+f64",
+        "data": "synthetic (network generated from N_total and seed by the reference's scaling law)",
+        "config": {"workload": f"single network N_total={args.n} seed={args.seed} dt=0.1ms",
+                   "n_total": args.n, "seed": args.seed, "dt_ms": 0.1,
+                   "precision": head["precision"], "method": head["method"],
+                   "parallelism": "single GPU"},
+        "legs": results,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(workload),
+                     "algorithmic_bytes_per_launch": bpl,
+                     "kernel": "step kernel of the headline leg"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,145 @@
+/* astrosim.h -- C ABI of the MI355X-native astrosim_fused step pipeline.
+ *
+ * The reference program (src/astrosim_fused.cu) has no library API: its only
+ * interface is the process command line and the terminal summary. Its main()
+ * is split here into the calls a host program (this project's own
+ * `astrosim_fused` CLI, the ensemble runner, the server, or any FFI caller)
+ * needs, each citing the reference span it replaces. Plain C types and POD
+ * pointers only; no C++ types cross the boundary and nothing throws. Every
+ * call returns an as_status; as_last_error() gives the message of the last
+ * failure on the calling thread.
+ *
+ * Threading: one as_sim is driven by one host thread at a time. Different
+ * as_sim objects may be driven concurrently from different threads; each
+ * owns its own HIP stream.
+ */
+#ifndef ASTROSIM_H
+#define ASTROSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum as_status {
+    AS_OK = 0,
+    AS_ERR_INVALID = 1,  /* bad argument or unsupported configuration */
+    AS_ERR_HIP = 2,      /* HIP runtime failure (message names file:line) */
+    AS_ERR_NOMEM = 3,    /* device or host allocation failed */
+    AS_ERR_NODEVICE = 4, /* no GPU visible (reference: src/astrosim_fused.cu:904-912) */
+    AS_ERR_IO = 5,       /* file could not be written */
+    AS_ERR_RANGE = 6     /* requested data is not (or no longer) held */
+} as_status;
+
+typedef enum as_precision {
+    AS_FP64 = 0,  /* IEEE double throughout; bit-identical to the FP64 host oracle */
+    AS_FP32C = 1  /* FP32 state and RHS, compensated hi+lo Ca/IP3/h, FP64 near-threshold AdEx exp */
+} as_precision;
+
+typedef enum as_method {
+    AS_METHOD_PLAIN = 0,  /* one launch per step, dense delay rings (reference data flow) */
+    AS_METHOD_BLOCKED = 1 /* one launch per min-delay block, bit-packed spike history */
+} as_method;
+
+typedef struct as_network as_network;
+typedef struct as_sim as_sim;
+
+/* Run options, the reference's command-line flags (src/astrosim_fused.cu:792-851). */
+typedef struct as_config {
+    double dt;          /* --dt, ms (default 0.1) */
+    int32_t substeps;   /* --substeps (default 1) */
+    int32_t block;      /* --block, threads per workgroup: 64..1024, multiple of 64 (default 128) */
+    double w_n2a;       /* --w-n2a; NaN keeps the default 0.2 */
+    double w_a2n;       /* --w-a2n; NaN keeps the default 1.0 */
+    int32_t stp_enabled;/* 0 for --no-stp */
+    int32_t precision;  /* as_precision */
+    int32_t method;     /* as_method */
+} as_config;
+
+/* Fills the defaults above (FP64, plain method). */
+void as_config_default(as_config* cfg);
+
+/* Sizes and edge counts of a built network. */
+typedef struct as_network_info {
+    uint32_t n_total, n_exc, n_inh, n_astro, n_neurons;
+    double p_primary;
+    uint64_t n_exc_edges, n_inh_edges, n_e2a_edges, n_sic_edges;
+    double build_seconds; /* connectivity generation + inversion, as printed at src:869-871 */
+} as_network_info;
+
+/* ---- network construction (host only, no GPU needed) ---- */
+
+/* Scaling law, connectivity and per-cell parameters for (n_total, seed).
+ * Replaces src/astrosim_fused.cu:846 (ScalingConfig::make), :867
+ * (build_connectivity) and :916-933 (population draws). */
+as_status as_network_build(uint32_t n_total, uint64_t seed, as_network** out);
+as_status as_network_info_get(const as_network* net, as_network_info* info);
+/* CSR by target. which: 0 exc, 1 inh, 2 sic (astro->neuron), 3 e2a (neuron->astro).
+ * Either pointer may be NULL; row_start needs n_targets+1 entries.
+ * Replaces HostCSR (src/astrosim_fused.cu:304-315). */
+as_status as_network_csr(const as_network* net, int which, uint32_t* row_start, uint32_t* src);
+/* Per-cell parameters. which: 0 Ca_tot 1 IP3_0 2 tau_IP3 3 delta_IP3 (n_astro values),
+ * 4 V_reset 5 b (n_neurons values). Replaces src/astrosim_fused.cu:918-933. */
+as_status as_network_params(const as_network* net, int which, double* out);
+/* --dump-edges CSV writer. Replaces src/astrosim_fused.cu:873-902. */
+as_status as_network_dump_edges(const as_network* net, const char* path);
+void as_network_free(as_network* net);
+
+/* ---- simulation on the GPU ---- */
+
+/* Device memory a simulation of this network would take, in bytes
+ * (CSR + state + rings), for admission control. */
+as_status as_sim_footprint(const as_network* net, const as_config* cfg, uint64_t* bytes);
+/* Allocates and uploads connectivity, parameters and initial state; the
+ * network may be freed afterwards. Replaces src/astrosim_fused.cu:904-1018. */
+as_status as_sim_create(const as_network* net, const as_config* cfg, as_sim** out);
+/* Enqueues k steps on the simulation's stream and returns (asynchronous).
+ * Replaces one pass of the time loop body, src/astrosim_fused.cu:1042-1064. */
+as_status as_sim_step(as_sim* sim, int64_t k);
+/* Waits for all enqueued work (src/astrosim_fused.cu:1103). */
+as_status as_sim_sync(as_sim* sim);
+/* Steps completed or enqueued so far. */
+int64_t as_sim_steps(const as_sim* sim);
+/* Total spikes since creation (synchronises). Replaces the spike-counter
+ * reads at src/astrosim_fused.cu:1068, 1084, 1108. */
+as_status as_sim_spike_count(as_sim* sim, uint64_t* out);
+/* Clears every astrocyte's ever-active flag (enqueued). src/astrosim_fused.cu:1075. */
+as_status as_sim_reset_ever_active(as_sim* sim);
+/* Reads a state array (synchronises). field: 0 Ca 1 IP3 2 h (n_astro);
+ * 3 V 4 w 5 g_ex 6 dg_ex 7 g_in 8 dg_in 9 I_sic_hold 10 stp_x 11 stp_u
+ * 12 stp_t_last (n_neurons). Values are converted to double. Replaces the
+ * final D2H reads at src/astrosim_fused.cu:1111-1112. */
+as_status as_sim_read_state(as_sim* sim, int field, double* out);
+as_status as_sim_read_refractory(as_sim* sim, int32_t* out);
+/* Per-astrocyte ever-active flags (src/astrosim_fused.cu:1121-1123). */
+as_status as_sim_read_ever_active(as_sim* sim, uint8_t* out);
+/* Spike raster of steps [first_step, first_step + n_steps): n_steps*n_neurons
+ * bytes, 1 = spiked. Only the most recent steps are held (at least the
+ * minimum synaptic delay); AS_ERR_RANGE otherwise. Test and analysis aid,
+ * the reference has no counterpart. */
+as_status as_sim_read_raster(as_sim* sim, int64_t first_step, int32_t n_steps, uint8_t* out);
+/* Device bytes held by this simulation. */
+uint64_t as_sim_bytes(const as_sim* sim);
+/* Kernel timing: when enabled, every step kernel launch is bracketed by HIP
+ * events on the simulation's stream. as_sim_kernel_time returns the summed
+ * kernel milliseconds and launch count since enabling (synchronises). */
+as_status as_sim_set_timing(as_sim* sim, int enable);
+as_status as_sim_kernel_time(as_sim* sim, double* total_ms, int64_t* launches);
+void as_sim_destroy(as_sim* sim);
+
+/* ---- misc ---- */
+/* Name of device 0 and its GCN arch ("gfx950"); AS_ERR_NODEVICE without a GPU. */
+as_status as_device_info(char* name, int name_len, char* arch, int arch_len);
+/* Free and total device memory in bytes. */
+as_status as_device_memory(uint64_t* free_bytes, uint64_t* total_bytes);
+const char* as_last_error(void);
+/* Library version string. */
+const char* as_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASTROSIM_H */

@@ -1,0 +1,97 @@
+"""GPU FP64 path against the FP64 host oracle: identical spike raster step
+by step and bit-identical state (reference: src/astrosim_fused.cu:374-587,
+1042-1064). Calls go through the C ABI (libastrosim.so)."""
+import numpy as np
+import pytest
+
+import astrosimgpu_cuda_amd as A
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["Ca", "IP3", "h", "V", "w", "g_ex", "dg_ex", "g_in", "dg_in", "I_sic_hold", "stp_x",
+          "stp_u", "stp_t_last"]
+
+
+def assert_state_bitwise(sim, osim, where=""):
+    for f in FIELDS:
+        g, o = sim.state(f), osim.state(f)
+        bad = np.nonzero(g.view(np.uint64) != o.view(np.uint64))[0]
+        assert bad.size == 0, f"{where} {f}: {bad.size} cells differ, first {bad[:5]} gpu={g[bad[:3]]} oracle={o[bad[:3]]}"
+    assert np.array_equal(sim.refractory(), osim.refractory())
+    assert np.array_equal(sim.ever_active(), osim.ever_active())
+    assert sim.spike_count() == osim.spike_count()
+
+
+def run_pair(n, seed, steps, chunk=10, method="plain", oracle_kw=None, **kw):
+    print(f"[parity] n={n} seed={seed} steps={steps} method={method} {kw}", flush=True)
+    net, onet = A.Network(n, seed), O.OracleNet(n, seed)
+    sim = A.Simulation(net, method=method, **kw)
+    okw = {k: kw[k] for k in ("dt", "substeps", "w_n2a", "w_a2n", "stp") if k in kw}
+    osim = O.OracleSim(onet, threads=0, **okw)
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        sim.step(k)
+        want = osim.step(k, raster=True)
+        got = sim.raster(done, k)
+        if not np.array_equal(got, want):
+            s, j = np.argwhere(got != want)[0]
+            pytest.fail(f"raster differs at step {done + s} neuron {j}: gpu={got[s, j]} oracle={want[s, j]}")
+        done += k
+    assert_state_bitwise(sim, osim, f"after {steps} steps")
+    return sim, osim
+
+
+def test_config1_first_100ms_bit_exact():
+    # N_total=500, seed 1, dt 0.1: the first 100 ms (1000 steps)
+    run_pair(500, 1, 1000)
+
+
+def test_overrides_no_stp_bit_exact():
+    run_pair(3000, 7, 400, w_n2a=0.5, w_a2n=2.0, stp=False)
+
+
+def test_substeps_and_dt_bit_exact():
+    run_pair(1000, 3, 400, dt=0.05, substeps=2, chunk=20)
+
+
+@pytest.mark.parametrize("block", [64, 256, 1024])
+def test_block_size_does_not_change_results(block):
+    run_pair(700, 5, 200, block=block)
+
+
+@pytest.mark.parametrize("n", [1, 7, 20])
+def test_tiny_networks(n):
+    run_pair(n, 2, 300)
+
+
+def test_config1_full_run_summary_matches():
+    # pre 1000 ms + sim 5000 ms as the reference's default run
+    net, onet = A.Network(500, 1), O.OracleNet(500, 1)
+    sim, osim = A.Simulation(net), O.OracleSim(onet, threads=0)
+    sim.step(10000)
+    osim.step(10000)
+    assert sim.spike_count() == osim.spike_count()
+    sim.reset_ever_active()
+    osim.reset_ever_active()
+    sim.step(50000)
+    osim.step(50000)
+    assert_state_bitwise(sim, osim, "config 1 end")
+
+
+def test_cli_summary_matches_oracle_cli():
+    import subprocess
+    args = ["--n", "500", "--seed", "1", "--pre-ms", "200", "--sim-ms", "800"]
+    g = A.run_cli(args, timeout=300)
+    assert g.returncode == 0, g.stderr
+    o = subprocess.run([str(O.BIN), *args], capture_output=True, text=True, timeout=300)
+    pick = lambda out, key: [l for l in out.splitlines() if l.startswith(key)]
+    for key in ("spikes (recorded win)", "mean firing rate", "mean calcium", "astrocytes active",
+                "astrocytes engaged", "connectivity: exc="):
+        gl, ol = pick(g.stdout, key), pick(o.stdout, key)
+        if key.startswith("connectivity"):
+            gl, ol = [l.split(", built")[0] for l in gl], [l.split(", built")[0] for l in ol]
+        assert gl == ol, (key, gl, ol)
+    assert "windowed rate" in g.stdout
+    assert "GPU: " in g.stdout
