@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdio>
 #include <stdexcept>
+#include <future>
+#include <memory>
 #include <thread>
 
 namespace astrosim {
@@ -84,6 +86,27 @@ double HostRng::normal() {
     return r * cs;
 }
 
+void HostRng::skip_normals(std::uint64_t count) {
+    // Each pair of normal() calls draws u1 (redrawn while u1 <= 1e-300) and
+    // u2 once; only the counter matters to later uniform() draws, and the
+    // transcendentals are evaluated solely for a trailing unpaired call.
+    while (count > 0) {
+        if (has_spare_) {
+            has_spare_ = false;
+            --count;
+            continue;
+        }
+        if (count == 1) {
+            normal();
+            return;
+        }
+        while (uniform() <= 1e-300) {
+        }
+        ++counter_;  // u2
+        count -= 2;
+    }
+}
+
 double HostRng::normal_redraw(double mean, double stddev, double lo, double hi, int max_tries) {
     for (int i = 0; i < max_tries; ++i) {
         const double v = mean + stddev * normal();
@@ -110,42 +133,146 @@ index_t geometric_skip(double p, double log1m_p, index_t remaining, HostRng& rng
 
 namespace {
 
-// Edges of one generation pass kept compactly: the targets of source `s` are
+unsigned worker_count() {
+    const unsigned hc = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(hc == 0 ? 4u : hc, 16u));
+}
+
+// Runs f(t) for t in [0, n) on n threads (the calling thread takes t = 0).
+template <class F>
+void parallel_for_threads(unsigned n, F&& f) {
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < n; ++t) pool.emplace_back([&f, t] { f(t); });
+    f(0);
+    for (auto& th : pool) th.join();
+}
+
+// Geometric-skip values are a pure function of the setup stream's counter:
+// G(c) = floor(log1p(-u_c) / log1p(-p)) with u_c = rng_uniform(seed, stream, c),
+// saturated at 2^32-1 (every `remaining` fits in 32 bits, so the saturated
+// value gives the same min(G, remaining)). Worker threads fill the values
+// for the next window of counters while the sequential generator consumes
+// the current one; the generator takes a value only where the reference
+// draws a skip, so the draw sequence is exactly the reference's.
+class SkipWindows {
+public:
+    static constexpr std::uint64_t kWindow = std::uint64_t(1) << 22;
+
+    SkipWindows(std::uint64_t seed, std::uint64_t stream, double log1m_p, std::uint64_t first,
+                unsigned threads)
+        : seed_(seed), stream_(stream), log1m_p_(log1m_p), threads_(threads) {
+        cur_.resize(kWindow);
+        nxt_.resize(kWindow);
+        cur_base_ = first;
+        fill(cur_, cur_base_);
+        pending_ = std::async(std::launch::async, [this] { fill(nxt_, cur_base_ + kWindow); });
+    }
+    ~SkipWindows() {
+        if (pending_.valid()) pending_.wait();
+    }
+
+    std::uint32_t at(std::uint64_t c) {
+        if (c >= cur_base_ + kWindow) {
+            pending_.wait();
+            std::swap(cur_, nxt_);
+            cur_base_ += kWindow;
+            pending_ = std::async(std::launch::async, [this] { fill(nxt_, cur_base_ + kWindow); });
+        }
+        return cur_[c - cur_base_];
+    }
+
+private:
+    void fill(std::vector<std::uint32_t>& buf, std::uint64_t base) {
+        parallel_for_threads(threads_, [&](unsigned t) {
+            const std::uint64_t lo = kWindow * t / threads_, hi = kWindow * (t + 1) / threads_;
+            for (std::uint64_t i = lo; i < hi; ++i) {
+                double u = rng_uniform(seed_, stream_, base + i);
+                if (u >= 1.0) u = std::nextafter(1.0, 0.0);
+                const double g = std::floor(std::log1p(-u) / log1m_p_);
+                buf[i] = g >= 4294967295.0 ? 0xFFFFFFFFu : static_cast<std::uint32_t>(g);
+            }
+        });
+    }
+
+    std::uint64_t seed_, stream_;
+    double log1m_p_;
+    unsigned threads_;
+    std::vector<std::uint32_t> cur_, nxt_;
+    std::uint64_t cur_base_ = 0;
+    std::future<void> pending_;
+};
+
+// Sequential draws on the setup stream: uniforms are computed directly,
+// geometric skips come from SkipWindows. Mirrors HostRng::uniform and
+// geometric_skip (src/astrosim_fused.cu:183, 617-624) draw for draw.
+class SetupDraws {
+public:
+    SetupDraws(std::uint64_t seed, std::uint64_t counter, double p, unsigned threads)
+        : seed_(seed), counter_(counter), p_(p) {
+        if (p > 0.0 && p < 1.0) skips_.reset(new SkipWindows(seed, kSetupStream, std::log1p(-p), counter, threads));
+    }
+    double uniform() { return rng_uniform(seed_, kSetupStream, counter_++); }
+    index_t skip(index_t remaining) {
+        if (remaining == 0 || p_ >= 1.0) return 0;
+        if (p_ <= 0.0) return remaining;
+        const std::uint32_t g = skips_->at(counter_++);
+        return g >= remaining ? remaining : g;
+    }
+
+private:
+    std::uint64_t seed_, counter_;
+    double p_;
+    std::unique_ptr<SkipWindows> skips_;
+};
+
+// Target-partitioned stable inversion. Edges are visited in generation
+// order by `visit(lo, hi, emit)`, which calls emit(src, dst) for every edge
+// whose dst lies in [lo, hi); each thread owns a contiguous target range,
+// counts its in-degrees, then scatters sources in generation order, so every
+// row lists its sources exactly as the reference's counting sort does
+// (src/astrosim_fused.cu:307-314).
+template <class Visit>
+void invert_parallel(index_t n_targets, std::size_t n_edges, unsigned threads, Visit&& visit, HostCSR& out) {
+    out.row_start.assign(std::size_t(n_targets) + 1, 0);
+    out.src.assign(n_edges, 0);
+    threads = std::max(1u, std::min<unsigned>(threads, std::max<index_t>(1, n_targets / 1024)));
+    std::vector<std::uint64_t> totals(threads, 0);
+    auto lo_of = [&](unsigned t) { return static_cast<index_t>(std::uint64_t(n_targets) * t / threads); };
+    // pass 1: in-degrees of each thread's range
+    parallel_for_threads(threads, [&](unsigned t) {
+        const index_t lo = lo_of(t), hi = lo_of(t + 1);
+        index_t* cnt = &out.row_start[std::size_t(lo) + 1];
+        visit(lo, hi, [&](index_t, index_t d) { cnt[d - lo]++; });
+        std::uint64_t s = 0;
+        for (index_t i = lo; i < hi; ++i) s += out.row_start[std::size_t(i) + 1];
+        totals[t] = s;
+    });
+    // exclusive prefix over ranges, then within each range
+    std::vector<std::uint64_t> base(threads, 0);
+    for (unsigned t = 1; t < threads; ++t) base[t] = base[t - 1] + totals[t - 1];
+    parallel_for_threads(threads, [&](unsigned t) {
+        const index_t lo = lo_of(t), hi = lo_of(t + 1);
+        std::uint64_t run = base[t];
+        for (index_t i = lo; i < hi; ++i) {
+            run += out.row_start[std::size_t(i) + 1];
+            out.row_start[std::size_t(i) + 1] = static_cast<index_t>(run);
+        }
+    });
+    // pass 2: scatter in generation order
+    parallel_for_threads(threads, [&](unsigned t) {
+        const index_t lo = lo_of(t), hi = lo_of(t + 1);
+        std::vector<index_t> cursor(out.row_start.begin() + lo, out.row_start.begin() + hi);
+        index_t* src = out.src.data();
+        visit(lo, hi, [&](index_t s, index_t d) { src[cursor[d - lo]++] = s; });
+    });
+}
+
+// Edges of one source-major generation pass: the targets of source s are
 // dst[off[s] .. off[s+1]), in generation order.
 struct SourceMajor {
     std::vector<index_t> dst;
-    std::vector<std::size_t> off;
+    std::vector<std::uint64_t> off;
 };
-
-// Stable counting-sort inversion of source-major passes into a per-target CSR.
-// The passes are consumed in order, so row t lists its sources exactly in
-// generation order. `src_base[k]` is added to source ids of pass k.
-void invert(index_t n_targets, const std::vector<const SourceMajor*>& passes,
-            const std::vector<index_t>& indeg, HostCSR& out) {
-    out.row_start.assign(std::size_t(n_targets) + 1, 0);
-    for (index_t t = 0; t < n_targets; ++t) out.row_start[t + 1] = out.row_start[t] + indeg[t];
-    out.src.assign(out.row_start[n_targets], 0);
-    std::vector<index_t> cursor(out.row_start.begin(), out.row_start.end() - 1);
-    for (const SourceMajor* pass : passes) {
-        const std::size_t n_src = pass->off.size() - 1;
-        for (std::size_t s = 0; s < n_src; ++s) {
-            for (std::size_t k = pass->off[s]; k < pass->off[s + 1]; ++k) {
-                out.src[cursor[pass->dst[k]]++] = static_cast<index_t>(s);
-            }
-        }
-    }
-}
-
-// Pairs variant for the third-factor edges (source ids are explicit).
-void invert_pairs(index_t n_targets, const std::vector<index_t>& srcs,
-                  const std::vector<index_t>& dsts, HostCSR& out) {
-    out.row_start.assign(std::size_t(n_targets) + 1, 0);
-    for (index_t d : dsts) out.row_start[std::size_t(d) + 1]++;
-    for (index_t t = 0; t < n_targets; ++t) out.row_start[t + 1] += out.row_start[t];
-    out.src.assign(srcs.size(), 0);
-    std::vector<index_t> cursor(out.row_start.begin(), out.row_start.end() - 1);
-    for (std::size_t k = 0; k < srcs.size(); ++k) out.src[cursor[dsts[k]]++] = srcs[k];
-}
 
 }  // namespace
 
@@ -157,18 +284,14 @@ void build_network(index_t n_total, std::uint64_t seed, Network& net) {
     net.seed = seed;
     const index_t n_exc = cfg.n_exc, n_inh = cfg.n_inh, n_astro = cfg.n_astro;
     const index_t n_neurons = n_exc + n_inh;
+    const unsigned threads = worker_count();
     const auto t0 = std::chrono::steady_clock::now();
 
     // ---- connectivity: one sequential setup stream (src/astrosim_fused.cu:631-735) ----
     HostRng rng(seed, kSetupStream);
     // Skip the draws the per-cell parameter generation would make first
     // (src/astrosim_fused.cu:644-645).
-    for (index_t i = 0; i < n_astro; ++i) {
-        rng.normal(); rng.normal(); rng.normal(); rng.normal();
-    }
-    for (index_t i = 0; i < n_neurons; ++i) {
-        rng.normal(); rng.normal();
-    }
+    rng.skip_normals(4ull * n_astro + 2ull * n_neurons);
 
     // Astrocyte pools, `take` distinct astrocytes per postsynaptic neuron
     // (src/astrosim_fused.cu:651-671).
@@ -198,84 +321,126 @@ void build_network(index_t n_total, std::uint64_t seed, Network& net) {
             }
         }
     }
+    const auto t_pools = std::chrono::steady_clock::now();
 
+    // Edge generation. Excitatory rows: pre over [0, n_exc), posts over one
+    // block of targets, a third-factor astrocyte pick on 20% of synapses
+    // (src/astrosim_fused.cu:688-710); inhibitory rows keep the LOCAL source
+    // id (src/astrosim_fused.cu:712-723). Third-factor synapses are recorded
+    // as (pre, post, pool slot) and resolved after the sequential pass.
     const double p = cfg.p_primary;
-    const double log1m_p = std::log1p(-p);
-    std::vector<index_t> astro_out_degree(n_astro, 0);
-    std::vector<index_t> indeg_exc(n_neurons, 0), indeg_inh(n_neurons, 0);
+    SetupDraws dr(seed, rng.counter(), p, threads);
     SourceMajor exc_pass[2], inh_pass;
-    std::vector<index_t> e2a_src, e2a_dst;  // (pre, astro) in generation order
-    std::vector<index_t> sic_src, sic_dst;  // (astro, post) in generation order
-    const std::size_t est = static_cast<std::size_t>(ScalingConfig::kSyn * 1.05 * n_neurons) + 64;
+    std::vector<index_t> tf_pre, tf_post;
+    std::vector<std::uint8_t> tf_slot;
+    const std::size_t est = static_cast<std::size_t>(ScalingConfig::kSyn * 1.02 * n_neurons) + 64;
     exc_pass[0].dst.reserve(static_cast<std::size_t>(est * ScalingConfig::kEFraction));
     exc_pass[1].dst.reserve(static_cast<std::size_t>(est * (1.0 - ScalingConfig::kEFraction)) + 64);
     inh_pass.dst.reserve(static_cast<std::size_t>(est * (1.0 - ScalingConfig::kEFraction)) + 64);
-    const std::size_t est3 = static_cast<std::size_t>(est * ScalingConfig::kPThird) + 64;
-    e2a_src.reserve(est3); e2a_dst.reserve(est3);
-    sic_src.reserve(est3); sic_dst.reserve(est3);
+    const std::size_t est3 = static_cast<std::size_t>(est * ScalingConfig::kPThird * 1.02) + 64;
+    tf_pre.reserve(est3); tf_post.reserve(est3); tf_slot.reserve(est3);
 
-    // Excitatory rows: pre over [0, n_exc), posts over one block of targets,
-    // with a third-factor astrocyte edge pair on 20% of synapses
-    // (src/astrosim_fused.cu:688-710).
     auto connect_block = [&](SourceMajor& pass, index_t post_offset, index_t post_count) {
         pass.off.assign(std::size_t(n_exc) + 1, 0);
         for (index_t pre = 0; pre < n_exc; ++pre) {
             pass.off[pre] = pass.dst.size();
             index_t pos = 0;
             while (pos < post_count) {
-                pos += geometric_skip(p, log1m_p, post_count - pos, rng);
+                pos += dr.skip(post_count - pos);
                 if (pos >= post_count) break;
                 const index_t post = post_offset + pos;
                 ++pos;
                 if (pre == post) continue;  // no autapses
                 pass.dst.push_back(post);
-                indeg_exc[post]++;
-                if (rng.uniform() >= ScalingConfig::kPThird) continue;  // n_astro >= 1 always
-                const index_t* pl = &pool[std::size_t(post) * take];
-                const index_t astro =
-                    pl[static_cast<std::size_t>(rng.uniform() * take) % static_cast<std::size_t>(take)];
-                if (astro_out_degree[astro] >= ScalingConfig::kMaxAstroOutDegree) continue;
-                e2a_src.push_back(pre); e2a_dst.push_back(astro);
-                sic_src.push_back(astro); sic_dst.push_back(post);
-                ++astro_out_degree[astro];
+                if (dr.uniform() >= ScalingConfig::kPThird) continue;  // n_astro >= 1 always
+                const std::size_t slot = static_cast<std::size_t>(dr.uniform() * take) % static_cast<std::size_t>(take);
+                tf_pre.push_back(pre);
+                tf_post.push_back(post);
+                tf_slot.push_back(static_cast<std::uint8_t>(slot));
             }
         }
         pass.off[n_exc] = pass.dst.size();
     };
     connect_block(exc_pass[0], 0, n_exc);
     connect_block(exc_pass[1], n_exc, n_inh);
-
-    // Inhibitory rows; the source id kept is the LOCAL inhibitory index
-    // (src/astrosim_fused.cu:712-723).
     inh_pass.off.assign(std::size_t(n_inh) + 1, 0);
     for (index_t pre = 0; pre < n_inh; ++pre) {
         inh_pass.off[pre] = inh_pass.dst.size();
         const index_t pre_global = n_exc + pre;
         index_t pos = 0;
         while (pos < n_neurons) {
-            pos += geometric_skip(p, log1m_p, n_neurons - pos, rng);
+            pos += dr.skip(n_neurons - pos);
             if (pos >= n_neurons) break;
             const index_t post = pos;
             ++pos;
             if (pre_global == post) continue;
             inh_pass.dst.push_back(post);
-            indeg_inh[post]++;
         }
     }
     inh_pass.off[n_inh] = inh_pass.dst.size();
 
-    // Four stable inversions, independent of each other: run them in parallel.
-    {
-        std::thread t_exc([&] { invert(n_neurons, {&exc_pass[0], &exc_pass[1]}, indeg_exc, net.inc_exc); });
-        std::thread t_inh([&] { invert(n_neurons, {&inh_pass}, indeg_inh, net.inc_inh); });
-        std::thread t_sic([&] { invert_pairs(n_neurons, sic_src, sic_dst, net.inc_sic); });
-        invert_pairs(n_astro, e2a_src, e2a_dst, net.inc_e2a);
-        t_exc.join();
-        t_inh.join();
-        t_sic.join();
+    // Third-factor synapses: astrocyte = pool[post][slot] (parallel), then the
+    // out-degree cap applied in generation order (sequential, as the reference).
+    const std::size_t n_tf = tf_pre.size();
+    std::vector<index_t> tf_astro(n_tf);
+    parallel_for_threads(threads, [&](unsigned t) {
+        const std::size_t lo = n_tf * t / threads, hi = n_tf * (t + 1) / threads;
+        for (std::size_t i = lo; i < hi; ++i) tf_astro[i] = pool[std::size_t(tf_post[i]) * take + tf_slot[i]];
+    });
+    std::vector<std::uint8_t>().swap(tf_slot);
+    std::vector<index_t>().swap(pool);
+    std::vector<index_t> astro_out_degree(n_astro, 0);
+    std::size_t kept = 0;
+    for (std::size_t i = 0; i < n_tf; ++i) {
+        const index_t a = tf_astro[i];
+        if (astro_out_degree[a] >= ScalingConfig::kMaxAstroOutDegree) continue;
+        ++astro_out_degree[a];
+        tf_pre[kept] = tf_pre[i];
+        tf_post[kept] = tf_post[i];
+        tf_astro[kept] = a;
+        ++kept;
     }
+    tf_pre.resize(kept); tf_post.resize(kept); tf_astro.resize(kept);
+    const auto t_gen = std::chrono::steady_clock::now();
+
+    // Four stable inversions, each target-partitioned over the worker threads.
+    auto visit_source_major = [](const SourceMajor& pass) {
+        return [&pass](index_t lo, index_t hi, auto&& emit) {
+            const std::size_t n_src = pass.off.size() - 1;
+            for (std::size_t s = 0; s < n_src; ++s)
+                for (std::uint64_t k = pass.off[s]; k < pass.off[s + 1]; ++k) {
+                    const index_t d = pass.dst[k];
+                    if (d >= lo && d < hi) emit(static_cast<index_t>(s), d);
+                }
+        };
+    };
+    {
+        auto v0 = visit_source_major(exc_pass[0]);
+        auto v1 = visit_source_major(exc_pass[1]);
+        // pass 0 holds targets < n_exc, pass 1 targets >= n_exc
+        invert_parallel(n_neurons, exc_pass[0].dst.size() + exc_pass[1].dst.size(), threads,
+                        [&](index_t lo, index_t hi, auto&& emit) {
+                            if (lo < n_exc) v0(lo, std::min(hi, n_exc), emit);
+                            if (hi > n_exc) v1(std::max(lo, n_exc), hi, emit);
+                        }, net.inc_exc);
+    }
+    exc_pass[0] = SourceMajor();
+    exc_pass[1] = SourceMajor();
+    invert_parallel(n_neurons, inh_pass.dst.size(), threads, visit_source_major(inh_pass), net.inc_inh);
+    inh_pass = SourceMajor();
+    invert_parallel(n_neurons, kept, threads, [&](index_t lo, index_t hi, auto&& emit) {
+        for (std::size_t i = 0; i < kept; ++i)
+            if (tf_post[i] >= lo && tf_post[i] < hi) emit(tf_astro[i], tf_post[i]);
+    }, net.inc_sic);
+    invert_parallel(n_astro, kept, threads, [&](index_t lo, index_t hi, auto&& emit) {
+        for (std::size_t i = 0; i < kept; ++i)
+            if (tf_astro[i] >= lo && tf_astro[i] < hi) emit(tf_pre[i], tf_astro[i]);
+    }, net.inc_e2a);
     const auto t1 = std::chrono::steady_clock::now();
     net.build_seconds = std::chrono::duration<double>(t1 - t0).count();
+    net.phase_seconds[0] = std::chrono::duration<double>(t_pools - t0).count();
+    net.phase_seconds[1] = std::chrono::duration<double>(t_gen - t_pools).count();
+    net.phase_seconds[2] = std::chrono::duration<double>(t1 - t_gen).count();
 
     // ---- per-cell parameters: a fresh stream from counter 0 (src/astrosim_fused.cu:917-933) ----
     const Biology bio;
@@ -296,7 +461,9 @@ void build_network(index_t n_total, std::uint64_t seed, Network& net) {
         net.b[i] = draw_scaled(exc ? bio.e_b : bio.i_b, bio.randomize_neuron, bio.rn_lower,
                                bio.rn_upper, bio.rn_var, prng);
     }
+    net.phase_seconds[3] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
 }
+
 
 bool dump_edges_csv(const Network& net, const std::string& path) {
     std::FILE* f = std::fopen(path.c_str(), "w");

@@ -19,6 +19,8 @@ public:
     HostRng(std::uint64_t seed, std::uint64_t stream) : seed_(seed), stream_(stream) {}
     double uniform() { return rng_uniform(seed_, stream_, counter_++); }
     double normal();
+    // Advances the stream exactly as `count` normal() calls would.
+    void skip_normals(std::uint64_t count);
     double normal_redraw(double mean, double stddev, double lo, double hi, int max_tries = 1000);
     std::uint64_t counter() const { return counter_; }
 
@@ -56,6 +58,7 @@ struct Network {
     std::vector<double> Ca_tot, IP3_0, tau_IP3, delta_IP3;  // per astrocyte
     std::vector<double> V_reset, b;                          // per neuron
     double build_seconds = 0.0;  // connectivity generation + inversion
+    double phase_seconds[4] = {0, 0, 0, 0};  // skip+pools, edge generation, inversion, parameters
 };
 
 // Builds the network for (n_total, seed). Throws std::invalid_argument for
