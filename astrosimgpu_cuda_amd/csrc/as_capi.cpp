@@ -59,6 +59,7 @@ void require_device() {
 std::unique_ptr<astrosim::Engine> make_engine(const astrosim::Network& net, const astrosim::RunParams& rp,
                                               hipStream_t s) {
     if (rp.precision == AS_FP64 && rp.method == AS_METHOD_PLAIN) return astrosim::make_plain_engine(net, rp, s);
+    if (rp.precision == AS_FP64 && rp.method == AS_METHOD_BLOCKED) return astrosim::make_blocked_engine(net, rp, s);
     throw astrosim::Error(AS_ERR_INVALID, "this precision/method combination is not available");
 }
 
@@ -142,7 +143,8 @@ as_status as_sim_footprint(const as_network* net, const as_config* cfg, uint64_t
     return guarded([&] {
         require(net && cfg && bytes, "NULL argument");
         const astrosim::RunParams rp = astrosim::resolve_params(*cfg);
-        *bytes = astrosim::plain_footprint(net->net, rp);
+        *bytes = rp.method == AS_METHOD_PLAIN ? astrosim::plain_footprint(net->net, rp)
+                                              : astrosim::blocked_footprint(net->net, rp);
     });
 }
 

@@ -103,5 +103,7 @@ protected:
 
 std::uint64_t plain_footprint(const Network& net, const RunParams& rp);
 std::unique_ptr<Engine> make_plain_engine(const Network& net, const RunParams& rp, hipStream_t s);
+std::uint64_t blocked_footprint(const Network& net, const RunParams& rp);
+std::unique_ptr<Engine> make_blocked_engine(const Network& net, const RunParams& rp, hipStream_t s);
 
 }  // namespace astrosim

@@ -43,33 +43,58 @@ def run_pair(n, seed, steps, chunk=10, method="plain", oracle_kw=None, **kw):
     return sim, osim
 
 
-def test_config1_first_100ms_bit_exact():
+METHODS = ["plain", "blocked"]
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_config1_first_100ms_bit_exact(method):
     # N_total=500, seed 1, dt 0.1: the first 100 ms (1000 steps)
-    run_pair(500, 1, 1000)
+    run_pair(500, 1, 1000, method=method)
 
 
-def test_overrides_no_stp_bit_exact():
-    run_pair(3000, 7, 400, w_n2a=0.5, w_a2n=2.0, stp=False)
+@pytest.mark.parametrize("method", METHODS)
+def test_overrides_no_stp_bit_exact(method):
+    run_pair(3000, 7, 400, w_n2a=0.5, w_a2n=2.0, stp=False, method=method)
 
 
-def test_substeps_and_dt_bit_exact():
-    run_pair(1000, 3, 400, dt=0.05, substeps=2, chunk=20)
+@pytest.mark.parametrize("method", METHODS)
+def test_substeps_and_dt_bit_exact(method):
+    # dt 0.05 ms: delays of 20 steps
+    run_pair(1000, 3, 400, dt=0.05, substeps=2, chunk=20, method=method)
 
 
-@pytest.mark.parametrize("block", [64, 256, 1024])
-def test_block_size_does_not_change_results(block):
-    run_pair(700, 5, 200, block=block)
+def test_blocked_coarse_dt_bit_exact():
+    # dt 0.3 ms: delays of 3 steps, blocks of 3
+    run_pair(2000, 4, 300, dt=0.3, chunk=3, method="blocked")
 
 
+def test_blocked_unaligned_chunks_bit_exact():
+    # launches that start and end inside blocks (partial blocks)
+    run_pair(4000, 9, 301, chunk=7, method="blocked")
+
+
+@pytest.mark.parametrize("method,block", [("plain", 64), ("plain", 1024), ("blocked", 64),
+                                          ("blocked", 256), ("blocked", 1024)])
+def test_block_size_does_not_change_results(method, block):
+    run_pair(700, 5, 200, block=block, method=method)
+
+
+@pytest.mark.parametrize("method", METHODS)
 @pytest.mark.parametrize("n", [1, 7, 20])
-def test_tiny_networks(n):
-    run_pair(n, 2, 300)
+def test_tiny_networks(method, n):
+    run_pair(n, 2, 300, method=method)
 
 
-def test_config1_full_run_summary_matches():
+def test_blocked_larger_network_bit_exact():
+    # 50k cells, 200 ms: exercises bucket overflow fallbacks at the onset
+    run_pair(50000, 11, 2000, method="blocked", chunk=10)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_config1_full_run_summary_matches(method):
     # pre 1000 ms + sim 5000 ms as the reference's default run
     net, onet = A.Network(500, 1), O.OracleNet(500, 1)
-    sim, osim = A.Simulation(net), O.OracleSim(onet, threads=0)
+    sim, osim = A.Simulation(net, method=method), O.OracleSim(onet, threads=0)
     sim.step(10000)
     osim.step(10000)
     assert sim.spike_count() == osim.spike_count()
@@ -80,10 +105,11 @@ def test_config1_full_run_summary_matches():
     assert_state_bitwise(sim, osim, "config 1 end")
 
 
-def test_cli_summary_matches_oracle_cli():
+@pytest.mark.parametrize("method", METHODS)
+def test_cli_summary_matches_oracle_cli(method):
     import subprocess
     args = ["--n", "500", "--seed", "1", "--pre-ms", "200", "--sim-ms", "800"]
-    g = A.run_cli(args, timeout=300)
+    g = A.run_cli(args + ["--method", method], timeout=300)
     assert g.returncode == 0, g.stderr
     o = subprocess.run([str(O.BIN), *args], capture_output=True, text=True, timeout=300)
     pick = lambda out, key: [l for l in out.splitlines() if l.startswith(key)]
