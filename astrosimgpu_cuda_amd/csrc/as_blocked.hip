@@ -25,6 +25,7 @@
 
 #include "as_engine.h"
 #include "as_model.h"
+#include "as_model32.h"
 
 namespace astrosim {
 namespace {
@@ -33,7 +34,9 @@ constexpr int kSpikeCounters = 64;
 constexpr unsigned kBucketCap = 8;
 enum { T_EXC = 0, T_INH = 1, T_SIC = 2, T_E2A = 3 };
 
-struct BlkDev {
+// Indices, buckets, emission words, emitter lists and counters: everything
+// that does not depend on the state precision.
+struct BlkCommon {
     // in-CSR by target (exc, inh, sic, e2a) and the transposed out-CSR by
     // source with (target, in-CSR slot) entries
     const index_t* in_rs[4];
@@ -44,20 +47,10 @@ struct BlkDev {
     // buckets per type: count per target, kBucketCap (slot, source) entries
     unsigned* bcnt[4];
     uint2* bent[4];
-    // astrocyte state
-    double *Ca, *IP3, *h;
-    const double *Ca_tot, *IP3_0, *tau_IP3, *delta_IP3;
     std::uint8_t* ever;
-    // neuron state
-    double *V, *w, *g_ex, *dg_ex, *g_in, *dg_in, *I_sic;
     std::int32_t* refr;
-    const double *V_reset, *b;
-    double *stp_x, *stp_u, *stp_t;
-    // emission history
     std::uint32_t* spk_w;  // [H][n_neurons] spike bits of a block
-    double* spk_v;         // [Rv][n_neurons] x*u at the spike step
     std::uint32_t* sic_w;  // [H][n_astro] nonzero-SIC bits of a block
-    double* sic_v;         // [Rv][n_astro] SIC factor where nonzero
     std::uint32_t* nlist;  // [H][n_neurons] neurons with a nonzero block word
     std::uint32_t* alist;  // [H][n_astro]
     std::uint32_t* lcount; // [H][2]
@@ -65,6 +58,20 @@ struct BlkDev {
     unsigned* overflow;  // bucket overflows seen (diagnostic)
     index_t n_astro, n_neurons, n_exc, n_inh;
     std::uint32_t astro_blocks;
+};
+
+// State arrays in the simulation's precision R (double: FP64; float: FP32c,
+// where Ca/IP3/h carry a compensation term and tau_IP3 holds 1/tau_IP3).
+template <class R>
+struct BlkState {
+    R *Ca, *IP3, *h;
+    R *Ca_lo, *IP3_lo, *h_lo;
+    const R *Ca_tot, *IP3_0, *tau_IP3, *delta_IP3;
+    R *V, *w, *g_ex, *dg_ex, *g_in, *dg_in, *I_sic;
+    const R *V_reset, *b;
+    R *stp_x, *stp_u, *stp_t;
+    R* spk_v;  // [Rv][n_neurons] x*u at the spike step
+    R* sic_v;  // [Rv][n_astro] SIC factor where nonzero
 };
 
 struct BlkLaunch {
@@ -93,10 +100,10 @@ __device__ __forceinline__ void wave_append(bool want, std::uint32_t id, std::ui
 // sum restricted to its nonzero terms, in ascending in-CSR slot order.
 // `word(src)` is the source's emission bits for the read block and
 // `val(src)` its emitted value at the delayed step.
-template <class WordF, class ValF>
-__device__ double input_sum(const uint2* ent, unsigned cnt, const index_t* rs, const index_t* src,
-                            index_t target, unsigned k, WordF word, ValF val) {
-    double s = 0.0;
+template <class R, class WordF, class ValF>
+__device__ R input_sum(const uint2* ent, unsigned cnt, const index_t* rs, const index_t* src,
+                       index_t target, unsigned k, WordF word, ValF val) {
+    R s = R(0);
     if (cnt > kBucketCap) {  // overflow: walk the whole row
         for (index_t e = rs[target]; e < rs[target + 1]; ++e) {
             const index_t q = src[e];
@@ -133,7 +140,7 @@ __device__ std::uint32_t input_mask(const uint2* ent, unsigned cnt, WordF word) 
 // SCATTER: emitters of block b - d/L append to their targets' buckets.
 // One wave per emitter, lanes over its out-edges.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void scatter_kernel(BlkDev D, std::uint32_t rslot, std::uint32_t wslot) {
+__global__ __launch_bounds__(256) void scatter_kernel(BlkCommon D, std::uint32_t rslot, std::uint32_t wslot) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         D.lcount[wslot * 2 + 0] = 0;
         D.lcount[wslot * 2 + 1] = 0;
@@ -171,164 +178,216 @@ __global__ __launch_bounds__(256) void scatter_kernel(BlkDev D, std::uint32_t rs
 // ---------------------------------------------------------------------------
 // INTEGRATE: advance every cell over steps [t0, t0 + n) of one block.
 // ---------------------------------------------------------------------------
-template <int MAXT>
-__global__ __launch_bounds__(MAXT) void integrate_kernel(BlkDev D, RunParams P, std::uint64_t seed, BlkLaunch B) {
+
+// Value-ring row of the delayed step t - d (only dereferenced when a source's
+// bit is set, which implies t >= d).
+__device__ __forceinline__ std::size_t value_row(std::int64_t t, const BlkLaunch& B) {
+    const std::int64_t tv = t - B.d;
+    return tv >= 0 ? std::size_t(tv % B.Rv) : 0;
+}
+
+// One astrocyte over the launch's steps (src/astrosim_fused.cu:374-427).
+template <class R>
+__device__ bool astro_cell(const BlkCommon& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
+                           std::uint64_t seed, const BlkLaunch& B, index_t a) {
+    const std::size_t rN = std::size_t(B.rslot) * D.n_neurons;
+    const unsigned cnt = D.bcnt[T_E2A][a];
+    const uint2* ent = D.bent[T_E2A] + std::size_t(a) * kBucketCap;
+    auto word = [&](index_t q) { return D.spk_w[rN + q]; };
+    const std::uint32_t any = cnt ? input_mask(ent, cnt, word) : 0u;
+    R ca = S.Ca[a], ip3 = S.IP3[a], hh = S.h[a];
+    R ca_lo = R(0), ip3_lo = R(0), hh_lo = R(0);
+    if constexpr (sizeof(R) == 4) {
+        ca_lo = S.Ca_lo[a];
+        ip3_lo = S.IP3_lo[a];
+        hh_lo = S.h_lo[a];
+    }
+    const R Ca_tot = S.Ca_tot[a], IP3_0 = S.IP3_0[a], tau = S.tau_IP3[a], dIP3 = S.delta_IP3[a];
+    std::uint32_t wout = B.first ? 0u : D.sic_w[std::size_t(B.wslot) * D.n_astro + a];
+    bool ever = false;
+    for (int i = 0; i < B.n; ++i) {
+        const std::int64_t t = B.t0 + i;
+        const unsigned k = static_cast<unsigned>(t - B.bstart);
+        R gathered = R(0);
+        if ((any >> k) & 1u) {
+            const std::size_t vrow = value_row(t, B) * D.n_neurons;
+            gathered = input_sum<R>(ent, cnt, D.in_rs[T_E2A], D.in_src[T_E2A], a, k, word,
+                                    [&](index_t q) { return S.spk_v[vrow + q]; });
+        }
+        const std::uint64_t stream = std::uint64_t(t) * kStepStride + a;
+        int ev = 0;
+        if (P.poiss_astro) ev = rng_poisson(seed ^ kKeyAstroPoisson, stream, P.lam_astro, P.lim_astro);
+        const double noise = P.noise_astro > 0.0 ? P.noise_astro * rng_normal(seed ^ kKeyAstroNoise, stream) : 0.0;
+        R factor;
+        if constexpr (sizeof(R) == 8) {
+            double ip3_input = P.w_n2a * gathered;
+            if (ev > 0) ip3_input += P.w_poiss_astro * static_cast<double>(ev);
+            astro_advance(P.ac, Ca_tot, IP3_0, tau, dIP3, ip3_input, noise, P.h_step, P.substeps, ca, ip3, hh);
+            factor = sic_factor(P.ac, ca);
+        } else {
+            float ip3_input = Q.w_n2a * gathered;
+            if (ev > 0) ip3_input += Q.w_poiss_astro * static_cast<float>(ev);
+            astro_advance32(Q.ac, Q, Ca_tot, IP3_0, tau, dIP3, ip3_input, static_cast<float>(noise), P.substeps, ca,
+                            ca_lo, ip3, ip3_lo, hh, hh_lo);
+            factor = sic_factor32(Q.ac, ca, ca_lo);
+        }
+        if (factor > R(0)) {
+            ever = true;
+            wout |= 1u << k;
+            S.sic_v[std::size_t(t % B.Rv) * D.n_astro + a] = factor;
+        }
+    }
+    S.Ca[a] = ca;
+    S.IP3[a] = ip3;
+    S.h[a] = hh;
+    if constexpr (sizeof(R) == 4) {
+        S.Ca_lo[a] = ca_lo;
+        S.IP3_lo[a] = ip3_lo;
+        S.h_lo[a] = hh_lo;
+    }
+    if (ever) D.ever[a] = 1;
+    D.sic_w[std::size_t(B.wslot) * D.n_astro + a] = wout;
+    if (B.last) D.bcnt[T_E2A][a] = 0;
+    return B.last && wout != 0u;
+}
+
+// One neuron over the launch's steps (src/astrosim_fused.cu:433-587).
+// Returns the number of spikes; *emit is set when the neuron joins the
+// block's emitter list.
+template <class R>
+__device__ unsigned neuron_cell(const BlkCommon& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
+                                std::uint64_t seed, const BlkLaunch& B, index_t j, bool* emit) {
     const std::size_t rN = std::size_t(B.rslot) * D.n_neurons;
     const std::size_t rA = std::size_t(B.rslot) * D.n_astro;
-    if (blockIdx.x < D.astro_blocks) {
-        // ---- astrocytes (src/astrosim_fused.cu:374-427), one block of steps ----
-        const index_t a = blockIdx.x * blockDim.x + threadIdx.x;
-        bool emit = false;
-        if (a < D.n_astro) {
-            const unsigned cnt = D.bcnt[T_E2A][a];
-            const uint2* ent = D.bent[T_E2A] + std::size_t(a) * kBucketCap;
-            auto word = [&](index_t q) { return D.spk_w[rN + q]; };
-            const std::uint32_t any = cnt ? input_mask(ent, cnt, word) : 0u;
-            double ca = D.Ca[a], ip3 = D.IP3[a], hh = D.h[a];
-            const double Ca_tot = D.Ca_tot[a], IP3_0 = D.IP3_0[a], tau = D.tau_IP3[a], dIP3 = D.delta_IP3[a];
-            std::uint32_t wout = B.first ? 0u : D.sic_w[std::size_t(B.wslot) * D.n_astro + a];
-            bool ever = false;
-            for (int i = 0; i < B.n; ++i) {
-                const std::int64_t t = B.t0 + i;
-                const unsigned k = static_cast<unsigned>(t - B.bstart);
-                double gathered = 0.0;
-                if ((any >> k) & 1u) {
-                    const std::size_t vrow = std::size_t((t - B.d) % B.Rv) * D.n_neurons;
-                    gathered = input_sum(ent, cnt, D.in_rs[T_E2A], D.in_src[T_E2A], a, k, word,
-                                         [&](index_t q) { return D.spk_v[vrow + q]; });
-                }
-                double ip3_input = P.w_n2a * gathered;
-                const std::uint64_t stream = std::uint64_t(t) * kStepStride + a;
-                if (P.poiss_astro) {
-                    const int ev = rng_poisson(seed ^ kKeyAstroPoisson, stream, P.lam_astro, P.lim_astro);
-                    if (ev > 0) ip3_input += P.w_poiss_astro * static_cast<double>(ev);
-                }
-                const double noise =
-                    P.noise_astro > 0.0 ? P.noise_astro * rng_normal(seed ^ kKeyAstroNoise, stream) : 0.0;
-                astro_advance(P.ac, Ca_tot, IP3_0, tau, dIP3, ip3_input, noise, P.h_step, P.substeps, ca, ip3, hh);
-                const double factor = sic_factor(P.ac, ca);
-                if (factor > 0.0) {
-                    ever = true;
-                    wout |= 1u << k;
-                    D.sic_v[std::size_t(t % B.Rv) * D.n_astro + a] = factor;
-                }
-            }
-            D.Ca[a] = ca;
-            D.IP3[a] = ip3;
-            D.h[a] = hh;
-            if (ever) D.ever[a] = 1;
-            D.sic_w[std::size_t(B.wslot) * D.n_astro + a] = wout;
-            if (B.last) {
-                D.bcnt[T_E2A][a] = 0;
-                emit = wout != 0u;
+    const bool exc = j < D.n_exc;
+    const unsigned ce = D.bcnt[T_EXC][j], ci = D.bcnt[T_INH][j], cs = D.bcnt[T_SIC][j];
+    const uint2* ee = D.bent[T_EXC] + std::size_t(j) * kBucketCap;
+    const uint2* ei = D.bent[T_INH] + std::size_t(j) * kBucketCap;
+    const uint2* es = D.bent[T_SIC] + std::size_t(j) * kBucketCap;
+    auto nword = [&](index_t q) { return D.spk_w[rN + q]; };
+    auto aword = [&](index_t q) { return D.sic_w[rA + q]; };
+    const std::uint32_t any_e = ce ? input_mask(ee, ce, nword) : 0u;
+    const std::uint32_t any_i = ci ? input_mask(ei, ci, nword) : 0u;
+    const std::uint32_t any_s = cs ? input_mask(es, cs, aword) : 0u;
+    R V = S.V[j], w = S.w[j], g_ex = S.g_ex[j], dg_ex = S.dg_ex[j], g_in = S.g_in[j], dg_in = S.dg_in[j];
+    int refr = D.refr[j];
+    R I_sic = S.I_sic[j];
+    const R V_reset = S.V_reset[j], bb = S.b[j];
+    const double nstd = exc ? P.noise_exc : P.noise_inh;
+    std::uint64_t noise_at = ~std::uint64_t(0);
+    R noise_val = R(0);
+    unsigned nspikes = 0;
+    std::uint32_t wout = B.first ? 0u : D.spk_w[std::size_t(B.wslot) * D.n_neurons + j];
+    for (int i = 0; i < B.n; ++i) {
+        const std::int64_t t = B.t0 + i;
+        const unsigned k = static_cast<unsigned>(t - B.bstart);
+        const std::size_t vrow = value_row(t, B);
+        auto nval = [&](index_t q) { return S.spk_v[vrow * D.n_neurons + q]; };
+        R exc_sum = R(0), inh_sum = R(0);
+        if ((any_e >> k) & 1u) exc_sum = input_sum<R>(ee, ce, D.in_rs[T_EXC], D.in_src[T_EXC], j, k, nword, nval);
+        // inhibitory source ids are local indices used as neuron ids (src/astrosim_fused.cu:459-462)
+        if ((any_i >> k) & 1u) inh_sum = input_sum<R>(ei, ci, D.in_rs[T_INH], D.in_src[T_INH], j, k, nword, nval);
+        int ev = 0;
+        if (exc ? P.poiss_exc : P.poiss_inh)
+            ev = rng_poisson(seed ^ kKeyNeuronPoisson, std::uint64_t(t) * kStepStride + j, exc ? P.lam_exc : P.lam_inh,
+                             exc ? P.lim_exc : P.lim_inh);
+        if ((t - P.d_a2n >= 0) && ((t - P.d_a2n) % P.sic_interval == 0)) {
+            R sic_sum = R(0);
+            if ((any_s >> k) & 1u)
+                sic_sum = input_sum<R>(es, cs, D.in_rs[T_SIC], D.in_src[T_SIC], j, k, aword,
+                                       [&](index_t q) { return S.sic_v[vrow * D.n_astro + q]; });
+            if constexpr (sizeof(R) == 8) I_sic = P.w_a2n * sic_sum;
+            else I_sic = Q.w_a2n * sic_sum;
+        }
+        if (nstd > 0.0) {
+            // the neuron noise stream changes every steps_per_noise steps
+            const std::uint64_t nidx = static_cast<std::uint64_t>(t / P.steps_per_noise);
+            if (nidx != noise_at) {
+                noise_at = nidx;
+                noise_val = static_cast<R>(nstd * rng_normal(seed ^ kKeyNeuronNoise, nidx * kNoiseStride + std::uint64_t(j)));
             }
         }
+        bool spiked;
+        if constexpr (sizeof(R) == 8) {
+            const NeuronConst& p = exc ? P.ne : P.ni;
+            double exc_input = P.w_e * exc_sum;
+            const double inh_input = (-P.w_i) * inh_sum;
+            if (ev > 0) exc_input += (exc ? P.w_poiss_exc : P.w_poiss_inh) * static_cast<double>(ev);
+            double I_ext = I_sic;
+            if (nstd > 0.0) I_ext += noise_val;
+            NeuronRegs r{V, w, g_ex, dg_ex + exc_input * p.psc_init_ex, g_in, dg_in + inh_input * p.psc_init_in, refr};
+            spiked = neuron_substeps(p, r, I_ext, V_reset, bb, P.h_step, P.substeps,
+                                     exc ? P.ex_decay_e : P.ex_decay_i, exc ? P.in_decay_e : P.in_decay_i);
+            V = r.V; w = r.w; g_ex = r.g_ex; dg_ex = r.dg_ex; g_in = r.g_in; dg_in = r.dg_in; refr = r.refractory;
+        } else {
+            const NeuronConst32& p = exc ? Q.ne : Q.ni;
+            float exc_input = Q.w_e_abs * exc_sum;
+            const float inh_input = Q.w_i_abs * inh_sum;
+            if (ev > 0) exc_input += (exc ? Q.w_poiss_exc : Q.w_poiss_inh) * static_cast<float>(ev);
+            const float I_ext = I_sic + noise_val;
+            NeuronRegs32 r{V, w, g_ex, dg_ex + exc_input * p.psc_init_ex, g_in, dg_in + inh_input * p.psc_init_in, refr};
+            spiked = neuron_substeps32(p, Q, r, I_ext, V_reset, bb, P.substeps, exc ? Q.ex_decay_e : Q.ex_decay_i,
+                                       exc ? Q.in_decay_e : Q.in_decay_i);
+            V = r.V; w = r.w; g_ex = r.g_ex; dg_ex = r.dg_ex; g_in = r.g_in; dg_in = r.dg_in; refr = r.refractory;
+        }
+        if (spiked) {
+            R out = R(1);
+            if (P.stp_enabled) {
+                R x = S.stp_x[j], u = S.stp_u[j], tl = S.stp_t[j];
+                if constexpr (sizeof(R) == 8) {
+                    out = stp_on_spike(x, u, tl, static_cast<double>(t) * P.dt, P.stp_U, P.stp_tau_rec, P.stp_tau_fac);
+                } else {
+                    const float t_now = static_cast<float>(static_cast<double>(t) * P.dt);
+                    const float elapsed = t_now - tl;
+                    const float x_decay = __expf(-elapsed / static_cast<float>(P.stp_tau_rec));
+                    const float u_decay =
+                        P.stp_tau_fac < 1e-10 ? 0.0f : __expf(-elapsed / static_cast<float>(P.stp_tau_fac));
+                    const float U = static_cast<float>(P.stp_U);
+                    x = 1.0f + (x - x * u - 1.0f) * x_decay;
+                    u = U + u * (1.0f - U) * u_decay;
+                    tl = t_now;
+                    out = x * u;
+                }
+                S.stp_x[j] = x;
+                S.stp_u[j] = u;
+                S.stp_t[j] = tl;
+            }
+            S.spk_v[std::size_t(t % B.Rv) * D.n_neurons + j] = out;
+            wout |= 1u << k;
+            ++nspikes;
+        }
+    }
+    S.V[j] = V;
+    S.w[j] = w;
+    S.g_ex[j] = g_ex;
+    S.dg_ex[j] = dg_ex;
+    S.g_in[j] = g_in;
+    S.dg_in[j] = dg_in;
+    D.refr[j] = refr;
+    S.I_sic[j] = I_sic;
+    D.spk_w[std::size_t(B.wslot) * D.n_neurons + j] = wout;
+    if (B.last) {
+        D.bcnt[T_EXC][j] = 0;
+        D.bcnt[T_INH][j] = 0;
+        D.bcnt[T_SIC][j] = 0;
+    }
+    *emit = B.last && wout != 0u;
+    return nspikes;
+}
+
+template <int MAXT, class R>
+__global__ __launch_bounds__(MAXT) void integrate_kernel(BlkCommon D, BlkState<R> S, RunParams P, Params32 Q,
+                                                         std::uint64_t seed, BlkLaunch B) {
+    if (blockIdx.x < D.astro_blocks) {
+        const index_t a = blockIdx.x * blockDim.x + threadIdx.x;
+        const bool emit = a < D.n_astro ? astro_cell<R>(D, S, P, Q, seed, B, a) : false;
         wave_append(emit, a, D.alist + std::size_t(B.wslot) * D.n_astro, &D.lcount[B.wslot * 2 + 1]);
         return;
     }
-
-    // ---- neurons (src/astrosim_fused.cu:433-587), one block of steps ----
     const index_t j = (blockIdx.x - D.astro_blocks) * blockDim.x + threadIdx.x;
     bool emit = false;
     unsigned nspikes = 0;
-    if (j < D.n_neurons) {
-        const bool exc = j < D.n_exc;
-        const NeuronConst& p = exc ? P.ne : P.ni;
-        const unsigned ce = D.bcnt[T_EXC][j], ci = D.bcnt[T_INH][j], cs = D.bcnt[T_SIC][j];
-        const uint2* ee = D.bent[T_EXC] + std::size_t(j) * kBucketCap;
-        const uint2* ei = D.bent[T_INH] + std::size_t(j) * kBucketCap;
-        const uint2* es = D.bent[T_SIC] + std::size_t(j) * kBucketCap;
-        auto nword = [&](index_t q) { return D.spk_w[rN + q]; };
-        auto aword = [&](index_t q) { return D.sic_w[rA + q]; };
-        const std::uint32_t any_e = ce ? input_mask(ee, ce, nword) : 0u;
-        const std::uint32_t any_i = ci ? input_mask(ei, ci, nword) : 0u;
-        const std::uint32_t any_s = cs ? input_mask(es, cs, aword) : 0u;
-        NeuronRegs r;
-        r.V = D.V[j];
-        r.w = D.w[j];
-        r.g_ex = D.g_ex[j];
-        r.dg_ex = D.dg_ex[j];
-        r.g_in = D.g_in[j];
-        r.dg_in = D.dg_in[j];
-        r.refractory = D.refr[j];
-        double I_sic = D.I_sic[j];
-        const double V_reset = D.V_reset[j], bb = D.b[j];
-        const double nstd = exc ? P.noise_exc : P.noise_inh;
-        const double ex_decay = exc ? P.ex_decay_e : P.ex_decay_i;
-        const double in_decay = exc ? P.in_decay_e : P.in_decay_i;
-        std::uint64_t noise_at = ~std::uint64_t(0);
-        double noise_val = 0.0;
-        std::uint32_t wout = B.first ? 0u : D.spk_w[std::size_t(B.wslot) * D.n_neurons + j];
-        for (int i = 0; i < B.n; ++i) {
-            const std::int64_t t = B.t0 + i;
-            const unsigned k = static_cast<unsigned>(t - B.bstart);
-            // value-ring row of the delayed step (only dereferenced when a
-            // source's bit is set, which implies t >= d)
-            const std::int64_t tv = t - B.d;
-            const std::size_t vrow = tv >= 0 ? std::size_t(tv % B.Rv) : 0;
-            const std::size_t vrowN = vrow * D.n_neurons;
-            const std::size_t vrowA = vrow * D.n_astro;
-            auto nval = [&](index_t q) { return D.spk_v[vrowN + q]; };
-            double exc_sum = 0.0, inh_sum = 0.0;
-            if ((any_e >> k) & 1u) exc_sum = input_sum(ee, ce, D.in_rs[T_EXC], D.in_src[T_EXC], j, k, nword, nval);
-            if ((any_i >> k) & 1u) inh_sum = input_sum(ei, ci, D.in_rs[T_INH], D.in_src[T_INH], j, k, nword, nval);
-            double exc_input = P.w_e * exc_sum;
-            const double inh_input = (-P.w_i) * inh_sum;
-            if (exc ? P.poiss_exc : P.poiss_inh) {
-                const int ev = rng_poisson(seed ^ kKeyNeuronPoisson, std::uint64_t(t) * kStepStride + j,
-                                           exc ? P.lam_exc : P.lam_inh, exc ? P.lim_exc : P.lim_inh);
-                if (ev > 0) exc_input += (exc ? P.w_poiss_exc : P.w_poiss_inh) * static_cast<double>(ev);
-            }
-            if ((t - P.d_a2n >= 0) && ((t - P.d_a2n) % P.sic_interval == 0)) {
-                double sic_sum = 0.0;
-                if ((any_s >> k) & 1u)
-                    sic_sum = input_sum(es, cs, D.in_rs[T_SIC], D.in_src[T_SIC], j, k, aword,
-                                        [&](index_t q) { return D.sic_v[vrowA + q]; });
-                I_sic = P.w_a2n * sic_sum;
-            }
-            double I_ext = I_sic;
-            if (nstd > 0.0) {
-                // the noise stream is refreshed every steps_per_noise steps
-                const std::uint64_t nidx = static_cast<std::uint64_t>(t / P.steps_per_noise);
-                if (nidx != noise_at) {
-                    noise_at = nidx;
-                    noise_val = nstd * rng_normal(seed ^ kKeyNeuronNoise, nidx * kNoiseStride + std::uint64_t(j));
-                }
-                I_ext += noise_val;
-            }
-            r.dg_ex = r.dg_ex + exc_input * p.psc_init_ex;
-            r.dg_in = r.dg_in + inh_input * p.psc_init_in;
-            if (neuron_substeps(p, r, I_ext, V_reset, bb, P.h_step, P.substeps, ex_decay, in_decay)) {
-                double out = 1.0;
-                if (P.stp_enabled) {
-                    double x = D.stp_x[j], u = D.stp_u[j], tl = D.stp_t[j];
-                    out = stp_on_spike(x, u, tl, static_cast<double>(t) * P.dt, P.stp_U, P.stp_tau_rec,
-                                       P.stp_tau_fac);
-                    D.stp_x[j] = x;
-                    D.stp_u[j] = u;
-                    D.stp_t[j] = tl;
-                }
-                D.spk_v[std::size_t(t % B.Rv) * D.n_neurons + j] = out;
-                wout |= 1u << k;
-                ++nspikes;
-            }
-        }
-        D.V[j] = r.V;
-        D.w[j] = r.w;
-        D.g_ex[j] = r.g_ex;
-        D.dg_ex[j] = r.dg_ex;
-        D.g_in[j] = r.g_in;
-        D.dg_in[j] = r.dg_in;
-        D.refr[j] = r.refractory;
-        D.I_sic[j] = I_sic;
-        D.spk_w[std::size_t(B.wslot) * D.n_neurons + j] = wout;
-        if (B.last) {
-            D.bcnt[T_EXC][j] = 0;
-            D.bcnt[T_INH][j] = 0;
-            D.bcnt[T_SIC][j] = 0;
-            emit = wout != 0u;
-        }
-    }
+    if (j < D.n_neurons) nspikes = neuron_cell<R>(D, S, P, Q, seed, B, j, &emit);
     wave_append(emit, j, D.nlist + std::size_t(B.wslot) * D.n_neurons, &D.lcount[B.wslot * 2 + 0]);
     // diagnostic spike total: one atomic per wave with spikes
     unsigned long long c = nspikes;
@@ -357,10 +416,6 @@ __global__ void fill_out_kernel(const index_t* rs, const index_t* src, index_t n
     }
 }
 
-__global__ void fill_f64_kernel(double* p, double v, std::size_t n) {
-    const std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
 
 // Block geometry shared by the engine and its footprint.
 struct Geometry {
@@ -382,7 +437,7 @@ Geometry geometry(const RunParams& rp) {
 
 struct BlkLayout {
     std::size_t in_rs[4], in_src[4], out_rs[4], out_ent[4], bcnt[4], bent[4];
-    std::size_t astro[7], ever, neuron[12], refr;
+    std::size_t astro[10], ever, neuron[12], refr;
     std::size_t spk_w, spk_v, sic_w, sic_v, nlist, alist, lcount, counters, overflow;
     std::size_t total;
 };
@@ -396,7 +451,8 @@ index_t out_sources(const Network& net, int T) {
     }
 }
 
-BlkLayout blk_layout(const Network& net, const RunParams& rp) {
+// rsize: bytes of one state value (8 FP64, 4 FP32c).
+BlkLayout blk_layout(const Network& net, const RunParams& rp, std::size_t rsize) {
     const Geometry g = geometry(rp);
     Layout L;
     BlkLayout o{};
@@ -412,14 +468,16 @@ BlkLayout blk_layout(const Network& net, const RunParams& rp) {
         o.bcnt[i] = L.add(nt * sizeof(unsigned));
         o.bent[i] = L.add(nt * kBucketCap * sizeof(uint2));
     }
-    for (int i = 0; i < 7; ++i) o.astro[i] = L.add(na * sizeof(double));
+    // Ca IP3 h, Ca_tot IP3_0 tau delta, and (FP32c) the three compensation terms
+    const int n_astro_arrays = rsize == 4 ? 10 : 7;
+    for (int i = 0; i < n_astro_arrays; ++i) o.astro[i] = L.add(na * rsize);
     o.ever = L.add(na);
-    for (int i = 0; i < 12; ++i) o.neuron[i] = L.add(nn * sizeof(double));
+    for (int i = 0; i < 12; ++i) o.neuron[i] = L.add(nn * rsize);
     o.refr = L.add(nn * sizeof(std::int32_t));
     o.spk_w = L.add(std::size_t(g.H) * nn * sizeof(std::uint32_t));
-    o.spk_v = L.add(std::size_t(g.Rv) * nn * sizeof(double));
+    o.spk_v = L.add(std::size_t(g.Rv) * nn * rsize);
     o.sic_w = L.add(std::size_t(g.H) * na * sizeof(std::uint32_t));
-    o.sic_v = L.add(std::size_t(g.Rv) * na * sizeof(double));
+    o.sic_v = L.add(std::size_t(g.Rv) * na * rsize);
     o.nlist = L.add(std::size_t(g.H) * nn * sizeof(std::uint32_t));
     o.alist = L.add(std::size_t(g.H) * na * sizeof(std::uint32_t));
     o.lcount = L.add(std::size_t(g.H) * 2 * sizeof(std::uint32_t));
@@ -429,15 +487,23 @@ BlkLayout blk_layout(const Network& net, const RunParams& rp) {
     return o;
 }
 
+template <class R>
+__global__ void fill_kernel(R* p, R v, std::size_t n) {
+    const std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <class R>
 class BlockedEngine final : public Engine {
 public:
-    BlockedEngine(const Network& net, const RunParams& rp, hipStream_t s) : P_(rp), seed_(net.seed) {
+    BlockedEngine(const Network& net, const RunParams& rp, hipStream_t s)
+        : P_(rp), Q_(make_params32(rp)), seed_(net.seed) {
         stream_ = s;
         geo_ = geometry(rp);
         n_neurons_ = net.cfg.n_neurons();
         n_astro_ = net.cfg.n_astro;
         n_exc_ = net.cfg.n_exc;
-        lay_ = blk_layout(net, rp);
+        lay_ = blk_layout(net, rp, sizeof(R));
         if (hipMalloc(&base_, lay_.total) != hipSuccess) {
             (void)hipGetLastError();
             base_ = nullptr;
@@ -463,21 +529,22 @@ public:
             AS_HIP_CHECK(hipMemsetAsync(D_.bcnt[i], 0, std::size_t(nt) * sizeof(unsigned), stream_));
             build_out_csr(i, nt, c[i]->n_edges());
         }
-        double* A[7];
-        for (int i = 0; i < 7; ++i) A[i] = (double*)dptr(lay_.astro[i]);
-        D_.Ca = A[0]; D_.IP3 = A[1]; D_.h = A[2];
-        D_.Ca_tot = A[3]; D_.IP3_0 = A[4]; D_.tau_IP3 = A[5]; D_.delta_IP3 = A[6];
+        R* A[10] = {};
+        for (int i = 0; i < (sizeof(R) == 4 ? 10 : 7); ++i) A[i] = (R*)dptr(lay_.astro[i]);
+        S_.Ca = A[0]; S_.IP3 = A[1]; S_.h = A[2];
+        S_.Ca_tot = A[3]; S_.IP3_0 = A[4]; S_.tau_IP3 = A[5]; S_.delta_IP3 = A[6];
+        S_.Ca_lo = A[7]; S_.IP3_lo = A[8]; S_.h_lo = A[9];
         D_.ever = (std::uint8_t*)dptr(lay_.ever);
-        double* N[12];
-        for (int i = 0; i < 12; ++i) N[i] = (double*)dptr(lay_.neuron[i]);
-        D_.V = N[0]; D_.w = N[1]; D_.g_ex = N[2]; D_.dg_ex = N[3]; D_.g_in = N[4]; D_.dg_in = N[5];
-        D_.I_sic = N[6]; D_.stp_x = N[7]; D_.stp_u = N[8]; D_.stp_t = N[9];
-        D_.V_reset = N[10]; D_.b = N[11];
+        R* N[12];
+        for (int i = 0; i < 12; ++i) N[i] = (R*)dptr(lay_.neuron[i]);
+        S_.V = N[0]; S_.w = N[1]; S_.g_ex = N[2]; S_.dg_ex = N[3]; S_.g_in = N[4]; S_.dg_in = N[5];
+        S_.I_sic = N[6]; S_.stp_x = N[7]; S_.stp_u = N[8]; S_.stp_t = N[9];
+        S_.V_reset = N[10]; S_.b = N[11];
         D_.refr = (std::int32_t*)dptr(lay_.refr);
         D_.spk_w = (std::uint32_t*)dptr(lay_.spk_w);
-        D_.spk_v = (double*)dptr(lay_.spk_v);
+        S_.spk_v = (R*)dptr(lay_.spk_v);
         D_.sic_w = (std::uint32_t*)dptr(lay_.sic_w);
-        D_.sic_v = (double*)dptr(lay_.sic_v);
+        S_.sic_v = (R*)dptr(lay_.sic_v);
         D_.nlist = (std::uint32_t*)dptr(lay_.nlist);
         D_.alist = (std::uint32_t*)dptr(lay_.alist);
         D_.lcount = (std::uint32_t*)dptr(lay_.lcount);
@@ -490,32 +557,45 @@ public:
         D_.astro_blocks = (n_astro_ + rp.block - 1) / rp.block;
         neuron_blocks_ = (n_neurons_ + rp.block - 1) / rp.block;
 
-        // Parameters and initial state (src/astrosim_fused.cu:981-1018).
+        // Parameters and initial state (src/astrosim_fused.cu:981-1018). In
+        // FP32c the astrocyte initial values keep their rounding residue in
+        // the compensation terms and tau_IP3 is stored as its reciprocal.
         const Biology bio;
-        AS_HIP_CHECK(hipMemcpy(A[3], net.Ca_tot.data(), n_astro_ * sizeof(double), hipMemcpyHostToDevice));
-        AS_HIP_CHECK(hipMemcpy(A[4], net.IP3_0.data(), n_astro_ * sizeof(double), hipMemcpyHostToDevice));
-        AS_HIP_CHECK(hipMemcpy(A[5], net.tau_IP3.data(), n_astro_ * sizeof(double), hipMemcpyHostToDevice));
-        AS_HIP_CHECK(hipMemcpy(A[6], net.delta_IP3.data(), n_astro_ * sizeof(double), hipMemcpyHostToDevice));
-        AS_HIP_CHECK(hipMemcpy(N[10], net.V_reset.data(), n_neurons_ * sizeof(double), hipMemcpyHostToDevice));
-        AS_HIP_CHECK(hipMemcpy(N[11], net.b.data(), n_neurons_ * sizeof(double), hipMemcpyHostToDevice));
-        fill(A[0], bio.Ca_init, n_astro_);
-        fill(A[1], bio.IP3_init, n_astro_);
-        fill(A[2], bio.h_init, n_astro_);
+        upload(A[3], net.Ca_tot);
+        upload(A[4], net.IP3_0);
+        if constexpr (sizeof(R) == 4) {
+            std::vector<double> inv(net.tau_IP3.size());
+            for (std::size_t i = 0; i < inv.size(); ++i) inv[i] = 1.0 / net.tau_IP3[i];
+            upload(A[5], inv);
+        } else {
+            upload(A[5], net.tau_IP3);
+        }
+        upload(A[6], net.delta_IP3);
+        upload(N[10], net.V_reset);
+        upload(N[11], net.b);
+        fill(A[0], R(bio.Ca_init), n_astro_);
+        fill(A[1], R(bio.IP3_init), n_astro_);
+        fill(A[2], R(bio.h_init), n_astro_);
+        if constexpr (sizeof(R) == 4) {
+            fill(A[7], R(bio.Ca_init - double(float(bio.Ca_init))), n_astro_);
+            fill(A[8], R(bio.IP3_init - double(float(bio.IP3_init))), n_astro_);
+            fill(A[9], R(bio.h_init - double(float(bio.h_init))), n_astro_);
+        }
         {
             std::vector<double> V0(n_neurons_);
             for (index_t i = 0; i < n_neurons_; ++i) V0[i] = i < n_exc_ ? bio.e_V_m : bio.i_V_m;
-            AS_HIP_CHECK(hipMemcpy(N[0], V0.data(), n_neurons_ * sizeof(double), hipMemcpyHostToDevice));
+            upload(N[0], V0);
         }
-        for (int i = 1; i <= 6; ++i) AS_HIP_CHECK(hipMemsetAsync(N[i], 0, n_neurons_ * sizeof(double), stream_));
-        fill(N[7], 1.0, n_neurons_);
-        fill(N[8], rp.stp_U, n_neurons_);
-        fill(N[9], -1e12, n_neurons_);
+        for (int i = 1; i <= 6; ++i) AS_HIP_CHECK(hipMemsetAsync(N[i], 0, n_neurons_ * sizeof(R), stream_));
+        fill(N[7], R(1), n_neurons_);
+        fill(N[8], R(rp.stp_U), n_neurons_);
+        fill(N[9], R(-1e12), n_neurons_);
         AS_HIP_CHECK(hipMemsetAsync(D_.refr, 0, n_neurons_ * sizeof(std::int32_t), stream_));
         AS_HIP_CHECK(hipMemsetAsync(D_.ever, 0, n_astro_, stream_));
         AS_HIP_CHECK(hipMemsetAsync(D_.spk_w, 0, std::size_t(geo_.H) * n_neurons_ * 4, stream_));
         AS_HIP_CHECK(hipMemsetAsync(D_.sic_w, 0, std::size_t(geo_.H) * n_astro_ * 4, stream_));
-        AS_HIP_CHECK(hipMemsetAsync(D_.spk_v, 0, std::size_t(geo_.Rv) * n_neurons_ * 8, stream_));
-        AS_HIP_CHECK(hipMemsetAsync(D_.sic_v, 0, std::size_t(geo_.Rv) * n_astro_ * 8, stream_));
+        AS_HIP_CHECK(hipMemsetAsync(S_.spk_v, 0, std::size_t(geo_.Rv) * n_neurons_ * sizeof(R), stream_));
+        AS_HIP_CHECK(hipMemsetAsync(S_.sic_v, 0, std::size_t(geo_.Rv) * n_astro_ * sizeof(R), stream_));
         AS_HIP_CHECK(hipMemsetAsync(D_.lcount, 0, std::size_t(geo_.H) * 2 * 4, stream_));
         AS_HIP_CHECK(hipMemsetAsync(D_.counters, 0, kSpikeCounters * sizeof(unsigned long long), stream_));
         AS_HIP_CHECK(hipMemsetAsync(D_.overflow, 0, sizeof(unsigned), stream_));
@@ -547,13 +627,15 @@ public:
             B.first = step_ == B.bstart;
             B.last = step_ + B.n == B.bstart + geo_.L;
             if (B.first) {
-                hipLaunchKernelGGL(scatter_kernel, dim3(scatter_blocks_), dim3(256), 0, stream_, D_, B.rslot, B.wslot);
+                hipLaunchKernelGGL(scatter_kernel, dim3(kScatterBlocks), dim3(256), 0, stream_, D_, B.rslot, B.wslot);
                 ++launches;
             }
             if (P_.block <= 256)
-                hipLaunchKernelGGL(integrate_kernel<256>, dim3(grid), dim3(P_.block), 0, stream_, D_, P_, seed_, B);
+                hipLaunchKernelGGL((integrate_kernel<256, R>), dim3(grid), dim3(P_.block), 0, stream_, D_, S_, P_, Q_,
+                                   seed_, B);
             else
-                hipLaunchKernelGGL(integrate_kernel<1024>, dim3(grid), dim3(P_.block), 0, stream_, D_, P_, seed_, B);
+                hipLaunchKernelGGL((integrate_kernel<1024, R>), dim3(grid), dim3(P_.block), 0, stream_, D_, S_, P_, Q_,
+                                   seed_, B);
             ++launches;
             step_ += B.n;
             k -= B.n;
@@ -574,26 +656,33 @@ public:
     void reset_ever_active() override { AS_HIP_CHECK(hipMemsetAsync(D_.ever, 0, n_astro_, stream_)); }
 
     void read_state(int field, double* out) override {
-        const double* src = nullptr;
+        const R* src = nullptr;
+        const R* lo = nullptr;
         std::size_t n = n_neurons_;
         switch (field) {
-            case 0: src = D_.Ca; n = n_astro_; break;
-            case 1: src = D_.IP3; n = n_astro_; break;
-            case 2: src = D_.h; n = n_astro_; break;
-            case 3: src = D_.V; break;
-            case 4: src = D_.w; break;
-            case 5: src = D_.g_ex; break;
-            case 6: src = D_.dg_ex; break;
-            case 7: src = D_.g_in; break;
-            case 8: src = D_.dg_in; break;
-            case 9: src = D_.I_sic; break;
-            case 10: src = D_.stp_x; break;
-            case 11: src = D_.stp_u; break;
-            case 12: src = D_.stp_t; break;
+            case 0: src = S_.Ca; lo = S_.Ca_lo; n = n_astro_; break;
+            case 1: src = S_.IP3; lo = S_.IP3_lo; n = n_astro_; break;
+            case 2: src = S_.h; lo = S_.h_lo; n = n_astro_; break;
+            case 3: src = S_.V; break;
+            case 4: src = S_.w; break;
+            case 5: src = S_.g_ex; break;
+            case 6: src = S_.dg_ex; break;
+            case 7: src = S_.g_in; break;
+            case 8: src = S_.dg_in; break;
+            case 9: src = S_.I_sic; break;
+            case 10: src = S_.stp_x; break;
+            case 11: src = S_.stp_u; break;
+            case 12: src = S_.stp_t; break;
             default: throw Error(AS_ERR_INVALID, "unknown state field " + std::to_string(field));
         }
-        AS_HIP_CHECK(hipMemcpyAsync(out, src, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        std::vector<R> h(n), l;
+        AS_HIP_CHECK(hipMemcpyAsync(h.data(), src, n * sizeof(R), hipMemcpyDeviceToHost, stream_));
+        if (sizeof(R) == 4 && lo) {
+            l.resize(n);
+            AS_HIP_CHECK(hipMemcpyAsync(l.data(), lo, n * sizeof(R), hipMemcpyDeviceToHost, stream_));
+        }
         AS_HIP_CHECK(hipStreamSynchronize(stream_));
+        for (std::size_t i = 0; i < n; ++i) out[i] = double(h[i]) + (l.empty() ? 0.0 : double(l[i]));
     }
 
     void read_refractory(std::int32_t* out) override {
@@ -630,9 +719,19 @@ public:
     std::uint64_t device_bytes() const override { return lay_.total; }
 
 private:
-    void fill(double* p, double v, std::size_t n) {
+    void upload(const R* dst, const std::vector<double>& v) {
+        if (v.empty()) return;
+        if constexpr (sizeof(R) == 8) {
+            AS_HIP_CHECK(hipMemcpy(const_cast<R*>(dst), v.data(), v.size() * sizeof(R), hipMemcpyHostToDevice));
+        } else {
+            std::vector<R> f(v.begin(), v.end());
+            AS_HIP_CHECK(hipMemcpy(const_cast<R*>(dst), f.data(), f.size() * sizeof(R), hipMemcpyHostToDevice));
+        }
+    }
+
+    void fill(R* p, R v, std::size_t n) {
         if (n == 0) return;
-        hipLaunchKernelGGL(fill_f64_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream_, p, v, n);
+        hipLaunchKernelGGL(fill_kernel<R>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream_, p, v, n);
         AS_HIP_CHECK(hipGetLastError());
     }
 
@@ -665,22 +764,27 @@ private:
         AS_HIP_CHECK(hipFree(cursor));
     }
 
+    static constexpr unsigned kScatterBlocks = 1024;
     RunParams P_;
+    Params32 Q_;
     std::uint64_t seed_;
     Geometry geo_;
-    BlkDev D_{};
+    BlkCommon D_{};
+    BlkState<R> S_{};
     BlkLayout lay_{};
     void* base_ = nullptr;
     std::uint32_t neuron_blocks_ = 0;
-    static constexpr unsigned scatter_blocks_ = 1024;
 };
 
 }  // namespace
 
-std::uint64_t blocked_footprint(const Network& net, const RunParams& rp) { return blk_layout(net, rp).total; }
+std::uint64_t blocked_footprint(const Network& net, const RunParams& rp) {
+    return blk_layout(net, rp, rp.precision == AS_FP64 ? 8 : 4).total;
+}
 
 std::unique_ptr<Engine> make_blocked_engine(const Network& net, const RunParams& rp, hipStream_t s) {
-    return std::unique_ptr<Engine>(new BlockedEngine(net, rp, s));
+    if (rp.precision == AS_FP64) return std::unique_ptr<Engine>(new BlockedEngine<double>(net, rp, s));
+    return std::unique_ptr<Engine>(new BlockedEngine<float>(net, rp, s));
 }
 
 }  // namespace astrosim

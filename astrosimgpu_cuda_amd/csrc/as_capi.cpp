@@ -59,7 +59,7 @@ void require_device() {
 std::unique_ptr<astrosim::Engine> make_engine(const astrosim::Network& net, const astrosim::RunParams& rp,
                                               hipStream_t s) {
     if (rp.precision == AS_FP64 && rp.method == AS_METHOD_PLAIN) return astrosim::make_plain_engine(net, rp, s);
-    if (rp.precision == AS_FP64 && rp.method == AS_METHOD_BLOCKED) return astrosim::make_blocked_engine(net, rp, s);
+    if (rp.method == AS_METHOD_BLOCKED) return astrosim::make_blocked_engine(net, rp, s);
     throw astrosim::Error(AS_ERR_INVALID, "this precision/method combination is not available");
 }
 

@@ -1,0 +1,226 @@
+// as_model32.h -- per-cell arithmetic of the FP32-compensated precision.
+//
+// State and right-hand sides are FP32 with hardware exp/log/reciprocal. The
+// astrocyte variables Ca, IP3 and h change by ~1e-6 of their value per step,
+// below FP32 resolution over thousands of steps, so each is carried as an
+// unevaluated hi+lo pair of floats and every increment is added with an
+// error-free TwoSum and renormalised (about 48 significant bits). The AdEx
+// spike current g_L*DeltaT*exp((V-V_T)/DeltaT) is evaluated in FP64 while
+// |V - V_T| < 3*DeltaT, where the spike onset is decided; far from V_T it is
+// either negligible or already runaway and FP32 suffices. Random draws stay
+// in FP64 (as_rng.h), so Poisson counts and Gaussian values are bit-identical
+// to the FP64 path. Formulas follow src/astrosim_fused.cu:236-290, 508-576.
+#pragma once
+
+#include "as_model.h"
+
+namespace astrosim {
+
+struct AstroConst32 {
+    float Kd_IP3_1, Kd_IP3_2, Kd_act, Kd_inh, Km_SERCA2;
+    float k_IP3R, rate_IP3R, rate_L, rate_SERCA, ratio_ER_cyt, inv_ratio_ER_cyt;
+    float SIC_scale, SIC_th;
+};
+
+struct NeuronConst32 {
+    float inv_C_m, g_L, E_L, V_th, Delta_T, inv_Delta_T, a, inv_tau_w, V_peak, E_ex, E_in, I_e;
+    float psc_init_ex, psc_init_in, g_L_Delta_T, near_band;  // near_band = 3*Delta_T
+    int refractory_steps;                                   // int(t_ref/h + 0.5)
+    double V_th_d, inv_Delta_T_d, g_L_Delta_T_d;
+};
+
+struct Params32 {
+    AstroConst32 ac;
+    NeuronConst32 ne, ni;
+    float h_step, h_half, h_sixth, ex_decay_e, in_decay_e, ex_decay_i, in_decay_i;
+    float w_e_abs, w_i_abs, w_n2a, w_a2n, w_poiss_astro, w_poiss_exc, w_poiss_inh;
+    float noise_astro, noise_exc, noise_inh;
+};
+
+inline Params32 make_params32(const RunParams& P) {
+    Params32 q{};
+    const AstroConst& c = P.ac;
+    q.ac = {float(c.Kd_IP3_1), float(c.Kd_IP3_2), float(c.Kd_act), float(c.Kd_inh),
+            float(c.Km_SERCA * c.Km_SERCA), float(c.k_IP3R), float(c.rate_IP3R), float(c.rate_L),
+            float(c.rate_SERCA), float(c.ratio_ER_cyt), float(1.0 / c.ratio_ER_cyt), float(c.SIC_scale),
+            float(c.SIC_th)};
+    auto nc = [&](const NeuronConst& p) {
+        NeuronConst32 n{};
+        n.inv_C_m = float(1.0 / p.C_m);
+        n.g_L = float(p.g_L);
+        n.E_L = float(p.E_L);
+        n.V_th = float(p.V_th);
+        n.Delta_T = float(p.Delta_T);
+        n.inv_Delta_T = float(1.0 / p.Delta_T);
+        n.a = float(p.a);
+        n.inv_tau_w = float(1.0 / p.tau_w);
+        n.V_peak = float(p.V_peak);
+        n.E_ex = float(p.E_ex);
+        n.E_in = float(p.E_in);
+        n.I_e = float(p.I_e);
+        n.psc_init_ex = float(p.psc_init_ex);
+        n.psc_init_in = float(p.psc_init_in);
+        n.g_L_Delta_T = float(p.g_L * p.Delta_T);
+        n.near_band = float(3.0 * p.Delta_T);
+        n.refractory_steps = static_cast<int>(p.t_ref / P.h_step + 0.5);
+        n.V_th_d = p.V_th;
+        n.inv_Delta_T_d = 1.0 / p.Delta_T;
+        n.g_L_Delta_T_d = p.g_L * p.Delta_T;
+        return n;
+    };
+    q.ne = nc(P.ne);
+    q.ni = nc(P.ni);
+    q.h_step = float(P.h_step);
+    q.h_half = float(0.5 * P.h_step);
+    q.h_sixth = float(P.h_step / 6.0);
+    q.ex_decay_e = float(P.ex_decay_e);
+    q.in_decay_e = float(P.in_decay_e);
+    q.ex_decay_i = float(P.ex_decay_i);
+    q.in_decay_i = float(P.in_decay_i);
+    q.w_e_abs = float(P.w_e);
+    q.w_i_abs = float(-P.w_i);
+    q.w_n2a = float(P.w_n2a);
+    q.w_a2n = float(P.w_a2n);
+    q.w_poiss_astro = float(P.w_poiss_astro);
+    q.w_poiss_exc = float(P.w_poiss_exc);
+    q.w_poiss_inh = float(P.w_poiss_inh);
+    q.noise_astro = float(P.noise_astro);
+    q.noise_exc = float(P.noise_exc);
+    q.noise_inh = float(P.noise_inh);
+    return q;
+}
+
+#if defined(__HIPCC__)
+
+__device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// hi+lo += x, error-free (TwoSum) then renormalised (Fast2Sum).
+__device__ __forceinline__ void comp_add(float& hi, float& lo, float x) {
+    const float s = hi + x;
+    const float bp = s - hi;
+    const float err = (hi - (s - bp)) + (x - bp);
+    const float l = lo + err;
+    const float h2 = s + l;
+    lo = l - (h2 - s);
+    hi = h2;
+}
+
+// Li-Rinzel right-hand side in FP32 (src/astrosim_fused.cu:236-253).
+__device__ __forceinline__ void astro_rhs32(const AstroConst32& c, float Ca_tot, float IP3_0, float inv_tau,
+                                            float Ca, float IP3, float h, float noise, float& dCa, float& dIP3,
+                                            float& dh) {
+    const float Ca_ER = (Ca_tot - Ca) * c.inv_ratio_ER_cyt;
+    const float m_inf = IP3 * rcp32(IP3 + c.Kd_IP3_1);
+    const float n_inf = Ca * rcp32(Ca + c.Kd_act);
+    const float m3 = m_inf * m_inf * m_inf;
+    const float n3 = n_inf * n_inf * n_inf;
+    const float h3 = h * h * h;
+    const float J_channel = c.ratio_ER_cyt * c.rate_IP3R * m3 * n3 * h3 * (Ca_ER - Ca);
+    const float Ca2 = Ca * Ca;
+    const float J_pump = c.rate_SERCA * Ca2 * rcp32(c.Km_SERCA2 + Ca2);
+    const float J_leak = c.ratio_ER_cyt * c.rate_L * (Ca_ER - Ca);
+    const float alpha_h = c.k_IP3R * c.Kd_inh * (IP3 + c.Kd_IP3_1) * rcp32(IP3 + c.Kd_IP3_2);
+    const float beta_h = c.k_IP3R * Ca;
+    dCa = J_channel - J_pump + J_leak + noise;
+    dIP3 = (IP3_0 - IP3) * inv_tau;
+    dh = alpha_h * (1.0f - h) - beta_h * h;
+}
+
+// IP3 jump, RK4 substeps and clamps with compensated state
+// (src/astrosim_fused.cu:255-278).
+__device__ __forceinline__ void astro_advance32(const AstroConst32& c, const Params32& q, float Ca_tot,
+                                                float IP3_0, float inv_tau, float delta_IP3, float ip3_input,
+                                                float noise, int substeps, float& Ca, float& Ca_lo, float& IP3,
+                                                float& IP3_lo, float& h, float& h_lo) {
+    if (ip3_input != 0.0f) comp_add(IP3, IP3_lo, delta_IP3 * ip3_input);
+    const float hs = q.h_step, hh = q.h_half;
+    for (int s = 0; s < substeps; ++s) {
+        float k1C, k1I, k1h, k2C, k2I, k2h, k3C, k3I, k3h, k4C, k4I, k4h;
+        astro_rhs32(c, Ca_tot, IP3_0, inv_tau, Ca, IP3, h, noise, k1C, k1I, k1h);
+        astro_rhs32(c, Ca_tot, IP3_0, inv_tau, Ca + hh * k1C, IP3 + hh * k1I, h + hh * k1h, noise, k2C, k2I, k2h);
+        astro_rhs32(c, Ca_tot, IP3_0, inv_tau, Ca + hh * k2C, IP3 + hh * k2I, h + hh * k2h, noise, k3C, k3I, k3h);
+        astro_rhs32(c, Ca_tot, IP3_0, inv_tau, Ca + hs * k3C, IP3 + hs * k3I, h + hs * k3h, noise, k4C, k4I, k4h);
+        comp_add(Ca, Ca_lo, q.h_sixth * (k1C + 2.0f * k2C + 2.0f * k3C + k4C));
+        comp_add(IP3, IP3_lo, q.h_sixth * (k1I + 2.0f * k2I + 2.0f * k3I + k4I));
+        comp_add(h, h_lo, q.h_sixth * (k1h + 2.0f * k2h + 2.0f * k3h + k4h));
+        if (Ca < 0.0f) { Ca = 0.0f; Ca_lo = 0.0f; }
+        else if (Ca > Ca_tot) { Ca = Ca_tot; Ca_lo = 0.0f; }
+        if (h < 0.0f) { h = 0.0f; h_lo = 0.0f; }
+        else if (h > 1.0f) { h = 1.0f; h_lo = 0.0f; }
+        if (IP3 < 0.0f) { IP3 = 0.0f; IP3_lo = 0.0f; }
+    }
+}
+
+__device__ __forceinline__ float sic_factor32(const AstroConst32& c, float Ca, float Ca_lo) {
+    const float y = ((Ca - c.SIC_th) + Ca_lo) * 1000.0f;
+    return (y <= 1.0f) ? 0.0f : c.SIC_scale * __logf(y);
+}
+
+// AdEx right-hand side in FP32, FP64 exponential near threshold
+// (src/astrosim_fused.cu:281-290).
+__device__ __forceinline__ void neuron_rhs32(const NeuronConst32& p, float V, float w, float g_ex, float g_in,
+                                             float I_ext, float& dV, float& dw) {
+    const float Vc = V < p.V_peak ? V : p.V_peak;
+    const float x = Vc - p.V_th;
+    float I_spike;
+    if (x > -p.near_band && x < p.near_band)
+        I_spike = static_cast<float>(p.g_L_Delta_T_d * as_exp((static_cast<double>(Vc) - p.V_th_d) * p.inv_Delta_T_d));
+    else
+        I_spike = p.g_L_Delta_T * __expf(x * p.inv_Delta_T);
+    dV = (-p.g_L * (Vc - p.E_L) + I_spike - g_ex * (Vc - p.E_ex) - g_in * (Vc - p.E_in) - w + p.I_e + I_ext) *
+         p.inv_C_m;
+    dw = (p.a * (Vc - p.E_L) - w) * p.inv_tau_w;
+}
+
+struct NeuronRegs32 {
+    float V, w, g_ex, dg_ex, g_in, dg_in;
+    int refractory;
+};
+
+// Substep loop with exact alpha-cascade propagator, refractory clamp, RK4,
+// spike and reset (src/astrosim_fused.cu:514-553).
+__device__ __forceinline__ bool neuron_substeps32(const NeuronConst32& p, const Params32& q, NeuronRegs32& r,
+                                                  float I_ext, float V_reset, float b, int substeps,
+                                                  float ex_decay, float in_decay) {
+    bool spiked = false;
+    const float hs = q.h_step, hh = q.h_half;
+    for (int sub = 0; sub < substeps; ++sub) {
+        const float g_ex_next = ex_decay * (r.g_ex + hs * r.dg_ex);
+        const float dg_ex_next = ex_decay * r.dg_ex;
+        const float g_in_next = in_decay * (r.g_in + hs * r.dg_in);
+        const float dg_in_next = in_decay * r.dg_in;
+        if (r.refractory > 0) {
+            r.V = V_reset;
+            float dV, dw;
+            neuron_rhs32(p, r.V, r.w, r.g_ex, r.g_in, I_ext, dV, dw);
+            r.w += hs * dw;
+        } else {
+            const float g_ex_mid = 0.5f * (r.g_ex + g_ex_next);
+            const float g_in_mid = 0.5f * (r.g_in + g_in_next);
+            float k1V, k1w, k2V, k2w, k3V, k3w, k4V, k4w;
+            neuron_rhs32(p, r.V, r.w, r.g_ex, r.g_in, I_ext, k1V, k1w);
+            neuron_rhs32(p, r.V + hh * k1V, r.w + hh * k1w, g_ex_mid, g_in_mid, I_ext, k2V, k2w);
+            neuron_rhs32(p, r.V + hh * k2V, r.w + hh * k2w, g_ex_mid, g_in_mid, I_ext, k3V, k3w);
+            neuron_rhs32(p, r.V + hs * k3V, r.w + hs * k3w, g_ex_next, g_in_next, I_ext, k4V, k4w);
+            r.V += q.h_sixth * (k1V + 2.0f * k2V + 2.0f * k3V + k4V);
+            r.w += q.h_sixth * (k1w + 2.0f * k2w + 2.0f * k3w + k4w);
+        }
+        r.g_ex = g_ex_next;
+        r.dg_ex = dg_ex_next;
+        r.g_in = g_in_next;
+        r.dg_in = dg_in_next;
+        if (r.refractory > 0) {
+            --r.refractory;
+        } else if (r.V >= p.V_peak) {
+            r.V = V_reset;
+            r.w += b;
+            r.refractory = p.refractory_steps;
+            spiked = true;
+        }
+    }
+    return spiked;
+}
+
+#endif  // __HIPCC__
+
+}  // namespace astrosim

@@ -111,7 +111,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=10_000_000)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--legs", default="fp64:plain,fp64:blocked")
+    ap.add_argument("--legs", default="fp64:plain,fp64:blocked,fp32c:blocked")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

@@ -1,0 +1,76 @@
+"""FP32-compensated precision against the FP64 path. The FP64 blocked path
+is bit-identical to the FP64 host oracle (test_gpu_parity.py), so it stands
+in for the oracle at sizes the host cannot run in seconds.
+
+Tolerances: over the first steps, before spike-time divergence, states agree
+to FP32 rounding (relative 1e-5 for V, w; 1e-7 for the compensated Ca, IP3,
+h); over a full config-2 run (N_total=1M, seed 7, pre 200 ms, sim 1000 ms)
+the recorded mean firing rate agrees within 1% and the fractions of
+astrocytes above the SIC threshold (final step and any time in the window)
+within 0.01."""
+import numpy as np
+import pytest
+
+import astrosimgpu_cuda_amd as A
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b, floor):
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))
+
+
+def test_fp32c_tracks_fp64_over_first_steps():
+    print("[fp32c] n=20000 seed 5, 8 steps", flush=True)
+    net = A.Network(20000, 5)
+    s64 = A.Simulation(net, precision="fp64", method="blocked")
+    s32 = A.Simulation(net, precision="fp32c", method="blocked")
+    s64.step(8)
+    s32.step(8)
+    r64, r32 = s64.raster(0, 8), s32.raster(0, 8)
+    agree = np.mean(r64 == r32)
+    assert agree > 0.9999, agree
+    same = ~np.any(r64 != r32, axis=0)  # neurons whose spike times agree
+    for f, tol, floor in (("V", 1e-5, 1.0), ("w", 1e-4, 1.0), ("g_ex", 1e-4, 1e-3), ("g_in", 1e-4, 1e-3)):
+        d = rel(s32.state(f)[same], s64.state(f)[same], floor)
+        print(f"[fp32c] {f} max rel diff {d:.3e}", flush=True)
+        assert d < tol, (f, d)
+    for f, tol, floor in (("Ca", 1e-7, 1e-3), ("IP3", 1e-7, 1e-3), ("h", 1e-7, 1e-3)):
+        d = rel(s32.state(f), s64.state(f), floor)
+        print(f"[fp32c] {f} max rel diff {d:.3e}", flush=True)
+        assert d < tol, (f, d)
+
+
+def run_summary(net, precision, pre_steps, sim_steps):
+    sim = A.Simulation(net, precision=precision, method="blocked")
+    sim.step(1000)  # 100 ms: keep the state for the error report
+    early = {f: sim.state(f) for f in ("V", "Ca", "IP3", "h")}
+    sim.step(pre_steps - 1000)
+    before = sim.spike_count()
+    sim.reset_ever_active()
+    sim.step(sim_steps)
+    spikes = sim.spike_count() - before
+    ca = sim.state("Ca")
+    active = np.mean((ca - 0.19669) * 1000.0 > 1.0)
+    engaged = np.mean(sim.ever_active())
+    sim.close()
+    return spikes, active, engaged, early
+
+
+def test_fp32c_config2_statistics_match_fp64():
+    n, seed, pre_ms, sim_ms, dt = 1_000_000, 7, 200.0, 1000.0, 0.1
+    print(f"[fp32c] config 2: n={n} seed={seed} pre={pre_ms} sim={sim_ms}", flush=True)
+    net = A.Network(n, seed)
+    pre, sim = int(pre_ms / dt + 0.5), int(sim_ms / dt + 0.5)
+    sp64, act64, eng64, e64 = run_summary(net, "fp64", pre, sim)
+    sp32, act32, eng32, e32 = run_summary(net, "fp32c", pre, sim)
+    rate64 = sp64 / (net.n_neurons * sim_ms * 1e-3)
+    rate32 = sp32 / (net.n_neurons * sim_ms * 1e-3)
+    print(f"[fp32c] rate fp64 {rate64:.4f} Hz fp32c {rate32:.4f} Hz (rel {rate32 / rate64 - 1:+.4%}); "
+          f"active {act64:.4f} vs {act32:.4f}; engaged {eng64:.4f} vs {eng32:.4f}", flush=True)
+    for f in ("V", "Ca", "IP3", "h"):
+        d = np.abs(e32[f] - e64[f])
+        print(f"[fp32c] state at 100 ms {f}: median |diff| {np.median(d):.3e} max {d.max():.3e}", flush=True)
+    assert abs(rate32 / rate64 - 1.0) < 0.01
+    assert abs(act32 - act64) < 0.01
+    assert abs(eng32 - eng64) < 0.01
