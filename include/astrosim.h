@@ -122,9 +122,11 @@ as_status as_sim_read_ever_active(as_sim* sim, uint8_t* out);
 as_status as_sim_read_raster(as_sim* sim, int64_t first_step, int32_t n_steps, uint8_t* out);
 /* Device bytes held by this simulation. */
 uint64_t as_sim_bytes(const as_sim* sim);
-/* Kernel timing: when enabled, every step kernel launch is bracketed by HIP
- * events on the simulation's stream. as_sim_kernel_time returns the summed
- * kernel milliseconds and launch count since enabling (synchronises). */
+/* Kernel timing: when enabled, every launch of the method's dominant step
+ * kernel (plain: the fused step kernel, one launch per step; blocked: the
+ * integrate kernel, one launch per block or partial block) is bracketed by HIP events on the simulation's stream.
+ * as_sim_kernel_time returns the summed milliseconds and launch count of the
+ * timed launches since the previous read (synchronises). */
 as_status as_sim_set_timing(as_sim* sim, int enable);
 as_status as_sim_kernel_time(as_sim* sim, double* total_ms, int64_t* launches);
 void as_sim_destroy(as_sim* sim);
