@@ -21,7 +21,7 @@ import numpy as np
 
 __all__ = [
     "AstrosimError", "LibraryNotBuilt", "Network", "Simulation", "lib_path", "cli_path",
-    "device_info", "device_memory", "FIELDS", "FP64", "FP32C", "PLAIN", "BLOCKED", "load",
+    "device_info", "device_memory", "FIELDS", "FP64", "FP32C", "PLAIN", "BLOCKED", "load", "Batch",
 ]
 
 PKG = Path(__file__).resolve().parent
@@ -66,6 +66,17 @@ class _NetInfo(C.Structure):
                 ("build_seconds", C.c_double)]
 
 
+class _BatchCap(C.Structure):
+    _fields_ = [("neurons", C.c_uint64), ("astros", C.c_uint64), ("edges", C.c_uint64 * 4),
+                ("members", C.c_uint32)]
+
+
+class _BatchInfo(C.Structure):
+    _fields_ = [("global_step", C.c_int64), ("block_steps", C.c_int32), ("active_members", C.c_int32),
+                ("block_threads", C.c_int32), ("device_bytes", C.c_uint64), ("free_neurons", C.c_uint64),
+                ("bucket_overflows", C.c_uint64)]
+
+
 _lib = None
 
 # (name, restype, argtypes) for every symbol include/astrosim.h declares.
@@ -92,6 +103,27 @@ SIGNATURES = [
     ("as_sim_set_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("as_sim_kernel_time", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     ("as_sim_destroy", None, [C.c_void_p]),
+    ("as_batch_capacity_for", C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(_BatchCap)]),
+    ("as_batch_footprint", C.c_int, [C.POINTER(_BatchCap), C.POINTER(_Config), C.POINTER(C.c_uint64)]),
+    ("as_batch_create", C.c_int, [C.POINTER(_BatchCap), C.POINTER(_Config), C.POINTER(C.c_void_p)]),
+    ("as_batch_fits", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]),
+    ("as_batch_add", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]),
+    ("as_batch_remove", C.c_int, [C.c_void_p, C.c_int32]),
+    ("as_batch_step", C.c_int, [C.c_void_p, C.c_int64]),
+    ("as_batch_run_graph", C.c_int, [C.c_void_p, C.c_int64]),
+    ("as_batch_autotune", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                                    C.POINTER(C.c_int32)]),
+    ("as_batch_sync", C.c_int, [C.c_void_p]),
+    ("as_batch_get_info", C.c_int, [C.c_void_p, C.POINTER(_BatchInfo)]),
+    ("as_batch_member_steps", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    ("as_batch_spike_count", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
+    ("as_batch_reset_ever_active", C.c_int, [C.c_void_p, C.c_int32]),
+    ("as_batch_read_state", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
+    ("as_batch_read_ever_active", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    ("as_batch_read_raster", C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_int32, C.c_void_p]),
+    ("as_batch_set_timing", C.c_int, [C.c_void_p, C.c_int]),
+    ("as_batch_kernel_time", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    ("as_batch_destroy", None, [C.c_void_p]),
     ("as_device_info", C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
     ("as_device_memory", C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("as_last_error", C.c_char_p, []),
@@ -296,6 +328,135 @@ class Simulation:
     def close(self) -> None:
         if self._h is not None:
             load().as_sim_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many networks advanced together in one launch per block (blocked
+    method). Members join and leave at block boundaries; each member's
+    results are identical to running it alone."""
+
+    def __init__(self, capacity: "_BatchCap | list[Network]", **kw):
+        lib = load()
+        kw.setdefault("method", "blocked")
+        self.cfg = make_config(**kw)
+        cap = capacity if isinstance(capacity, _BatchCap) else Batch.capacity_for(capacity)
+        self.capacity = cap
+        h = C.c_void_p()
+        _check(lib.as_batch_create(C.byref(cap), C.byref(self.cfg), C.byref(h)))
+        self._h = h
+        self._sizes = {}
+
+    @staticmethod
+    def capacity_for(nets: list, members: int | None = None) -> _BatchCap:
+        cap = _BatchCap()
+        arr = (C.c_void_p * max(1, len(nets)))(*[n.handle.value for n in nets])
+        _check(load().as_batch_capacity_for(arr, len(nets), C.byref(cap)))
+        if members is not None:
+            cap.members = members
+        return cap
+
+    @staticmethod
+    def footprint(cap: _BatchCap, **kw) -> int:
+        kw.setdefault("method", "blocked")
+        cfg = make_config(**kw)
+        out = C.c_uint64()
+        _check(load().as_batch_footprint(C.byref(cap), C.byref(cfg), C.byref(out)))
+        return out.value
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("batch was destroyed")
+        return self._h
+
+    def fits(self, net: Network) -> bool:
+        f = C.c_int32()
+        _check(load().as_batch_fits(self.handle, net.handle, C.byref(f)))
+        return bool(f.value)
+
+    def add(self, net: Network, n_steps: int) -> int:
+        m = C.c_int32()
+        _check(load().as_batch_add(self.handle, net.handle, C.c_int64(n_steps), C.byref(m)))
+        self._sizes[m.value] = (net.n_neurons, net.n_astro)
+        return m.value
+
+    def remove(self, member: int) -> None:
+        _check(load().as_batch_remove(self.handle, member))
+
+    def step(self, k: int) -> None:
+        _check(load().as_batch_step(self.handle, C.c_int64(k)))
+
+    def run_graph(self, blocks: int) -> None:
+        _check(load().as_batch_run_graph(self.handle, C.c_int64(blocks)))
+
+    def autotune(self, candidates=(64, 128, 256, 512), blocks_each: int = 2) -> int:
+        arr = (C.c_int32 * len(candidates))(*candidates)
+        out = C.c_int32()
+        _check(load().as_batch_autotune(self.handle, arr, len(candidates), blocks_each, C.byref(out)))
+        return out.value
+
+    def sync(self) -> None:
+        _check(load().as_batch_sync(self.handle))
+
+    def info(self) -> dict:
+        i = _BatchInfo()
+        _check(load().as_batch_get_info(self.handle, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in _BatchInfo._fields_}
+
+    def member_steps(self, member: int) -> tuple[int, int]:
+        d, t = C.c_int64(), C.c_int64()
+        _check(load().as_batch_member_steps(self.handle, member, C.byref(d), C.byref(t)))
+        return d.value, t.value
+
+    def spike_count(self, member: int) -> int:
+        out = C.c_uint64()
+        _check(load().as_batch_spike_count(self.handle, member, C.byref(out)))
+        return out.value
+
+    def reset_ever_active(self, member: int) -> None:
+        _check(load().as_batch_reset_ever_active(self.handle, member))
+
+    def state(self, member: int, field: str) -> np.ndarray:
+        nn, na = self._sizes[member]
+        out = np.empty(na if field in ASTRO_FIELDS else nn, dtype=np.float64)
+        _check(load().as_batch_read_state(self.handle, member, FIELDS[field], _ptr(out)))
+        return out
+
+    def ever_active(self, member: int) -> np.ndarray:
+        out = np.empty(self._sizes[member][1], dtype=np.uint8)
+        _check(load().as_batch_read_ever_active(self.handle, member, _ptr(out)))
+        return out
+
+    def raster(self, member: int, first_step: int, n_steps: int) -> np.ndarray:
+        out = np.empty((n_steps, self._sizes[member][0]), dtype=np.uint8)
+        _check(load().as_batch_read_raster(self.handle, member, C.c_int64(first_step), C.c_int32(n_steps),
+                                           _ptr(out)))
+        return out
+
+    def set_timing(self, on: bool) -> None:
+        _check(load().as_batch_set_timing(self.handle, 1 if on else 0))
+
+    def kernel_time(self) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_int64()
+        _check(load().as_batch_kernel_time(self.handle, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def close(self) -> None:
+        if self._h is not None:
+            load().as_batch_destroy(self._h)
             self._h = None
 
     def __enter__(self):

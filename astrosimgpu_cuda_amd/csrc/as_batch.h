@@ -1,0 +1,73 @@
+// as_batch.h -- the blocked, event-driven engine over a batch of networks.
+//
+// A batch owns one device arena with fixed capacities (neurons, astrocytes,
+// edges per connection type, members). Each member network occupies
+// contiguous ranges of those spaces; ids are rebased to global ones, and a
+// per-workgroup tile table maps each workgroup to one member. Members may be
+// added and removed at block boundaries while others run (continuous
+// batching); results of a member are bit-identical to running it alone.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+
+#include "as_engine.h"
+
+namespace astrosim {
+
+struct BatchCapacity {
+    std::uint64_t neurons = 0, astros = 0;
+    std::uint64_t edges[4] = {0, 0, 0, 0};  // exc, inh, sic, e2a
+    std::uint32_t members = 1;
+};
+
+// Exact capacity for the given networks side by side.
+BatchCapacity capacity_for(const Network* const* nets, int n);
+
+class Batch {
+public:
+    virtual ~Batch() = default;
+    // Whether `net` fits in the free ranges now.
+    virtual bool fits(const Network& net) const = 0;
+    // Admits `net` to run `n_steps` steps starting at the current global
+    // step, which must be a block boundary. Returns the member slot.
+    virtual int add(const Network& net, std::int64_t n_steps) = 0;
+    // Retires a member; its ranges are reused once no in-flight block can
+    // still deliver its emissions.
+    virtual void remove(int member) = 0;
+    // Advances every active member k steps (any k; partial blocks allowed).
+    virtual void enqueue_steps(std::int64_t k) = 0;
+    // Advances whole blocks through a captured hipGraph (global step must be
+    // a block boundary). The graph is captured once per membership and block
+    // count and replayed afterwards.
+    virtual void run_blocks_graph(std::int64_t blocks) = 0;
+    // Times candidate workgroup sizes on the next blocks of the live run and
+    // keeps the fastest (results do not depend on it). Returns the choice.
+    virtual int autotune(const int* candidates, int n, int blocks_each) = 0;
+    virtual int block_threads() const = 0;
+
+    virtual std::uint64_t spike_count(int member) = 0;
+    virtual void reset_ever_active(int member) = 0;
+    virtual void read_state(int member, int field, double* out) = 0;
+    virtual void read_refractory(int member, std::int32_t* out) = 0;
+    virtual void read_ever_active(int member, std::uint8_t* out) = 0;
+    virtual void read_raster(int member, std::int64_t first_local, int n, std::uint8_t* out) = 0;
+    virtual std::int64_t member_steps(int member) const = 0;  // local steps enqueued
+    virtual std::int64_t member_target(int member) const = 0;
+    virtual bool member_active(int member) const = 0;
+    virtual int active_members() const = 0;
+    virtual std::uint64_t member_overflows() = 0;  // bucket overflows seen (diagnostic)
+
+    virtual std::int64_t global_step() const = 0;
+    virtual int block_len() const = 0;
+    virtual hipStream_t stream() const = 0;
+    virtual std::uint64_t device_bytes() const = 0;
+    virtual std::uint64_t free_neurons() const = 0;
+    virtual void set_timing(bool on) = 0;
+    virtual void kernel_time(double* total_ms, std::int64_t* launches) = 0;
+};
+
+std::uint64_t batch_footprint(const BatchCapacity& cap, const RunParams& rp);
+std::unique_ptr<Batch> make_batch(const RunParams& rp, const BatchCapacity& cap, hipStream_t s);
+
+}  // namespace astrosim

@@ -1,0 +1,1243 @@
+// as_batch.hip -- the blocked, event-driven method over a batch of networks.
+//
+// Every synaptic delay is the same d >= 1 steps (1 ms at dt = 0.1 ms), so
+// within an aligned block of L steps (L divides d, L <= 32) no cell reads
+// anything written in the same block: all inputs of block b come from block
+// b - d/L. One INTEGRATE launch therefore advances every cell of every
+// member network a whole block with its state in registers.
+//
+// Inputs are delivered event-driven instead of by probing every in-edge:
+//   * each cell records the steps of its block at which it emitted (a spike,
+//     or a nonzero SIC factor) as a bit word, the emitted values in a ring
+//     indexed by absolute step, and appends itself to its block's emitter
+//     list when the word is nonzero;
+//   * a SCATTER launch walks only the emitters' out-edges (a transposed copy
+//     of the in-CSR built on the device) and appends (in-CSR slot, source)
+//     to the target's fixed-capacity bucket for that connection type;
+//   * INTEGRATE sums a target's bucket in ascending in-CSR slot order, which
+//     is the reference's CSR-order sum with its exact-zero terms removed
+//     (src/astrosim_fused.cu:384-389, 450-463, 482-490), so FP64 results are
+//     bit-identical to the plain method and to the host oracle. A bucket that
+//     overflowed falls back to walking the whole in-row.
+//
+// Members occupy contiguous ranges of the arena's cell and edge spaces with
+// rebased (global) ids. Workgroups are tiles of one member's astrocytes or
+// neurons; every random stream uses the member's seed, local cell index and
+// local step, so a member's results do not depend on its batch mates.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <limits>
+#include <map>
+#include <vector>
+
+#include "as_batch.h"
+#include "as_model.h"
+#include "as_model32.h"
+
+namespace astrosim {
+namespace {
+
+constexpr unsigned kBucketCap = 8;
+constexpr int kCountersPerMember = 8;
+enum { T_EXC = 0, T_INH = 1, T_SIC = 2, T_E2A = 3 };
+
+struct Member {
+    std::uint64_t seed;
+    index_t nb, nn, ne;  // neuron base (global), neurons, excitatory neurons
+    index_t ab, na;      // astrocyte base (global), astrocytes
+    std::int64_t step0, step_end;  // global steps [step0, step_end) are simulated
+    std::uint32_t slot;            // member slot (spike counters)
+    std::uint32_t pad;
+};
+
+struct Tile {
+    std::uint32_t member, first, kind, pad;  // kind 0: astrocytes, 1: neurons
+};
+
+// Everything that does not depend on the state precision.
+struct BatchDev {
+    const uint2* in_rs[4];  // (begin, end) per target: neurons (exc, inh, sic), astrocytes (e2a)
+    const index_t* in_src[4];
+    const uint2* out_rs[4];  // (begin, end) per source: neurons (exc, inh, e2a), astrocytes (sic)
+    const uint2* out_ent[4];  // (global target, global in-CSR slot)
+    unsigned* bcnt[4];
+    uint2* bent[4];
+    std::uint8_t* ever;
+    std::int32_t* refr;
+    std::uint32_t* spk_w;  // [H][ncap]
+    std::uint32_t* sic_w;  // [H][acap]
+    std::uint32_t* nlist;  // [H][ncap]
+    std::uint32_t* alist;  // [H][acap]
+    std::uint32_t* lcount; // [H][2]
+    unsigned long long* counters;  // [members][kCountersPerMember]
+    unsigned* overflow;
+    const Member* members;
+    const Tile* tiles;
+    std::int64_t* clock;  // block index for graph replays
+    std::uint64_t ncap, acap;
+};
+
+// State arrays in the precision R (double: FP64; float: FP32c, where
+// Ca/IP3/h carry compensation terms and tau_IP3 holds 1/tau_IP3).
+template <class R>
+struct BlkState {
+    R *Ca, *IP3, *h;
+    R *Ca_lo, *IP3_lo, *h_lo;
+    R *Ca_tot, *IP3_0, *tau_IP3, *delta_IP3;
+    R *V, *w, *g_ex, *dg_ex, *g_in, *dg_in, *I_sic;
+    R *V_reset, *b;
+    R *stp_x, *stp_u, *stp_t;
+    R* spk_v;  // [Rv][ncap] x*u at the spike step
+    R* sic_v;  // [Rv][acap] SIC factor where nonzero
+};
+
+struct BlkLaunch {
+    std::int64_t t0, bstart;
+    int n;             // steps in this launch
+    int d, L, H, Rv;   // delay, block length, history blocks, value-ring depth
+    std::uint32_t wslot, rslot;  // history slots of block b and block b - d/L
+    int first, last;   // the launch starts / completes its block
+};
+
+// Graph replays read the block index from device memory instead of the
+// launch arguments: the launch is a whole block.
+__device__ __forceinline__ void resolve_launch(BlkLaunch& B, const std::int64_t* clock) {
+    if (!clock) return;
+    const std::int64_t b = *clock;
+    B.bstart = b * B.L;
+    B.t0 = B.bstart;
+    B.n = B.L;
+    B.first = 1;
+    B.last = 1;
+    B.wslot = static_cast<std::uint32_t>(b % B.H);
+    B.rslot = static_cast<std::uint32_t>(((b - B.d / B.L) % B.H + B.H) % B.H);
+}
+
+__device__ __forceinline__ unsigned lane_id() { return threadIdx.x & 63u; }
+
+// Appends `id` (when `want`) to a list with one atomic per wave.
+__device__ __forceinline__ void wave_append(bool want, std::uint32_t id, std::uint32_t* list,
+                                            std::uint32_t* count) {
+    const unsigned long long m = __ballot(want);
+    if (m == 0ull) return;
+    const unsigned leader = __ffsll((long long)m) - 1;
+    std::uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(count, (std::uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (want) list[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = id;
+}
+
+// Delayed-input sum of one target for block step k: the reference's in-row
+// sum restricted to its nonzero terms, in ascending in-CSR slot order.
+template <class R, class WordF, class ValF>
+__device__ R input_sum(const uint2* ent, unsigned cnt, uint2 row, const index_t* src, unsigned k, WordF word,
+                       ValF val) {
+    R s = R(0);
+    if (cnt > kBucketCap) {  // overflow: walk the whole row
+        for (index_t e = row.x; e < row.y; ++e) {
+            const index_t q = src[e];
+            if ((word(q) >> k) & 1u) s += val(q);
+        }
+        return s;
+    }
+    unsigned last = 0;
+    for (unsigned r = 0; r < cnt; ++r) {
+        unsigned best = 0xFFFFFFFFu, bsrc = 0;
+        for (unsigned q = 0; q < cnt; ++q) {
+            const uint2 x = ent[q];
+            if ((r == 0 || x.x > last) && x.x < best) {
+                best = x.x;
+                bsrc = x.y;
+            }
+        }
+        last = best;
+        if ((word(bsrc) >> k) & 1u) s += val(bsrc);
+    }
+    return s;
+}
+
+// Union of the emission bits of a target's inputs (all ones on overflow).
+template <class WordF>
+__device__ std::uint32_t input_mask(const uint2* ent, unsigned cnt, WordF word) {
+    if (cnt > kBucketCap) return 0xFFFFFFFFu;
+    std::uint32_t m = 0;
+    for (unsigned q = 0; q < cnt; ++q) m |= word(ent[q].y);
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// SCATTER: emitters of block b - d/L append to their targets' buckets.
+// One wave per emitter, lanes over its out-edges.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void scatter_kernel(BatchDev D, BlkLaunch B) {
+    resolve_launch(B, D.clock);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        D.lcount[B.wslot * 2 + 0] = 0;
+        D.lcount[B.wslot * 2 + 1] = 0;
+    }
+    const std::uint32_t nN = D.lcount[B.rslot * 2 + 0];
+    const std::uint32_t nA = D.lcount[B.rslot * 2 + 1];
+    const unsigned waves = gridDim.x * (blockDim.x / 64);
+    const unsigned wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    for (std::uint32_t i = wave; i < nN + nA; i += waves) {
+        const bool is_neuron = i < nN;
+        const std::uint32_t src = is_neuron ? D.nlist[std::size_t(B.rslot) * D.ncap + i]
+                                            : D.alist[std::size_t(B.rslot) * D.acap + (i - nN)];
+        // A neuron walks its exc and n2a out-edges and the inhibitory rows
+        // whose stored LOCAL id equals its own local id (src/astrosim_fused.cu:
+        // 459-462); an astrocyte walks its a2n out-edges.
+        for (int pass = 0; pass < (is_neuron ? 3 : 1); ++pass) {
+            const int T = is_neuron ? (pass == 0 ? T_EXC : (pass == 1 ? T_INH : T_E2A)) : T_SIC;
+            const uint2 r = D.out_rs[T][src];
+            for (index_t e = r.x + lane_id(); e < r.y; e += 64) {
+                const uint2 te = D.out_ent[T][e];  // (target, in-CSR slot)
+                const unsigned slot = atomicAdd(&D.bcnt[T][te.x], 1u);
+                if (slot < kBucketCap) D.bent[T][std::size_t(te.x) * kBucketCap + slot] = make_uint2(te.y, src);
+                else if (slot == kBucketCap) atomicAdd(D.overflow, 1u);
+            }
+        }
+    }
+}
+
+__global__ void tick_kernel(std::int64_t* clock) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *clock += 1;
+}
+
+// ---------------------------------------------------------------------------
+// INTEGRATE: advance every cell over steps [t0, t0 + n) of one block.
+// ---------------------------------------------------------------------------
+
+// Value-ring row of the delayed step t - d (dereferenced only when a source's
+// bit is set, which implies t >= d).
+__device__ __forceinline__ std::size_t value_row(std::int64_t t, const BlkLaunch& B) {
+    const std::int64_t tv = t - B.d;
+    return tv >= 0 ? std::size_t(tv % B.Rv) : 0;
+}
+
+// One astrocyte (global id ga, local id a) over global steps [lo, hi)
+// (src/astrosim_fused.cu:374-427). Returns whether it joins the emitter list.
+template <class R>
+__device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
+                           const Member& M, const BlkLaunch& B, index_t ga, index_t a, std::int64_t lo,
+                           std::int64_t hi) {
+    const std::size_t rN = std::size_t(B.rslot) * D.ncap;
+    const unsigned cnt = D.bcnt[T_E2A][ga];
+    const uint2* ent = D.bent[T_E2A] + std::size_t(ga) * kBucketCap;
+    auto word = [&](index_t q) { return D.spk_w[rN + q]; };
+    const std::uint32_t any = cnt ? input_mask(ent, cnt, word) : 0u;
+    R ca = S.Ca[ga], ip3 = S.IP3[ga], hh = S.h[ga];
+    R ca_lo = R(0), ip3_lo = R(0), hh_lo = R(0);
+    if constexpr (sizeof(R) == 4) {
+        ca_lo = S.Ca_lo[ga];
+        ip3_lo = S.IP3_lo[ga];
+        hh_lo = S.h_lo[ga];
+    }
+    const R Ca_tot = S.Ca_tot[ga], IP3_0 = S.IP3_0[ga], tau = S.tau_IP3[ga], dIP3 = S.delta_IP3[ga];
+    std::uint32_t wout = B.first ? 0u : D.sic_w[std::size_t(B.wslot) * D.acap + ga];
+    bool ever = false;
+    for (std::int64_t t = lo; t < hi; ++t) {
+        const unsigned k = static_cast<unsigned>(t - B.bstart);
+        const std::int64_t tl = t - M.step0;  // the member's own step
+        R gathered = R(0);
+        if ((any >> k) & 1u) {
+            const std::size_t vrow = value_row(t, B) * D.ncap;
+            gathered = input_sum<R>(ent, cnt, D.in_rs[T_E2A][ga], D.in_src[T_E2A], k, word,
+                                    [&](index_t q) { return S.spk_v[vrow + q]; });
+        }
+        const std::uint64_t stream = std::uint64_t(tl) * kStepStride + a;
+        int ev = 0;
+        if (P.poiss_astro) ev = rng_poisson(M.seed ^ kKeyAstroPoisson, stream, P.lam_astro, P.lim_astro);
+        const double noise = P.noise_astro > 0.0 ? P.noise_astro * rng_normal(M.seed ^ kKeyAstroNoise, stream) : 0.0;
+        R factor;
+        if constexpr (sizeof(R) == 8) {
+            double ip3_input = P.w_n2a * gathered;
+            if (ev > 0) ip3_input += P.w_poiss_astro * static_cast<double>(ev);
+            astro_advance(P.ac, Ca_tot, IP3_0, tau, dIP3, ip3_input, noise, P.h_step, P.substeps, ca, ip3, hh);
+            factor = sic_factor(P.ac, ca);
+        } else {
+            float ip3_input = Q.w_n2a * gathered;
+            if (ev > 0) ip3_input += Q.w_poiss_astro * static_cast<float>(ev);
+            astro_advance32(Q.ac, Q, Ca_tot, IP3_0, tau, dIP3, ip3_input, static_cast<float>(noise), P.substeps, ca,
+                            ca_lo, ip3, ip3_lo, hh, hh_lo);
+            factor = sic_factor32(Q.ac, ca, ca_lo);
+        }
+        if (factor > R(0)) {
+            ever = true;
+            wout |= 1u << k;
+            S.sic_v[std::size_t(t % B.Rv) * D.acap + ga] = factor;
+        }
+    }
+    S.Ca[ga] = ca;
+    S.IP3[ga] = ip3;
+    S.h[ga] = hh;
+    if constexpr (sizeof(R) == 4) {
+        S.Ca_lo[ga] = ca_lo;
+        S.IP3_lo[ga] = ip3_lo;
+        S.h_lo[ga] = hh_lo;
+    }
+    if (ever) D.ever[ga] = 1;
+    D.sic_w[std::size_t(B.wslot) * D.acap + ga] = wout;
+    if (B.last) D.bcnt[T_E2A][ga] = 0;
+    return B.last && wout != 0u;
+}
+
+// One neuron (global id gj, local id j) over global steps [lo, hi)
+// (src/astrosim_fused.cu:433-587). Returns the number of spikes.
+template <class R>
+__device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
+                                const Member& M, const BlkLaunch& B, index_t gj, index_t j, std::int64_t lo,
+                                std::int64_t hi, bool* emit) {
+    const std::size_t rN = std::size_t(B.rslot) * D.ncap;
+    const std::size_t rA = std::size_t(B.rslot) * D.acap;
+    const bool exc = j < M.ne;
+    const unsigned ce = D.bcnt[T_EXC][gj], ci = D.bcnt[T_INH][gj], cs = D.bcnt[T_SIC][gj];
+    const uint2* ee = D.bent[T_EXC] + std::size_t(gj) * kBucketCap;
+    const uint2* ei = D.bent[T_INH] + std::size_t(gj) * kBucketCap;
+    const uint2* es = D.bent[T_SIC] + std::size_t(gj) * kBucketCap;
+    auto nword = [&](index_t q) { return D.spk_w[rN + q]; };
+    auto aword = [&](index_t q) { return D.sic_w[rA + q]; };
+    const std::uint32_t any_e = ce ? input_mask(ee, ce, nword) : 0u;
+    const std::uint32_t any_i = ci ? input_mask(ei, ci, nword) : 0u;
+    const std::uint32_t any_s = cs ? input_mask(es, cs, aword) : 0u;
+    R V = S.V[gj], w = S.w[gj], g_ex = S.g_ex[gj], dg_ex = S.dg_ex[gj], g_in = S.g_in[gj], dg_in = S.dg_in[gj];
+    int refr = D.refr[gj];
+    R I_sic = S.I_sic[gj];
+    const R V_reset = S.V_reset[gj], bb = S.b[gj];
+    const double nstd = exc ? P.noise_exc : P.noise_inh;
+    std::uint64_t noise_at = ~std::uint64_t(0);
+    R noise_val = R(0);
+    unsigned nspikes = 0;
+    std::uint32_t wout = B.first ? 0u : D.spk_w[std::size_t(B.wslot) * D.ncap + gj];
+    for (std::int64_t t = lo; t < hi; ++t) {
+        const unsigned k = static_cast<unsigned>(t - B.bstart);
+        const std::int64_t tl = t - M.step0;  // the member's own step
+        const std::size_t vrow = value_row(t, B);
+        auto nval = [&](index_t q) { return S.spk_v[vrow * D.ncap + q]; };
+        R exc_sum = R(0), inh_sum = R(0);
+        if ((any_e >> k) & 1u) exc_sum = input_sum<R>(ee, ce, D.in_rs[T_EXC][gj], D.in_src[T_EXC], k, nword, nval);
+        // inhibitory source ids are local indices used as neuron ids (src/astrosim_fused.cu:459-462)
+        if ((any_i >> k) & 1u) inh_sum = input_sum<R>(ei, ci, D.in_rs[T_INH][gj], D.in_src[T_INH], k, nword, nval);
+        int ev = 0;
+        if (exc ? P.poiss_exc : P.poiss_inh)
+            ev = rng_poisson(M.seed ^ kKeyNeuronPoisson, std::uint64_t(tl) * kStepStride + j,
+                             exc ? P.lam_exc : P.lam_inh, exc ? P.lim_exc : P.lim_inh);
+        if ((tl - P.d_a2n >= 0) && ((tl - P.d_a2n) % P.sic_interval == 0)) {
+            R sic_sum = R(0);
+            if ((any_s >> k) & 1u)
+                sic_sum = input_sum<R>(es, cs, D.in_rs[T_SIC][gj], D.in_src[T_SIC], k, aword,
+                                       [&](index_t q) { return S.sic_v[vrow * D.acap + q]; });
+            if constexpr (sizeof(R) == 8) I_sic = P.w_a2n * sic_sum;
+            else I_sic = Q.w_a2n * sic_sum;
+        }
+        if (nstd > 0.0) {
+            // the neuron noise stream changes every steps_per_noise steps
+            const std::uint64_t nidx = static_cast<std::uint64_t>(tl / P.steps_per_noise);
+            if (nidx != noise_at) {
+                noise_at = nidx;
+                noise_val = static_cast<R>(nstd * rng_normal(M.seed ^ kKeyNeuronNoise, nidx * kNoiseStride + std::uint64_t(j)));
+            }
+        }
+        bool spiked;
+        if constexpr (sizeof(R) == 8) {
+            const NeuronConst& p = exc ? P.ne : P.ni;
+            double exc_input = P.w_e * exc_sum;
+            const double inh_input = (-P.w_i) * inh_sum;
+            if (ev > 0) exc_input += (exc ? P.w_poiss_exc : P.w_poiss_inh) * static_cast<double>(ev);
+            double I_ext = I_sic;
+            if (nstd > 0.0) I_ext += noise_val;
+            NeuronRegs r{V, w, g_ex, dg_ex + exc_input * p.psc_init_ex, g_in, dg_in + inh_input * p.psc_init_in, refr};
+            spiked = neuron_substeps(p, r, I_ext, V_reset, bb, P.h_step, P.substeps,
+                                     exc ? P.ex_decay_e : P.ex_decay_i, exc ? P.in_decay_e : P.in_decay_i);
+            V = r.V; w = r.w; g_ex = r.g_ex; dg_ex = r.dg_ex; g_in = r.g_in; dg_in = r.dg_in; refr = r.refractory;
+        } else {
+            const NeuronConst32& p = exc ? Q.ne : Q.ni;
+            float exc_input = Q.w_e_abs * exc_sum;
+            const float inh_input = Q.w_i_abs * inh_sum;
+            if (ev > 0) exc_input += (exc ? Q.w_poiss_exc : Q.w_poiss_inh) * static_cast<float>(ev);
+            const float I_ext = I_sic + noise_val;
+            NeuronRegs32 r{V, w, g_ex, dg_ex + exc_input * p.psc_init_ex, g_in, dg_in + inh_input * p.psc_init_in, refr};
+            spiked = neuron_substeps32(p, Q, r, I_ext, V_reset, bb, P.substeps, exc ? Q.ex_decay_e : Q.ex_decay_i,
+                                       exc ? Q.in_decay_e : Q.in_decay_i);
+            V = r.V; w = r.w; g_ex = r.g_ex; dg_ex = r.dg_ex; g_in = r.g_in; dg_in = r.dg_in; refr = r.refractory;
+        }
+        if (spiked) {
+            R out = R(1);
+            if (P.stp_enabled) {
+                R x = S.stp_x[gj], u = S.stp_u[gj], tlast = S.stp_t[gj];
+                if constexpr (sizeof(R) == 8) {
+                    out = stp_on_spike(x, u, tlast, static_cast<double>(tl) * P.dt, P.stp_U, P.stp_tau_rec,
+                                       P.stp_tau_fac);
+                } else {
+                    const float t_now = static_cast<float>(static_cast<double>(tl) * P.dt);
+                    const float elapsed = t_now - tlast;
+                    const float x_decay = __expf(-elapsed / static_cast<float>(P.stp_tau_rec));
+                    const float u_decay =
+                        P.stp_tau_fac < 1e-10 ? 0.0f : __expf(-elapsed / static_cast<float>(P.stp_tau_fac));
+                    const float U = static_cast<float>(P.stp_U);
+                    x = 1.0f + (x - x * u - 1.0f) * x_decay;
+                    u = U + u * (1.0f - U) * u_decay;
+                    tlast = t_now;
+                    out = x * u;
+                }
+                S.stp_x[gj] = x;
+                S.stp_u[gj] = u;
+                S.stp_t[gj] = tlast;
+            }
+            S.spk_v[std::size_t(t % B.Rv) * D.ncap + gj] = out;
+            wout |= 1u << k;
+            ++nspikes;
+        }
+    }
+    S.V[gj] = V;
+    S.w[gj] = w;
+    S.g_ex[gj] = g_ex;
+    S.dg_ex[gj] = dg_ex;
+    S.g_in[gj] = g_in;
+    S.dg_in[gj] = dg_in;
+    D.refr[gj] = refr;
+    S.I_sic[gj] = I_sic;
+    D.spk_w[std::size_t(B.wslot) * D.ncap + gj] = wout;
+    if (B.last) {
+        D.bcnt[T_EXC][gj] = 0;
+        D.bcnt[T_INH][gj] = 0;
+        D.bcnt[T_SIC][gj] = 0;
+    }
+    *emit = B.last && wout != 0u;
+    return nspikes;
+}
+
+template <int MAXT, class R>
+__global__ __launch_bounds__(MAXT) void integrate_kernel(BatchDev D, BlkState<R> S, RunParams P, Params32 Q,
+                                                         BlkLaunch B) {
+    resolve_launch(B, D.clock);
+    const Tile tile = D.tiles[blockIdx.x];
+    const Member M = D.members[tile.member];
+    // the part of this launch's steps inside the member's run
+    const std::int64_t lo = B.t0 > M.step0 ? B.t0 : M.step0;
+    const std::int64_t end = B.t0 + B.n;
+    const std::int64_t hi = end < M.step_end ? end : M.step_end;
+    const index_t local = tile.first + threadIdx.x;
+    if (tile.kind == 0) {
+        const bool run = local < M.na && lo < hi;
+        const bool emit = run ? astro_cell<R>(D, S, P, Q, M, B, M.ab + local, local, lo, hi) : false;
+        wave_append(emit, M.ab + local, D.alist + std::size_t(B.wslot) * D.acap, &D.lcount[B.wslot * 2 + 1]);
+        return;
+    }
+    const bool run = local < M.nn && lo < hi;
+    bool emit = false;
+    unsigned nspikes = 0;
+    if (run) nspikes = neuron_cell<R>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, &emit);
+    wave_append(emit, M.nb + local, D.nlist + std::size_t(B.wslot) * D.ncap, &D.lcount[B.wslot * 2 + 0]);
+    // diagnostic spike total of the member: one atomic per wave with spikes
+    unsigned long long c = nspikes;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
+    if (lane_id() == 0 && c != 0ull)
+        atomicAdd(&D.counters[std::size_t(M.slot) * kCountersPerMember + (blockIdx.x % kCountersPerMember)], c);
+}
+
+// ---------------------------------------------------------------------------
+// Member admission kernels.
+// ---------------------------------------------------------------------------
+// in_rs pairs from a raw row_start, shifted to the member's edge range.
+__global__ void rows_kernel(const index_t* rs, index_t n, index_t ebase, uint2* out) {
+    const index_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = make_uint2(ebase + rs[t], ebase + rs[t + 1]);
+}
+
+__global__ void rebase_kernel(index_t* p, std::size_t n, index_t base) {
+    const std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] += base;
+}
+
+// Counts out-edges per local source (sources outside [sbase, sbase+ns) skip).
+__global__ void count_sources_kernel(const index_t* src, std::size_t n_edges, index_t sbase, index_t ns,
+                                     index_t* cnt) {
+    const std::size_t e = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < n_edges) {
+        const index_t s = src[e] - sbase;
+        if (s < ns) atomicAdd(&cnt[s], 1u);
+    }
+}
+
+// out_rs of a member's source space [sbase, sbase + n_space) from the
+// exclusive scan of the counts (sources >= ns have no out-edges).
+__global__ void out_rows_kernel(const index_t* scan, index_t ns, index_t n_space, index_t sbase, index_t ebase,
+                                uint2* out_rs) {
+    const index_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n_space) out_rs[sbase + s] = s < ns ? make_uint2(ebase + scan[s], ebase + scan[s + 1]) : make_uint2(0, 0);
+}
+
+__global__ void fill_out_kernel(const uint2* in_rs, const index_t* src, index_t tbase, index_t n_targets,
+                                index_t sbase, index_t ns, index_t ebase, index_t* cursor, uint2* out) {
+    const index_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_targets) return;
+    const uint2 r = in_rs[tbase + t];
+    for (index_t e = r.x; e < r.y; ++e) {
+        const index_t s = src[e] - sbase;
+        if (s < ns) out[ebase + atomicAdd(&cursor[s], 1u)] = make_uint2(tbase + t, e);
+    }
+}
+
+template <class R>
+__global__ void fill_kernel(R* p, R v, std::size_t n) {
+    const std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------
+struct Geometry {
+    int d = 10, L = 10, H = 2, Rv = 20;
+};
+
+Geometry geometry(const RunParams& rp) {
+    if (rp.d_e != rp.d_i || rp.d_e != rp.d_a2n)
+        throw Error(AS_ERR_INVALID, "the blocked method needs equal exc/inh/a2n delays");
+    Geometry g;
+    g.d = rp.d_e;
+    g.L = 1;
+    for (int l = std::min(g.d, 32); l >= 1; --l)
+        if (g.d % l == 0) {
+            g.L = l;
+            break;
+        }
+    g.H = g.d / g.L + 1;
+    g.Rv = g.H * g.L;
+    return g;
+}
+
+// First-fit allocator of [begin, end) ranges in one id space.
+class RangeAllocator {
+public:
+    explicit RangeAllocator(std::uint64_t cap) : cap_(cap) {
+        if (cap > 0) free_.push_back({0, cap});
+    }
+    bool can(std::uint64_t n) const {
+        if (n == 0) return true;
+        for (auto& r : free_)
+            if (r.second - r.first >= n) return true;
+        return false;
+    }
+    std::uint64_t take(std::uint64_t n) {
+        if (n == 0) return 0;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second - it->first >= n) {
+                const std::uint64_t b = it->first;
+                it->first += n;
+                if (it->first == it->second) free_.erase(it);
+                return b;
+            }
+        }
+        throw Error(AS_ERR_NOMEM, "batch capacity exhausted");
+    }
+    void give(std::uint64_t b, std::uint64_t n) {
+        if (n == 0) return;
+        free_.push_back({b, b + n});
+        std::sort(free_.begin(), free_.end());
+        std::vector<std::pair<std::uint64_t, std::uint64_t>> m;
+        for (auto& r : free_) {
+            if (!m.empty() && m.back().second == r.first) m.back().second = r.second;
+            else m.push_back(r);
+        }
+        free_.swap(m);
+    }
+    std::uint64_t largest() const {
+        std::uint64_t best = 0;
+        for (auto& r : free_) best = std::max(best, r.second - r.first);
+        return best;
+    }
+
+private:
+    std::uint64_t cap_;
+    std::vector<std::pair<std::uint64_t, std::uint64_t>> free_;
+};
+
+struct BatchLayout {
+    std::size_t in_rs[4], in_src[4], out_rs[4], out_ent[4], bcnt[4], bent[4];
+    std::size_t astro[10], ever, neuron[12], refr;
+    std::size_t spk_w, spk_v, sic_w, sic_v, nlist, alist, lcount, counters, overflow;
+    std::size_t members, tiles, clock, scratch0, scratch1, scan_tmp;
+    std::size_t scan_tmp_bytes, tiles_cap;
+    std::size_t total;
+};
+
+std::size_t scan_temp_bytes(std::uint64_t n) {
+    std::size_t bytes = 0;
+    index_t* p = nullptr;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, p, p, int(n + 1));
+    return std::max<std::size_t>(bytes, 256);
+}
+
+BatchLayout batch_layout(const BatchCapacity& c, const RunParams& rp) {
+    const Geometry g = geometry(rp);
+    const std::size_t rsize = rp.precision == AS_FP64 ? 8 : 4;
+    Layout L;
+    BatchLayout o{};
+    const std::uint64_t nn = std::max<std::uint64_t>(1, c.neurons), na = std::max<std::uint64_t>(1, c.astros);
+    for (int i = 0; i < 4; ++i) {
+        const std::uint64_t nt = i == T_E2A ? na : nn;   // targets
+        const std::uint64_t ns = i == T_SIC ? na : nn;   // sources
+        const std::uint64_t ne = std::max<std::uint64_t>(1, c.edges[i]);
+        o.in_rs[i] = L.add(nt * sizeof(uint2));
+        o.in_src[i] = L.add(ne * sizeof(index_t));
+        o.out_rs[i] = L.add(ns * sizeof(uint2));
+        o.out_ent[i] = L.add(ne * sizeof(uint2));
+        o.bcnt[i] = L.add(nt * sizeof(unsigned));
+        o.bent[i] = L.add(nt * kBucketCap * sizeof(uint2));
+    }
+    const int n_astro_arrays = rsize == 4 ? 10 : 7;
+    for (int i = 0; i < n_astro_arrays; ++i) o.astro[i] = L.add(na * rsize);
+    o.ever = L.add(na);
+    for (int i = 0; i < 12; ++i) o.neuron[i] = L.add(nn * rsize);
+    o.refr = L.add(nn * sizeof(std::int32_t));
+    o.spk_w = L.add(std::size_t(g.H) * nn * 4);
+    o.spk_v = L.add(std::size_t(g.Rv) * nn * rsize);
+    o.sic_w = L.add(std::size_t(g.H) * na * 4);
+    o.sic_v = L.add(std::size_t(g.Rv) * na * rsize);
+    o.nlist = L.add(std::size_t(g.H) * nn * 4);
+    o.alist = L.add(std::size_t(g.H) * na * 4);
+    o.lcount = L.add(std::size_t(g.H) * 2 * 4);
+    o.counters = L.add(std::size_t(c.members) * kCountersPerMember * sizeof(unsigned long long));
+    o.overflow = L.add(sizeof(unsigned));
+    o.members = L.add(std::size_t(c.members) * sizeof(Member));
+    o.tiles_cap = (nn + 63) / 64 + (na + 63) / 64 + 2 * std::size_t(c.members);
+    o.tiles = L.add(o.tiles_cap * sizeof(Tile));
+    o.clock = L.add(sizeof(std::int64_t));
+    const std::uint64_t sm = std::max(nn, na) + 1;
+    o.scratch0 = L.add(sm * sizeof(index_t));
+    o.scratch1 = L.add(sm * sizeof(index_t));
+    o.scan_tmp_bytes = scan_temp_bytes(sm);
+    o.scan_tmp = L.add(o.scan_tmp_bytes);
+    o.total = L.total;
+    return o;
+}
+
+struct MemberHost {
+    bool used = false, active = false;
+    Member m{};
+    std::uint64_t ebase[4] = {0, 0, 0, 0}, ecount[4] = {0, 0, 0, 0};
+    std::int64_t free_at_block = -1;  // ranges released once the global block reaches this
+};
+
+template <class R>
+class BatchImpl final : public Batch {
+public:
+    BatchImpl(const RunParams& rp, const BatchCapacity& cap, hipStream_t s)
+        : P_(rp), Q_(make_params32(rp)), cap_(cap), stream_(s), geo_(geometry(rp)),
+          nalloc_(cap.neurons), aalloc_(cap.astros),
+          ealloc_{RangeAllocator(cap.edges[0]), RangeAllocator(cap.edges[1]), RangeAllocator(cap.edges[2]),
+                  RangeAllocator(cap.edges[3])},
+          hosts_(cap.members), threads_(rp.block) {
+        lay_ = batch_layout(cap, rp);
+        if (hipMalloc(&base_, lay_.total) != hipSuccess) {
+            (void)hipGetLastError();
+            base_ = nullptr;
+            throw Error(AS_ERR_NOMEM, "device allocation of " + std::to_string(lay_.total) + " bytes failed");
+        }
+        char* b = static_cast<char*>(base_);
+        auto dptr = [&](std::size_t off) { return static_cast<void*>(b + off); };
+        const std::uint64_t nn = std::max<std::uint64_t>(1, cap.neurons), na = std::max<std::uint64_t>(1, cap.astros);
+        for (int i = 0; i < 4; ++i) {
+            D_.in_rs[i] = (const uint2*)dptr(lay_.in_rs[i]);
+            D_.in_src[i] = (const index_t*)dptr(lay_.in_src[i]);
+            D_.out_rs[i] = (const uint2*)dptr(lay_.out_rs[i]);
+            D_.out_ent[i] = (const uint2*)dptr(lay_.out_ent[i]);
+            D_.bcnt[i] = (unsigned*)dptr(lay_.bcnt[i]);
+            D_.bent[i] = (uint2*)dptr(lay_.bent[i]);
+        }
+        R* A[10] = {};
+        for (int i = 0; i < (sizeof(R) == 4 ? 10 : 7); ++i) A[i] = (R*)dptr(lay_.astro[i]);
+        S_.Ca = A[0]; S_.IP3 = A[1]; S_.h = A[2];
+        S_.Ca_tot = A[3]; S_.IP3_0 = A[4]; S_.tau_IP3 = A[5]; S_.delta_IP3 = A[6];
+        S_.Ca_lo = A[7]; S_.IP3_lo = A[8]; S_.h_lo = A[9];
+        R* N[12];
+        for (int i = 0; i < 12; ++i) N[i] = (R*)dptr(lay_.neuron[i]);
+        S_.V = N[0]; S_.w = N[1]; S_.g_ex = N[2]; S_.dg_ex = N[3]; S_.g_in = N[4]; S_.dg_in = N[5];
+        S_.I_sic = N[6]; S_.stp_x = N[7]; S_.stp_u = N[8]; S_.stp_t = N[9]; S_.V_reset = N[10]; S_.b = N[11];
+        S_.spk_v = (R*)dptr(lay_.spk_v);
+        S_.sic_v = (R*)dptr(lay_.sic_v);
+        D_.ever = (std::uint8_t*)dptr(lay_.ever);
+        D_.refr = (std::int32_t*)dptr(lay_.refr);
+        D_.spk_w = (std::uint32_t*)dptr(lay_.spk_w);
+        D_.sic_w = (std::uint32_t*)dptr(lay_.sic_w);
+        D_.nlist = (std::uint32_t*)dptr(lay_.nlist);
+        D_.alist = (std::uint32_t*)dptr(lay_.alist);
+        D_.lcount = (std::uint32_t*)dptr(lay_.lcount);
+        D_.counters = (unsigned long long*)dptr(lay_.counters);
+        D_.overflow = (unsigned*)dptr(lay_.overflow);
+        D_.members = (const Member*)dptr(lay_.members);
+        D_.tiles = (const Tile*)dptr(lay_.tiles);
+        D_.clock = nullptr;
+        clock_ = (std::int64_t*)dptr(lay_.clock);
+        D_.ncap = nn;
+        D_.acap = na;
+        scratch0_ = (index_t*)dptr(lay_.scratch0);
+        scratch1_ = (index_t*)dptr(lay_.scratch1);
+        scan_tmp_ = dptr(lay_.scan_tmp);
+        AS_HIP_CHECK(hipMemsetAsync(D_.lcount, 0, std::size_t(geo_.H) * 2 * 4, stream_));
+        AS_HIP_CHECK(hipMemsetAsync(D_.overflow, 0, sizeof(unsigned), stream_));
+        AS_HIP_CHECK(hipMemsetAsync(D_.counters, 0, std::size_t(cap.members) * kCountersPerMember * 8, stream_));
+        AS_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    ~BatchImpl() override {
+        drop_graphs();
+        if (base_) {
+            (void)hipStreamSynchronize(stream_);
+            (void)hipFree(base_);
+        }
+    }
+
+    bool fits(const Network& net) const override {
+        if (free_slot() < 0) return false;
+        if (!nalloc_.can(net.cfg.n_neurons()) || !aalloc_.can(net.cfg.n_astro)) return false;
+        const HostCSR* c[4] = {&net.inc_exc, &net.inc_inh, &net.inc_sic, &net.inc_e2a};
+        for (int i = 0; i < 4; ++i)
+            if (!ealloc_[i].can(c[i]->n_edges())) return false;
+        return true;
+    }
+
+    int add(const Network& net, std::int64_t n_steps) override {
+        if (step_ % geo_.L != 0) throw Error(AS_ERR_INVALID, "members join at block boundaries");
+        if (n_steps < 0) throw Error(AS_ERR_INVALID, "n_steps must be >= 0");
+        release_due();
+        if (!fits(net)) throw Error(AS_ERR_NOMEM, "network does not fit in the batch's free capacity");
+        const int slot = free_slot();
+        MemberHost& h = hosts_[slot];
+        const index_t nn = net.cfg.n_neurons(), na = net.cfg.n_astro;
+        Member& m = h.m;
+        m.seed = net.seed;
+        m.nb = static_cast<index_t>(nalloc_.take(nn));
+        m.nn = nn;
+        m.ne = net.cfg.n_exc;
+        m.ab = static_cast<index_t>(aalloc_.take(na));
+        m.na = na;
+        m.step0 = step_;
+        m.step_end = step_ + n_steps;
+        m.slot = static_cast<std::uint32_t>(slot);
+        const HostCSR* c[4] = {&net.inc_exc, &net.inc_inh, &net.inc_sic, &net.inc_e2a};
+        for (int i = 0; i < 4; ++i) {
+            h.ecount[i] = c[i]->n_edges();
+            h.ebase[i] = ealloc_[i].take(h.ecount[i]);
+        }
+        upload_member(net, h);
+        h.used = true;
+        h.active = true;
+        h.free_at_block = -1;
+        rebuild_tiles();
+        return slot;
+    }
+
+    void remove(int member) override {
+        MemberHost& h = host(member);
+        if (!h.active) return;
+        h.active = false;
+        // emissions of the member's last block are still scattered d/L
+        // blocks later (into its own ranges): release after that
+        h.free_at_block = (step_ + geo_.L - 1) / geo_.L + geo_.d / geo_.L + 1;
+        rebuild_tiles();
+    }
+
+    void enqueue_steps(std::int64_t k) override {
+        while (k > 0) {
+            if (step_ % geo_.L == 0) release_due();
+            const std::int64_t bidx = step_ / geo_.L;
+            BlkLaunch B = launch_for(bidx);
+            B.t0 = step_;
+            B.n = static_cast<int>(std::min<std::int64_t>(k, B.bstart + geo_.L - step_));
+            B.first = step_ == B.bstart;
+            B.last = step_ + B.n == B.bstart + geo_.L;
+            BatchDev D = D_;
+            D.clock = nullptr;
+            if (B.first) hipLaunchKernelGGL(scatter_kernel, dim3(kScatterBlocks), dim3(256), 0, stream_, D, B);
+            timing_begin();
+            launch_integrate(D, B);
+            timing_end();
+            step_ += B.n;
+            k -= B.n;
+        }
+        AS_HIP_CHECK(hipGetLastError());
+        if (step_ % geo_.L == 0) release_due();
+    }
+
+    void run_blocks_graph(std::int64_t blocks) override {
+        if (blocks <= 0) return;
+        if (step_ % geo_.L != 0) throw Error(AS_ERR_INVALID, "graph replays start at a block boundary");
+        release_due();
+        hipGraphExec_t exec = graph_for(blocks);
+        const std::int64_t b0 = step_ / geo_.L;
+        AS_HIP_CHECK(hipMemcpyAsync(clock_, &b0, sizeof b0, hipMemcpyHostToDevice, stream_));
+        AS_HIP_CHECK(hipStreamSynchronize(stream_));  // b0 lives on the host stack
+        timing_begin();
+        AS_HIP_CHECK(hipGraphLaunch(exec, stream_));
+        timing_end(blocks);
+        step_ += blocks * geo_.L;
+        release_due();
+    }
+
+    int autotune(const int* cands, int n, int blocks_each) override {
+        if (n <= 0) return threads_;
+        if (step_ % geo_.L != 0) throw Error(AS_ERR_INVALID, "autotuning starts at a block boundary");
+        double best_ms = 1e300;
+        int best = threads_;
+        for (int i = 0; i < n; ++i) {
+            const int c = cands[i];
+            if (c < 64 || c > 1024 || c % 64 != 0) continue;
+            set_threads(c);
+            hipEvent_t e0, e1;
+            AS_HIP_CHECK(hipEventCreate(&e0));
+            AS_HIP_CHECK(hipEventCreate(&e1));
+            AS_HIP_CHECK(hipEventRecord(e0, stream_));
+            enqueue_steps(std::int64_t(blocks_each) * geo_.L);
+            AS_HIP_CHECK(hipEventRecord(e1, stream_));
+            AS_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            AS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        set_threads(best);
+        return best;
+    }
+
+    int block_threads() const override { return threads_; }
+
+    std::uint64_t spike_count(int member) override {
+        const MemberHost& h = host(member);
+        unsigned long long c[kCountersPerMember];
+        AS_HIP_CHECK(hipMemcpyAsync(c, D_.counters + std::size_t(h.m.slot) * kCountersPerMember, sizeof c,
+                                    hipMemcpyDeviceToHost, stream_));
+        AS_HIP_CHECK(hipStreamSynchronize(stream_));
+        std::uint64_t t = 0;
+        for (auto v : c) t += v;
+        return t;
+    }
+
+    void reset_ever_active(int member) override {
+        const MemberHost& h = host(member);
+        AS_HIP_CHECK(hipMemsetAsync(D_.ever + h.m.ab, 0, h.m.na, stream_));
+    }
+
+    void read_state(int member, int field, double* out) override {
+        const MemberHost& h = host(member);
+        const R* src = nullptr;
+        const R* lo = nullptr;
+        std::size_t base = h.m.nb, n = h.m.nn;
+        switch (field) {
+            case 0: src = S_.Ca; lo = S_.Ca_lo; base = h.m.ab; n = h.m.na; break;
+            case 1: src = S_.IP3; lo = S_.IP3_lo; base = h.m.ab; n = h.m.na; break;
+            case 2: src = S_.h; lo = S_.h_lo; base = h.m.ab; n = h.m.na; break;
+            case 3: src = S_.V; break;
+            case 4: src = S_.w; break;
+            case 5: src = S_.g_ex; break;
+            case 6: src = S_.dg_ex; break;
+            case 7: src = S_.g_in; break;
+            case 8: src = S_.dg_in; break;
+            case 9: src = S_.I_sic; break;
+            case 10: src = S_.stp_x; break;
+            case 11: src = S_.stp_u; break;
+            case 12: src = S_.stp_t; break;
+            default: throw Error(AS_ERR_INVALID, "unknown state field " + std::to_string(field));
+        }
+        std::vector<R> hv(n), lv;
+        if (n) AS_HIP_CHECK(hipMemcpyAsync(hv.data(), src + base, n * sizeof(R), hipMemcpyDeviceToHost, stream_));
+        if (sizeof(R) == 4 && lo && n) {
+            lv.resize(n);
+            AS_HIP_CHECK(hipMemcpyAsync(lv.data(), lo + base, n * sizeof(R), hipMemcpyDeviceToHost, stream_));
+        }
+        AS_HIP_CHECK(hipStreamSynchronize(stream_));
+        for (std::size_t i = 0; i < n; ++i) out[i] = double(hv[i]) + (lv.empty() ? 0.0 : double(lv[i]));
+    }
+
+    void read_refractory(int member, std::int32_t* out) override {
+        const MemberHost& h = host(member);
+        AS_HIP_CHECK(hipMemcpyAsync(out, D_.refr + h.m.nb, h.m.nn * 4, hipMemcpyDeviceToHost, stream_));
+        AS_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    void read_ever_active(int member, std::uint8_t* out) override {
+        const MemberHost& h = host(member);
+        AS_HIP_CHECK(hipMemcpyAsync(out, D_.ever + h.m.ab, h.m.na, hipMemcpyDeviceToHost, stream_));
+        AS_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    void read_raster(int member, std::int64_t first, int n, std::uint8_t* out) override {
+        const MemberHost& h = host(member);
+        const std::int64_t done = member_steps(member);
+        const std::int64_t cur_block = step_ > 0 ? (step_ - 1) / geo_.L : 0;
+        const std::int64_t g0 = h.m.step0 + first;
+        if (n < 0 || first < 0 || first + n > done || (n > 0 && g0 / geo_.L <= cur_block - geo_.H))
+            throw Error(AS_ERR_RANGE, "raster steps not held (the blocked method keeps the last H blocks)");
+        std::vector<std::uint32_t> words(h.m.nn);
+        std::int64_t have = -1;
+        for (int i = 0; i < n; ++i) {
+            const std::int64_t s = g0 + i;
+            const std::int64_t bl = s / geo_.L;
+            if (bl != have) {
+                AS_HIP_CHECK(hipMemcpyAsync(words.data(), D_.spk_w + std::size_t(bl % geo_.H) * D_.ncap + h.m.nb,
+                                            h.m.nn * 4, hipMemcpyDeviceToHost, stream_));
+                AS_HIP_CHECK(hipStreamSynchronize(stream_));
+                have = bl;
+            }
+            const unsigned bit = static_cast<unsigned>(s - bl * geo_.L);
+            for (index_t j = 0; j < h.m.nn; ++j) out[std::size_t(i) * h.m.nn + j] = (words[j] >> bit) & 1u;
+        }
+    }
+
+    std::int64_t member_steps(int member) const override {
+        const MemberHost& h = host(member);
+        return std::max<std::int64_t>(0, std::min(step_, h.m.step_end) - h.m.step0);
+    }
+    std::int64_t member_target(int member) const override { return host(member).m.step_end - host(member).m.step0; }
+    bool member_active(int member) const override { return host(member).active; }
+    int active_members() const override {
+        int n = 0;
+        for (auto& h : hosts_) n += h.active ? 1 : 0;
+        return n;
+    }
+    std::uint64_t member_overflows() override {
+        unsigned v = 0;
+        AS_HIP_CHECK(hipMemcpyAsync(&v, D_.overflow, sizeof v, hipMemcpyDeviceToHost, stream_));
+        AS_HIP_CHECK(hipStreamSynchronize(stream_));
+        return v;
+    }
+
+    std::int64_t global_step() const override { return step_; }
+    int block_len() const override { return geo_.L; }
+    hipStream_t stream() const override { return stream_; }
+    std::uint64_t device_bytes() const override { return lay_.total; }
+    std::uint64_t free_neurons() const override { return nalloc_.largest(); }
+
+    void set_timing(bool on) override { timing_ = on; }
+    void kernel_time(double* total_ms, std::int64_t* launches) override {
+        AS_HIP_CHECK(hipStreamSynchronize(stream_));
+        double t = 0.0;
+        for (auto& e : events_) {
+            float ms = 0.0f;
+            AS_HIP_CHECK(hipEventElapsedTime(&ms, e.first, e.second));
+            t += ms;
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        events_.clear();
+        *total_ms = t;
+        *launches = timed_launches_;
+        timed_launches_ = 0;
+    }
+
+private:
+    static constexpr unsigned kScatterBlocks = 1024;
+
+    const MemberHost& host(int member) const {
+        if (member < 0 || member >= int(hosts_.size()) || !hosts_[member].used)
+            throw Error(AS_ERR_INVALID, "no such batch member");
+        return hosts_[member];
+    }
+    MemberHost& host(int member) { return const_cast<MemberHost&>(static_cast<const BatchImpl*>(this)->host(member)); }
+
+    int free_slot() const {
+        for (std::size_t i = 0; i < hosts_.size(); ++i)
+            if (!hosts_[i].used) return int(i);
+        return -1;
+    }
+
+    BlkLaunch launch_for(std::int64_t bidx) const {
+        BlkLaunch B{};
+        B.bstart = bidx * geo_.L;
+        B.t0 = B.bstart;
+        B.n = geo_.L;
+        B.d = geo_.d;
+        B.L = geo_.L;
+        B.H = geo_.H;
+        B.Rv = geo_.Rv;
+        B.wslot = static_cast<std::uint32_t>(bidx % geo_.H);
+        B.rslot = static_cast<std::uint32_t>(((bidx - geo_.d / geo_.L) % geo_.H + geo_.H) % geo_.H);
+        B.first = 1;
+        B.last = 1;
+        return B;
+    }
+
+    void launch_integrate(const BatchDev& D, const BlkLaunch& B) {
+        if (n_tiles_ == 0) return;
+        if (threads_ <= 256)
+            hipLaunchKernelGGL((integrate_kernel<256, R>), dim3(n_tiles_), dim3(threads_), 0, stream_, D, S_, P_, Q_, B);
+        else
+            hipLaunchKernelGGL((integrate_kernel<1024, R>), dim3(n_tiles_), dim3(threads_), 0, stream_, D, S_, P_, Q_, B);
+    }
+
+    void timing_begin() {
+        if (!timing_) return;
+        AS_HIP_CHECK(hipEventCreate(&pending_));
+        AS_HIP_CHECK(hipEventRecord(pending_, stream_));
+    }
+    void timing_end(std::int64_t launches = 1) {
+        if (!timing_ || !pending_) return;
+        hipEvent_t stop;
+        AS_HIP_CHECK(hipEventCreate(&stop));
+        AS_HIP_CHECK(hipEventRecord(stop, stream_));
+        events_.emplace_back(pending_, stop);
+        pending_ = nullptr;
+        timed_launches_ += launches;
+    }
+
+    void set_threads(int t) {
+        if (t == threads_) return;
+        threads_ = t;
+        rebuild_tiles();
+    }
+
+    void rebuild_tiles() {
+        std::vector<Tile> tiles;
+        std::vector<Member> ms(hosts_.size());
+        for (std::size_t i = 0; i < hosts_.size(); ++i) {
+            const MemberHost& h = hosts_[i];
+            ms[i] = h.m;
+            if (!h.active) continue;
+            for (index_t f = 0; f < h.m.na; f += threads_) tiles.push_back({std::uint32_t(i), f, 0u, 0u});
+            for (index_t f = 0; f < h.m.nn; f += threads_) tiles.push_back({std::uint32_t(i), f, 1u, 0u});
+        }
+        if (tiles.size() > lay_.tiles_cap) throw Error(AS_ERR_NOMEM, "tile table capacity exceeded");
+        n_tiles_ = static_cast<unsigned>(tiles.size());
+        // pageable copies are ordered with the stream and complete before return
+        if (!tiles.empty())
+            AS_HIP_CHECK(hipMemcpyAsync(const_cast<Tile*>(D_.tiles), tiles.data(), tiles.size() * sizeof(Tile),
+                                        hipMemcpyHostToDevice, stream_));
+        AS_HIP_CHECK(hipMemcpyAsync(const_cast<Member*>(D_.members), ms.data(), ms.size() * sizeof(Member),
+                                    hipMemcpyHostToDevice, stream_));
+        AS_HIP_CHECK(hipStreamSynchronize(stream_));
+        drop_graphs();
+    }
+
+    void release_due() {
+        const std::int64_t blk = step_ / geo_.L;
+        for (auto& h : hosts_) {
+            if (h.used && !h.active && h.free_at_block >= 0 && blk >= h.free_at_block) {
+                nalloc_.give(h.m.nb, h.m.nn);
+                aalloc_.give(h.m.ab, h.m.na);
+                for (int i = 0; i < 4; ++i) ealloc_[i].give(h.ebase[i], h.ecount[i]);
+                h = MemberHost();
+            }
+        }
+    }
+
+    void drop_graphs() {
+        for (auto& g : graphs_) {
+            (void)hipGraphExecDestroy(g.second.second);
+            (void)hipGraphDestroy(g.second.first);
+        }
+        graphs_.clear();
+    }
+
+    hipGraphExec_t graph_for(std::int64_t blocks) {
+        auto it = graphs_.find(blocks);
+        if (it != graphs_.end()) return it->second.second;
+        BatchDev D = D_;
+        D.clock = clock_;
+        const BlkLaunch B = launch_for(0);  // the kernels take the block index from the clock
+        AS_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        for (std::int64_t i = 0; i < blocks; ++i) {
+            hipLaunchKernelGGL(scatter_kernel, dim3(kScatterBlocks), dim3(256), 0, stream_, D, B);
+            launch_integrate(D, B);
+            hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(64), 0, stream_, clock_);
+        }
+        hipGraph_t g = nullptr;
+        AS_HIP_CHECK(hipStreamEndCapture(stream_, &g));
+        hipGraphExec_t exec = nullptr;
+        AS_HIP_CHECK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+        graphs_[blocks] = {g, exec};
+        return exec;
+    }
+
+    template <class T>
+    void fill(T* p, T v, std::size_t n) {
+        if (n == 0) return;
+        hipLaunchKernelGGL(fill_kernel<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream_, p, v, n);
+        AS_HIP_CHECK(hipGetLastError());
+    }
+
+    void upload_values(R* dst, const std::vector<double>& v, bool reciprocal = false) {
+        if (v.empty()) return;
+        std::vector<R> f(v.size());
+        for (std::size_t i = 0; i < v.size(); ++i) f[i] = R(reciprocal ? 1.0 / v[i] : v[i]);
+        AS_HIP_CHECK(hipMemcpyAsync(dst, f.data(), f.size() * sizeof(R), hipMemcpyHostToDevice, stream_));
+        AS_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    // Uploads one member: rebased in-CSRs, out-CSRs built on the device,
+    // parameters and initial state (src/astrosim_fused.cu:981-1018).
+    void upload_member(const Network& net, MemberHost& h) {
+        const Member& m = h.m;
+        const HostCSR* c[4] = {&net.inc_exc, &net.inc_inh, &net.inc_sic, &net.inc_e2a};
+        // source-id base and source-space size of each type (global ids)
+        const index_t sbase[4] = {m.nb, m.nb, m.ab, m.nb};
+        const index_t tbase[4] = {m.nb, m.nb, m.nb, m.ab};
+        const index_t nsrc[4] = {m.ne, net.cfg.n_inh, m.na, m.ne};
+        const index_t nspace[4] = {m.nn, m.nn, m.na, m.nn};
+        for (int T = 0; T < 4; ++T) {
+            const index_t nt = c[T]->n_targets();
+            const std::size_t ne = c[T]->n_edges();
+            const index_t eb = static_cast<index_t>(h.ebase[T]);
+            index_t* src = const_cast<index_t*>(D_.in_src[T]) + eb;
+            if (ne) {
+                AS_HIP_CHECK(hipMemcpyAsync(src, c[T]->src.data(), ne * sizeof(index_t), hipMemcpyHostToDevice, stream_));
+                if (sbase[T] != 0)
+                    hipLaunchKernelGGL(rebase_kernel, dim3(unsigned((ne + 255) / 256)), dim3(256), 0, stream_, src, ne,
+                                       sbase[T]);
+            }
+            AS_HIP_CHECK(hipMemcpyAsync(scratch0_, c[T]->row_start.data(), (std::size_t(nt) + 1) * sizeof(index_t),
+                                        hipMemcpyHostToDevice, stream_));
+            uint2* in_rs = const_cast<uint2*>(D_.in_rs[T]) + tbase[T];
+            if (nt) hipLaunchKernelGGL(rows_kernel, dim3((nt + 255) / 256), dim3(256), 0, stream_, scratch0_, nt, eb, in_rs);
+            // out-CSR over the member's source space
+            const index_t ns = nsrc[T];
+            AS_HIP_CHECK(hipMemsetAsync(scratch0_, 0, (std::size_t(ns) + 1) * sizeof(index_t), stream_));
+            if (ne)
+                hipLaunchKernelGGL(count_sources_kernel, dim3(unsigned((ne + 255) / 256)), dim3(256), 0, stream_,
+                                   D_.in_src[T] + eb, ne, sbase[T], ns, scratch0_);
+            std::size_t tmp = lay_.scan_tmp_bytes;
+            AS_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp_, tmp, scratch0_, scratch1_, int(ns) + 1, stream_));
+            if (nspace[T])
+                hipLaunchKernelGGL(out_rows_kernel, dim3((nspace[T] + 255) / 256), dim3(256), 0, stream_, scratch1_, ns,
+                                   nspace[T], sbase[T], eb, const_cast<uint2*>(D_.out_rs[T]));
+            if (nt)
+                hipLaunchKernelGGL(fill_out_kernel, dim3((nt + 255) / 256), dim3(256), 0, stream_, D_.in_rs[T],
+                                   D_.in_src[T], tbase[T], nt, sbase[T], ns, eb, scratch1_,
+                                   const_cast<uint2*>(D_.out_ent[T]));
+            AS_HIP_CHECK(hipGetLastError());
+            AS_HIP_CHECK(hipMemsetAsync(D_.bcnt[T] + tbase[T], 0, std::size_t(nt) * sizeof(unsigned), stream_));
+            AS_HIP_CHECK(hipStreamSynchronize(stream_));
+        }
+        // parameters: FP32c stores 1/tau_IP3
+        upload_values(S_.Ca_tot + m.ab, net.Ca_tot);
+        upload_values(S_.IP3_0 + m.ab, net.IP3_0);
+        upload_values(S_.tau_IP3 + m.ab, net.tau_IP3, sizeof(R) == 4);
+        upload_values(S_.delta_IP3 + m.ab, net.delta_IP3);
+        upload_values(S_.V_reset + m.nb, net.V_reset);
+        upload_values(S_.b + m.nb, net.b);
+        const Biology bio;
+        fill(S_.Ca + m.ab, R(bio.Ca_init), m.na);
+        fill(S_.IP3 + m.ab, R(bio.IP3_init), m.na);
+        fill(S_.h + m.ab, R(bio.h_init), m.na);
+        if constexpr (sizeof(R) == 4) {  // rounding residue of the initial values
+            fill(S_.Ca_lo + m.ab, R(bio.Ca_init - double(float(bio.Ca_init))), m.na);
+            fill(S_.IP3_lo + m.ab, R(bio.IP3_init - double(float(bio.IP3_init))), m.na);
+            fill(S_.h_lo + m.ab, R(bio.h_init - double(float(bio.h_init))), m.na);
+        }
+        fill(S_.V + m.nb, R(bio.e_V_m), m.ne);
+        fill(S_.V + m.nb + m.ne, R(bio.i_V_m), m.nn - m.ne);
+        for (R* p : {S_.w, S_.g_ex, S_.dg_ex, S_.g_in, S_.dg_in, S_.I_sic}) fill(p + m.nb, R(0), m.nn);
+        fill(S_.stp_x + m.nb, R(1), m.nn);
+        fill(S_.stp_u + m.nb, R(P_.stp_U), m.nn);
+        fill(S_.stp_t + m.nb, R(-1e12), m.nn);
+        fill(D_.refr + m.nb, std::int32_t(0), m.nn);
+        fill(D_.ever + m.ab, std::uint8_t(0), m.na);
+        for (int s = 0; s < geo_.H; ++s) {
+            fill(D_.spk_w + std::size_t(s) * D_.ncap + m.nb, std::uint32_t(0), m.nn);
+            fill(D_.sic_w + std::size_t(s) * D_.acap + m.ab, std::uint32_t(0), m.na);
+        }
+        fill(D_.counters + std::size_t(m.slot) * kCountersPerMember, 0ull, kCountersPerMember);
+        AS_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    RunParams P_;
+    Params32 Q_;
+    BatchCapacity cap_;
+    hipStream_t stream_;
+    Geometry geo_;
+    RangeAllocator nalloc_, aalloc_;
+    RangeAllocator ealloc_[4];
+    std::vector<MemberHost> hosts_;
+    int threads_;
+    unsigned n_tiles_ = 0;
+    BatchLayout lay_{};
+    void* base_ = nullptr;
+    BatchDev D_{};
+    BlkState<R> S_{};
+    std::int64_t* clock_ = nullptr;
+    index_t *scratch0_ = nullptr, *scratch1_ = nullptr;
+    void* scan_tmp_ = nullptr;
+    std::int64_t step_ = 0;
+    std::map<std::int64_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
+    bool timing_ = false;
+    hipEvent_t pending_ = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
+    std::int64_t timed_launches_ = 0;
+};
+
+// A single simulation on the blocked method: a batch with one member whose
+// capacity is exactly its network.
+class BlockedSim final : public Engine {
+public:
+    BlockedSim(const Network& net, const RunParams& rp, hipStream_t s) {
+        stream_ = s;
+        n_neurons_ = net.cfg.n_neurons();
+        n_astro_ = net.cfg.n_astro;
+        n_exc_ = net.cfg.n_exc;
+        const Network* p = &net;
+        batch_ = make_batch(rp, capacity_for(&p, 1), s);
+        member_ = batch_->add(net, std::numeric_limits<std::int64_t>::max() / 4);
+    }
+    void enqueue_steps(std::int64_t k) override {
+        batch_->enqueue_steps(k);
+        step_ += k;
+    }
+    std::uint64_t spike_count() override { return batch_->spike_count(member_); }
+    void reset_ever_active() override { batch_->reset_ever_active(member_); }
+    void read_state(int field, double* out) override { batch_->read_state(member_, field, out); }
+    void read_refractory(std::int32_t* out) override { batch_->read_refractory(member_, out); }
+    void read_ever_active(std::uint8_t* out) override { batch_->read_ever_active(member_, out); }
+    void read_raster(std::int64_t first, int n, std::uint8_t* out) override {
+        batch_->read_raster(member_, first, n, out);
+    }
+    std::uint64_t device_bytes() const override { return batch_->device_bytes(); }
+    void set_timing(bool on) override { batch_->set_timing(on); }
+    void kernel_time(double* total_ms, std::int64_t* launches) override { batch_->kernel_time(total_ms, launches); }
+
+private:
+    std::unique_ptr<Batch> batch_;
+    int member_ = 0;
+};
+
+}  // namespace
+
+BatchCapacity capacity_for(const Network* const* nets, int n) {
+    BatchCapacity c{};
+    c.members = static_cast<std::uint32_t>(std::max(1, n));
+    for (int i = 0; i < n; ++i) {
+        const Network& net = *nets[i];
+        c.neurons += net.cfg.n_neurons();
+        c.astros += net.cfg.n_astro;
+        c.edges[0] += net.inc_exc.n_edges();
+        c.edges[1] += net.inc_inh.n_edges();
+        c.edges[2] += net.inc_sic.n_edges();
+        c.edges[3] += net.inc_e2a.n_edges();
+    }
+    return c;
+}
+
+std::uint64_t batch_footprint(const BatchCapacity& cap, const RunParams& rp) { return batch_layout(cap, rp).total; }
+
+std::unique_ptr<Batch> make_batch(const RunParams& rp, const BatchCapacity& cap, hipStream_t s) {
+    if (cap.neurons + cap.astros >= (std::uint64_t(1) << 32) ||
+        std::max({cap.edges[0], cap.edges[1], cap.edges[2], cap.edges[3]}) >= (std::uint64_t(1) << 32))
+        throw Error(AS_ERR_INVALID, "batch capacity exceeds 32-bit ids");
+    if (rp.precision == AS_FP64) return std::unique_ptr<Batch>(new BatchImpl<double>(rp, cap, s));
+    return std::unique_ptr<Batch>(new BatchImpl<float>(rp, cap, s));
+}
+
+std::uint64_t blocked_footprint(const Network& net, const RunParams& rp) {
+    const Network* p = &net;
+    return batch_footprint(capacity_for(&p, 1), rp);
+}
+
+std::unique_ptr<Engine> make_blocked_engine(const Network& net, const RunParams& rp, hipStream_t s) {
+    return std::unique_ptr<Engine>(new BlockedSim(net, rp, s));
+}
+
+}  // namespace astrosim

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/astrosim.h"
+#include "as_batch.h"
 #include "as_engine.h"
 #include "as_setup.h"
 
@@ -16,6 +17,11 @@ struct as_network {
 
 struct as_sim {
     std::unique_ptr<astrosim::Engine> eng;
+    hipStream_t stream = nullptr;
+};
+
+struct as_batch {
+    std::unique_ptr<astrosim::Batch> b;
     hipStream_t stream = nullptr;
 };
 
@@ -247,6 +253,193 @@ void as_sim_destroy(as_sim* sim) {
     sim->eng.reset();
     if (sim->stream) (void)hipStreamDestroy(sim->stream);
     delete sim;
+}
+
+static astrosim::BatchCapacity to_cap(const as_batch_capacity& c) {
+    astrosim::BatchCapacity k;
+    k.neurons = c.neurons;
+    k.astros = c.astros;
+    for (int i = 0; i < 4; ++i) k.edges[i] = c.edges[i];
+    k.members = c.members;
+    return k;
+}
+
+as_status as_batch_capacity_for(const as_network* const* nets, int32_t n, as_batch_capacity* cap) {
+    return guarded([&] {
+        require(cap && (n == 0 || nets), "NULL argument");
+        require(n >= 0, "n must be >= 0");
+        std::vector<const astrosim::Network*> v;
+        for (int i = 0; i < n; ++i) {
+            require(nets[i] != nullptr, "NULL network");
+            v.push_back(&nets[i]->net);
+        }
+        const astrosim::BatchCapacity k = astrosim::capacity_for(v.data(), n);
+        cap->neurons = k.neurons;
+        cap->astros = k.astros;
+        for (int i = 0; i < 4; ++i) cap->edges[i] = k.edges[i];
+        cap->members = k.members;
+    });
+}
+
+as_status as_batch_footprint(const as_batch_capacity* cap, const as_config* cfg, uint64_t* bytes) {
+    return guarded([&] {
+        require(cap && cfg && bytes, "NULL argument");
+        astrosim::RunParams rp = astrosim::resolve_params(*cfg);
+        require(rp.method == AS_METHOD_BLOCKED, "batches run the blocked method");
+        *bytes = astrosim::batch_footprint(to_cap(*cap), rp);
+    });
+}
+
+as_status as_batch_create(const as_batch_capacity* cap, const as_config* cfg, as_batch** out) {
+    return guarded([&] {
+        require(cap && cfg && out, "NULL argument");
+        *out = nullptr;
+        const astrosim::RunParams rp = astrosim::resolve_params(*cfg);
+        require(rp.method == AS_METHOD_BLOCKED, "batches run the blocked method");
+        require(cap->members >= 1, "capacity needs at least one member slot");
+        require_device();
+        std::unique_ptr<as_batch> b(new as_batch);
+        AS_HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        try {
+            b->b = astrosim::make_batch(rp, to_cap(*cap), b->stream);
+        } catch (...) {
+            (void)hipStreamDestroy(b->stream);
+            b->stream = nullptr;
+            throw;
+        }
+        *out = b.release();
+    });
+}
+
+as_status as_batch_fits(as_batch* b, const as_network* net, int32_t* fits) {
+    return guarded([&] {
+        require(b && net && fits, "NULL argument");
+        *fits = b->b->fits(net->net) ? 1 : 0;
+    });
+}
+
+as_status as_batch_add(as_batch* b, const as_network* net, int64_t n_steps, int32_t* member) {
+    return guarded([&] {
+        require(b && net && member, "NULL argument");
+        *member = b->b->add(net->net, n_steps);
+    });
+}
+
+as_status as_batch_remove(as_batch* b, int32_t member) {
+    return guarded([&] {
+        require(b != nullptr, "batch is NULL");
+        b->b->remove(member);
+    });
+}
+
+as_status as_batch_step(as_batch* b, int64_t k) {
+    return guarded([&] {
+        require(b != nullptr, "batch is NULL");
+        require(k >= 0, "k must be >= 0");
+        b->b->enqueue_steps(k);
+    });
+}
+
+as_status as_batch_run_graph(as_batch* b, int64_t blocks) {
+    return guarded([&] {
+        require(b != nullptr, "batch is NULL");
+        require(blocks >= 0, "blocks must be >= 0");
+        b->b->run_blocks_graph(blocks);
+    });
+}
+
+as_status as_batch_autotune(as_batch* b, const int32_t* candidates, int32_t n, int32_t blocks_each,
+                            int32_t* chosen) {
+    return guarded([&] {
+        require(b && candidates && chosen, "NULL argument");
+        require(n > 0 && blocks_each > 0, "need candidates and blocks_each > 0");
+        std::vector<int> c(candidates, candidates + n);
+        *chosen = b->b->autotune(c.data(), n, blocks_each);
+    });
+}
+
+as_status as_batch_sync(as_batch* b) {
+    return guarded([&] {
+        require(b != nullptr, "batch is NULL");
+        AS_HIP_CHECK(hipStreamSynchronize(b->stream));
+    });
+}
+
+as_status as_batch_get_info(as_batch* b, as_batch_info* info) {
+    return guarded([&] {
+        require(b && info, "NULL argument");
+        info->global_step = b->b->global_step();
+        info->block_steps = b->b->block_len();
+        info->active_members = b->b->active_members();
+        info->block_threads = b->b->block_threads();
+        info->device_bytes = b->b->device_bytes();
+        info->free_neurons = b->b->free_neurons();
+        info->bucket_overflows = b->b->member_overflows();
+    });
+}
+
+as_status as_batch_member_steps(as_batch* b, int32_t member, int64_t* done, int64_t* target) {
+    return guarded([&] {
+        require(b && done && target, "NULL argument");
+        *done = b->b->member_steps(member);
+        *target = b->b->member_target(member);
+    });
+}
+
+as_status as_batch_spike_count(as_batch* b, int32_t member, uint64_t* out) {
+    return guarded([&] {
+        require(b && out, "NULL argument");
+        *out = b->b->spike_count(member);
+    });
+}
+
+as_status as_batch_reset_ever_active(as_batch* b, int32_t member) {
+    return guarded([&] {
+        require(b != nullptr, "batch is NULL");
+        b->b->reset_ever_active(member);
+    });
+}
+
+as_status as_batch_read_state(as_batch* b, int32_t member, int32_t field, double* out) {
+    return guarded([&] {
+        require(b && out, "NULL argument");
+        b->b->read_state(member, field, out);
+    });
+}
+
+as_status as_batch_read_ever_active(as_batch* b, int32_t member, uint8_t* out) {
+    return guarded([&] {
+        require(b && out, "NULL argument");
+        b->b->read_ever_active(member, out);
+    });
+}
+
+as_status as_batch_read_raster(as_batch* b, int32_t member, int64_t first_step, int32_t n_steps, uint8_t* out) {
+    return guarded([&] {
+        require(b && out, "NULL argument");
+        b->b->read_raster(member, first_step, n_steps, out);
+    });
+}
+
+as_status as_batch_set_timing(as_batch* b, int enable) {
+    return guarded([&] {
+        require(b != nullptr, "batch is NULL");
+        b->b->set_timing(enable != 0);
+    });
+}
+
+as_status as_batch_kernel_time(as_batch* b, double* total_ms, int64_t* launches) {
+    return guarded([&] {
+        require(b && total_ms && launches, "NULL argument");
+        b->b->kernel_time(total_ms, launches);
+    });
+}
+
+void as_batch_destroy(as_batch* b) {
+    if (!b) return;
+    b->b.reset();
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
 }
 
 as_status as_device_info(char* name, int name_len, char* arch, int arch_len) {

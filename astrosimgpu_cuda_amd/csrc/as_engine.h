@@ -84,9 +84,9 @@ public:
     index_t n_neurons() const { return n_neurons_; }
     index_t n_astro() const { return n_astro_; }
 
-    // Kernel timing (HIP events around each enqueue_steps call).
-    void set_timing(bool on);
-    void kernel_time(double* total_ms, std::int64_t* launches);
+    // Kernel timing (HIP events around each dominant-kernel launch).
+    virtual void set_timing(bool on);
+    virtual void kernel_time(double* total_ms, std::int64_t* launches);
 
 protected:
     // Brackets one launch of the engine's dominant step kernel with HIP

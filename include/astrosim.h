@@ -131,6 +131,63 @@ as_status as_sim_set_timing(as_sim* sim, int enable);
 as_status as_sim_kernel_time(as_sim* sim, double* total_ms, int64_t* launches);
 void as_sim_destroy(as_sim* sim);
 
+/* ---- batches: many networks advanced together (blocked method) ----
+ *
+ * No counterpart in the reference, which runs one network per process. A
+ * batch holds an HBM arena of fixed capacity; member networks occupy ranges
+ * of it, join and leave at block boundaries (1 ms at dt 0.1), and advance
+ * together in one scatter and one integrate launch per block. A member's
+ * results are bit-identical to running it alone (its random streams use its
+ * own seed, local cell ids and local steps). Members are numbered by slot. */
+typedef struct as_batch as_batch;
+typedef struct as_batch_capacity {
+    uint64_t neurons, astros;
+    uint64_t edges[4]; /* exc, inh, sic, e2a */
+    uint32_t members;
+} as_batch_capacity;
+typedef struct as_batch_info {
+    int64_t global_step;      /* steps enqueued since creation */
+    int32_t block_steps;      /* block length L in steps */
+    int32_t active_members;
+    int32_t block_threads;    /* workgroup (tile) size in use */
+    uint64_t device_bytes;
+    uint64_t free_neurons;    /* largest free neuron range */
+    uint64_t bucket_overflows;/* targets that fell back to a full in-row walk */
+} as_batch_info;
+
+/* Exact capacity for the given networks side by side. */
+as_status as_batch_capacity_for(const as_network* const* nets, int32_t n, as_batch_capacity* cap);
+as_status as_batch_footprint(const as_batch_capacity* cap, const as_config* cfg, uint64_t* bytes);
+/* cfg->method must be AS_METHOD_BLOCKED. */
+as_status as_batch_create(const as_batch_capacity* cap, const as_config* cfg, as_batch** out);
+/* *fits = 1 when the network fits the free ranges and a member slot is free. */
+as_status as_batch_fits(as_batch* b, const as_network* net, int32_t* fits);
+/* Admits a network to run n_steps steps from the current global step (a
+ * block boundary). Uploads it and builds its out-CSR on the batch's stream. */
+as_status as_batch_add(as_batch* b, const as_network* net, int64_t n_steps, int32_t* member);
+/* Retires a member; its ranges are reused once in-flight deliveries drain. */
+as_status as_batch_remove(as_batch* b, int32_t member);
+/* Advances all active members k steps (asynchronous; partial blocks allowed). */
+as_status as_batch_step(as_batch* b, int64_t k);
+/* Advances whole blocks by replaying a captured hipGraph (asynchronous). */
+as_status as_batch_run_graph(as_batch* b, int64_t blocks);
+/* Times each candidate workgroup size over `blocks_each` blocks of the live
+ * run (the results do not depend on it) and keeps the fastest. */
+as_status as_batch_autotune(as_batch* b, const int32_t* candidates, int32_t n, int32_t blocks_each,
+                            int32_t* chosen);
+as_status as_batch_sync(as_batch* b);
+as_status as_batch_get_info(as_batch* b, as_batch_info* info);
+/* Steps the member has run and the steps it was admitted for. */
+as_status as_batch_member_steps(as_batch* b, int32_t member, int64_t* done, int64_t* target);
+as_status as_batch_spike_count(as_batch* b, int32_t member, uint64_t* out);
+as_status as_batch_reset_ever_active(as_batch* b, int32_t member);
+as_status as_batch_read_state(as_batch* b, int32_t member, int32_t field, double* out);
+as_status as_batch_read_ever_active(as_batch* b, int32_t member, uint8_t* out);
+as_status as_batch_read_raster(as_batch* b, int32_t member, int64_t first_step, int32_t n_steps, uint8_t* out);
+as_status as_batch_set_timing(as_batch* b, int enable);
+as_status as_batch_kernel_time(as_batch* b, double* total_ms, int64_t* launches);
+void as_batch_destroy(as_batch* b);
+
 /* ---- misc ---- */
 /* Name of device 0 and its GCN arch ("gfx950"); AS_ERR_NODEVICE without a GPU. */
 as_status as_device_info(char* name, int name_len, char* arch, int arch_len);
