@@ -156,11 +156,11 @@ void parallel_for_threads(unsigned n, F&& f) {
 // draws a skip, so the draw sequence is exactly the reference's.
 class SkipWindows {
 public:
-    static constexpr std::uint64_t kWindow = std::uint64_t(1) << 22;
-
+    // window: counters per window (sized to the network so small builds do
+    // not precompute far past their last draw)
     SkipWindows(std::uint64_t seed, std::uint64_t stream, double log1m_p, std::uint64_t first,
-                unsigned threads)
-        : seed_(seed), stream_(stream), log1m_p_(log1m_p), threads_(threads) {
+                unsigned threads, std::uint64_t window)
+        : kWindow(window), seed_(seed), stream_(stream), log1m_p_(log1m_p), threads_(threads) {
         cur_.resize(kWindow);
         nxt_.resize(kWindow);
         cur_base_ = first;
@@ -194,6 +194,7 @@ private:
         });
     }
 
+    const std::uint64_t kWindow;
     std::uint64_t seed_, stream_;
     double log1m_p_;
     unsigned threads_;
@@ -207,9 +208,12 @@ private:
 // geometric_skip (src/astrosim_fused.cu:183, 617-624) draw for draw.
 class SetupDraws {
 public:
-    SetupDraws(std::uint64_t seed, std::uint64_t counter, double p, unsigned threads)
+    SetupDraws(std::uint64_t seed, std::uint64_t counter, double p, unsigned threads, std::uint64_t expected_draws)
         : seed_(seed), counter_(counter), p_(p) {
-        if (p > 0.0 && p < 1.0) skips_.reset(new SkipWindows(seed, kSetupStream, std::log1p(-p), counter, threads));
+        std::uint64_t window = std::uint64_t(1) << 16;
+        while (window < (std::uint64_t(1) << 22) && window * 8 < expected_draws) window <<= 1;
+        if (p > 0.0 && p < 1.0)
+            skips_.reset(new SkipWindows(seed, kSetupStream, std::log1p(-p), counter, threads, window));
     }
     double uniform() { return rng_uniform(seed_, kSetupStream, counter_++); }
     index_t skip(index_t remaining) {
@@ -329,7 +333,8 @@ void build_network(index_t n_total, std::uint64_t seed, Network& net) {
     // id (src/astrosim_fused.cu:712-723). Third-factor synapses are recorded
     // as (pre, post, pool slot) and resolved after the sequential pass.
     const double p = cfg.p_primary;
-    SetupDraws dr(seed, rng.counter(), p, threads);
+    // about 2.3 draws per synapse (skip, third-factor coin, pool pick on 20%)
+    SetupDraws dr(seed, rng.counter(), p, threads, static_cast<std::uint64_t>(2.5 * ScalingConfig::kSyn * n_neurons));
     SourceMajor exc_pass[2], inh_pass;
     std::vector<index_t> tf_pre, tf_post;
     std::vector<std::uint8_t> tf_slot;

@@ -115,6 +115,45 @@ def cpu_baseline(net, seed: int, n_steps: int) -> dict:
                       f"network (after 1 warm step), OpenMP over cells"}
 
 
+def ensemble_leg(pre_ms: float = 100.0, sim_ms: float = 500.0, n_sample: int = 8) -> dict:
+    """Batched ensemble: 256 networks, N_total log-uniform in [500, 200000]
+    from seed 3, seeds 1000+i, dt 0.1 ms, pre 100 ms + sim 500 ms, FP32c, in
+    one batch (autotuned tile size, hipGraph replays). Compared with running
+    networks one at a time: plain FP64 (the reference's method) and blocked
+    FP32c, each on an 8-network sample (the size quantiles) extrapolated to
+    the 256 networks by the sample's mean time per simulation."""
+    from astrosimgpu_cuda_amd import ensemble as E
+    steps = int((pre_ms + sim_ms) / 0.1 + 0.5)
+    sizes = E.ensemble_sizes(256)
+    t0 = time.perf_counter()
+    nets = E.build_networks(sizes)
+    log(f"ensemble: built 256 networks ({sum(sizes)} cells) in {time.perf_counter() - t0:.1f} s")
+    batched = E.run_batched(nets, steps, precision="fp32c")
+    log(f"ensemble: batched {batched['sims_per_s']:.2f} sims/s ({batched['wall_s']:.2f} s, "
+        f"tile {batched['block_threads']})")
+    order = sorted(range(len(nets)), key=lambda i: sizes[i])
+    pick = [order[int((k + 0.5) * len(order) / n_sample)] for k in range(n_sample)]
+    sample = [nets[i] for i in pick]
+    plain = E.run_one_at_a_time(sample, steps, "fp64", "plain")
+    log(f"ensemble: one-at-a-time plain FP64 {plain['sims_per_s']:.2f} sims/s (sample of {n_sample})")
+    blk = E.run_one_at_a_time(sample, steps, "fp32c", "blocked")
+    log(f"ensemble: one-at-a-time blocked FP32c {blk['sims_per_s']:.2f} sims/s (sample of {n_sample})")
+    same = all(batched["spikes"][i] == s for i, s in zip(pick, blk["spikes"]))
+    return {
+        "networks": 256, "cells": sum(sizes) + sum(max(1, round(n / 5)) for n in sizes),
+        "steps_per_sim": steps, "precision": "fp32c",
+        "batched_sims_per_s": batched["sims_per_s"], "batched_wall_s": batched["wall_s"],
+        "batched_setup_s": batched["setup_s"], "tile_threads": batched["block_threads"],
+        "bucket_overflows": batched["bucket_overflows"], "device_bytes": batched["device_bytes"],
+        "one_at_a_time_plain_fp64_sims_per_s": plain["sims_per_s"],
+        "one_at_a_time_blocked_fp32c_sims_per_s": blk["sims_per_s"],
+        "speedup_vs_plain_fp64": batched["sims_per_s"] / plain["sims_per_s"],
+        "speedup_vs_blocked_one_at_a_time": batched["sims_per_s"] / blk["sims_per_s"],
+        "sample": f"{n_sample} networks at the size quantiles, N={[sizes[i] for i in pick]}",
+        "members_equal_alone": same,
+    }
+
+
 def load_traffic(workload: str) -> float | None:
     p = ROOT / "profiles" / f"traffic_{workload}.json"
     if p.exists():
@@ -135,6 +174,7 @@ def main() -> int:
     ap.add_argument("--legs", default="fp64:plain,fp64:blocked,fp32c:blocked")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -162,8 +202,11 @@ def main() -> int:
         log("cpu baseline: FP64 host oracle")
         cpu = cpu_baseline(net, args.seed, args.cpu_steps)
         log(f"cpu baseline: {cpu['value']:.3f} steps/s on {cpu['cores']} threads")
+    ens = None
+    if not args.no_ensemble:
+        ens = ensemble_leg()
     line = {
-        "metric": "steps/s at N=10M FP32 vs FP64 (1 GPU)",
+        "metric": "steps/s at N=10M FP32 vs FP64 (1 GPU); ensemble sims/s + p50/p99 latency",
         "value": head["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
@@ -190,6 +233,7 @@ def main() -> int:
                      "reference_layout_equivalent_gbs": ref_equiv,
                      "note": "the blocked integrate kernel is FP64/FP32 vector-ALU bound; see DESIGN.md"},
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     print(json.dumps(line), flush=True)
     return 0
