@@ -21,7 +21,7 @@ import numpy as np
 
 __all__ = [
     "AstrosimError", "LibraryNotBuilt", "Network", "Simulation", "lib_path", "cli_path",
-    "device_info", "device_memory", "FIELDS", "FP64", "FP32C", "PLAIN", "BLOCKED", "load", "Batch",
+    "device_info", "device_memory", "FIELDS", "FP64", "FP32C", "PLAIN", "BLOCKED", "load", "Batch", "Server",
 ]
 
 PKG = Path(__file__).resolve().parent
@@ -77,6 +77,31 @@ class _BatchInfo(C.Structure):
                 ("bucket_overflows", C.c_uint64)]
 
 
+class _ServerConfig(C.Structure):
+    _fields_ = [("dt", C.c_double), ("block", C.c_int32), ("hbm_fraction", C.c_double),
+                ("fp64_share", C.c_double), ("build_threads", C.c_int32), ("blocks_per_round", C.c_int32),
+                ("max_members", C.c_int32)]
+
+
+class _Request(C.Structure):
+    _fields_ = [("n_total", C.c_uint32), ("seed", C.c_uint64), ("n_steps", C.c_int64), ("precision", C.c_int32)]
+
+
+class _RequestResult(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("n_neurons", C.c_uint32), ("n_astro", C.c_uint32),
+                ("t_submit", C.c_double), ("t_built", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("spikes", C.c_uint64), ("mean_ca", C.c_double),
+                ("active_astro", C.c_uint32)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("p50_latency_s", C.c_double), ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double),
+                ("sims_per_s", C.c_double), ("device_bytes", C.c_uint64), ("peak_active", C.c_uint64),
+                ("peak_resident_cells", C.c_uint64), ("capacity_cells", C.c_uint64), ("gpu_busy_s", C.c_double),
+                ("cell_steps", C.c_double)]
+
+
 _lib = None
 
 # (name, restype, argtypes) for every symbol include/astrosim.h declares.
@@ -124,6 +149,13 @@ SIGNATURES = [
     ("as_batch_set_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("as_batch_kernel_time", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     ("as_batch_destroy", None, [C.c_void_p]),
+    ("as_server_config_default", None, [C.POINTER(_ServerConfig)]),
+    ("as_server_create", C.c_int, [C.POINTER(_ServerConfig), C.POINTER(C.c_void_p)]),
+    ("as_server_submit", C.c_int, [C.c_void_p, C.POINTER(_Request), C.POINTER(C.c_int64)]),
+    ("as_server_result", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(_RequestResult)]),
+    ("as_server_wait_all", C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_int32)]),
+    ("as_server_get_stats", C.c_int, [C.c_void_p, C.POINTER(_ServerStats)]),
+    ("as_server_destroy", None, [C.c_void_p]),
     ("as_device_info", C.c_int, [C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
     ("as_device_memory", C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("as_last_error", C.c_char_p, []),
@@ -457,6 +489,66 @@ class Batch:
     def close(self) -> None:
         if self._h is not None:
             load().as_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+REQUEST_STATUS = {0: "building", 1: "queued", 2: "running", 3: "done", 4: "rejected"}
+
+
+class Server:
+    """Continuous-batching simulation server (native, include/astrosim.h):
+    requests are built on host threads, admitted into resident FP32c/FP64
+    batches as their footprint fits, advanced with both batches overlapping
+    on the device, and retired when done."""
+
+    def __init__(self, **kw):
+        lib = load()
+        cfg = _ServerConfig()
+        lib.as_server_config_default(C.byref(cfg))
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        h = C.c_void_p()
+        _check(lib.as_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def submit(self, n_total: int, seed: int, n_steps: int, precision: str = "fp32c") -> int:
+        r = _Request(n_total, seed, n_steps, {"fp64": FP64, "fp32c": FP32C}[precision])
+        out = C.c_int64()
+        _check(load().as_server_submit(self._h, C.byref(r), C.byref(out)))
+        return out.value
+
+    def result(self, rid: int) -> dict:
+        r = _RequestResult()
+        _check(load().as_server_result(self._h, rid, C.byref(r)))
+        d = {f: getattr(r, f) for f, _ in _RequestResult._fields_}
+        d["status"] = REQUEST_STATUS[d["status"]]
+        return d
+
+    def wait_all(self, timeout_s: float) -> bool:
+        done = C.c_int32()
+        _check(load().as_server_wait_all(self._h, timeout_s, C.byref(done)))
+        return bool(done.value)
+
+    def stats(self) -> dict:
+        s = _ServerStats()
+        _check(load().as_server_get_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _ServerStats._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            load().as_server_destroy(self._h)
             self._h = None
 
     def __enter__(self):

@@ -37,7 +37,7 @@ DEVICE = ["-x", "hip", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HOST = ["-D__HIP_PLATFORM_AMD__", "-I", str(ROCM / "include")]
 
-LIB_SOURCES = ["as_setup.cpp", "as_engine.cpp", "as_capi.cpp", "as_plain.hip", "as_batch.hip"]
+LIB_SOURCES = ["as_setup.cpp", "as_engine.cpp", "as_capi.cpp", "as_plain.hip", "as_batch.hip", "as_server.cpp"]
 HEADERS = ["as_dmath.h", "as_rng.h", "as_model.h", "as_model32.h", "as_setup.h", "as_engine.h", "as_batch.h"]
 
 

@@ -27,24 +27,22 @@ struct as_batch {
 
 namespace {
 
-thread_local std::string g_last_error;
-
 template <class F>
 as_status guarded(F&& f) {
     try {
         f();
         return AS_OK;
     } catch (const astrosim::Error& e) {
-        g_last_error = e.what();
+        astrosim::set_last_error(e.what());
         return e.code;
     } catch (const std::bad_alloc&) {
-        g_last_error = "host allocation failed";
+        astrosim::set_last_error("host allocation failed");
         return AS_ERR_NOMEM;
     } catch (const std::invalid_argument& e) {
-        g_last_error = e.what();
+        astrosim::set_last_error(e.what());
         return AS_ERR_INVALID;
     } catch (const std::exception& e) {
-        g_last_error = e.what();
+        astrosim::set_last_error(e.what());
         return AS_ERR_INVALID;
     }
 }
@@ -468,7 +466,7 @@ as_status as_device_memory(uint64_t* free_bytes, uint64_t* total_bytes) {
     });
 }
 
-const char* as_last_error(void) { return g_last_error.c_str(); }
+const char* as_last_error(void) { return astrosim::last_error(); }
 
 const char* as_version(void) { return "astrosim-mi355x 0.1.0 (gfx950)"; }
 

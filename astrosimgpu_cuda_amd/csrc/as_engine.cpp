@@ -6,6 +6,13 @@
 
 namespace astrosim {
 
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
 void throw_hip(hipError_t e, const char* file, int line, const char* expr) {
     char buf[512];
     std::snprintf(buf, sizeof buf, "HIP error %s:%d: %s (%s)", file, line, hipGetErrorString(e), expr);

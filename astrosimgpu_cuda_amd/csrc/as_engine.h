@@ -20,6 +20,10 @@ struct Error : std::runtime_error {
     Error(as_status c, const std::string& msg) : std::runtime_error(msg), code(c) {}
 };
 
+// Message of the last failed C-ABI call on this thread (as_last_error).
+void set_last_error(const std::string& msg);
+const char* last_error();
+
 [[noreturn]] void throw_hip(hipError_t e, const char* file, int line, const char* expr);
 
 #define AS_HIP_CHECK(call)                                                        \

@@ -1,0 +1,435 @@
+// as_server.cpp -- continuous-batching simulation server.
+//
+// Requests (N_total, seed, steps, precision) arrive at any time. A pool of
+// host threads builds each request's network (connectivity and parameters);
+// a driver thread admits built networks into one of two resident batches
+// (FP32c and FP64, each on its own HIP stream, capacities sized from an HBM
+// budget) as soon as their ranges fit, advances both batches a few blocks at
+// a time (the two streams overlap on the device), and retires members that
+// reached their step count, reading their results and freeing their ranges
+// for the next admissions. No request waits for a batch to drain.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/astrosim.h"
+#include "as_batch.h"
+#include "as_engine.h"
+#include "as_setup.h"
+
+namespace astrosim {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Req {
+    as_request in{};
+    as_request_result out{};
+    std::unique_ptr<Network> net;
+    int member = -1;
+    int which = 0;  // batch: 0 FP32c, 1 FP64
+};
+
+// Capacity whose footprint fits `bytes`, with the scaling law's proportions
+// (per neuron: 80 exc, 20 inh, 16 sic in-edges; per astrocyte 80 n2a) and 6%
+// headroom for the binomial spread of edge counts.
+BatchCapacity capacity_for_budget(std::uint64_t bytes, const RunParams& rp, std::uint32_t members) {
+    auto cap_for = [&](std::uint64_t neurons) {
+        BatchCapacity c;
+        c.neurons = neurons;
+        c.astros = neurons / 4 + 64;
+        c.edges[0] = neurons * 85;
+        c.edges[1] = neurons * 22;
+        c.edges[2] = neurons * 18;
+        c.edges[3] = neurons * 18;
+        c.members = members;
+        return c;
+    };
+    // the largest edge space (85 per neuron) must stay below 2^32 ids
+    std::uint64_t lo = 1024, hi = 4000000000ull / 85;
+    if (batch_footprint(cap_for(lo), rp) > bytes) throw Error(AS_ERR_NOMEM, "HBM budget too small for a batch");
+    while (hi - lo > 1024) {
+        const std::uint64_t mid = (lo + hi) / 2;
+        if (batch_footprint(cap_for(mid), rp) <= bytes) lo = mid;
+        else hi = mid;
+    }
+    return cap_for(lo);
+}
+
+}  // namespace
+
+class Server {
+public:
+    Server(const as_server_config& c) : cfg_(c), t0_(Clock::now()) {
+        as_config base;
+        as_config_default(&base);
+        base.dt = c.dt;
+        base.block = c.block;
+        base.method = AS_METHOD_BLOCKED;
+        as_config c32 = base, c64 = base;
+        c32.precision = AS_FP32C;
+        c64.precision = AS_FP64;
+        rp_[0] = resolve_params(c32);
+        rp_[1] = resolve_params(c64);
+        size_t free_b = 0, total_b = 0;
+        AS_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+        const double frac = c.hbm_fraction > 0.0 && c.hbm_fraction < 0.95 ? c.hbm_fraction : 0.5;
+        const std::uint64_t budget = static_cast<std::uint64_t>(frac * double(free_b));
+        const double share64 = std::min(0.9, std::max(0.05, c.fp64_share));
+        const std::uint32_t members = std::max(8, c.max_members);
+        cap_[0] = capacity_for_budget(std::uint64_t(budget * (1.0 - share64)), rp_[0], members);
+        cap_[1] = capacity_for_budget(std::uint64_t(budget * share64), rp_[1], members);
+        for (int i = 0; i < 2; ++i) {
+            AS_HIP_CHECK(hipStreamCreateWithFlags(&stream_[i], hipStreamNonBlocking));
+            batch_[i] = make_batch(rp_[i], cap_[i], stream_[i]);
+        }
+        const int nb = std::max(1, c.build_threads);
+        for (int i = 0; i < nb; ++i) builders_.emplace_back([this] { build_loop(); });
+        driver_ = std::thread([this] { drive_loop(); });
+    }
+
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_build_.notify_all();
+        cv_ready_.notify_all();
+        for (auto& t : builders_) t.join();
+        if (driver_.joinable()) driver_.join();
+        for (int i = 0; i < 2; ++i) {
+            batch_[i].reset();
+            if (stream_[i]) (void)hipStreamDestroy(stream_[i]);
+        }
+    }
+
+    std::int64_t submit(const as_request& r) {
+        if (r.n_total == 0 || r.n_total > kMaxNTotal) throw Error(AS_ERR_INVALID, "n_total out of range");
+        if (r.n_steps < 0) throw Error(AS_ERR_INVALID, "n_steps must be >= 0");
+        if (r.precision != AS_FP32C && r.precision != AS_FP64) throw Error(AS_ERR_INVALID, "unknown precision");
+        std::lock_guard<std::mutex> g(mu_);
+        auto q = std::make_shared<Req>();
+        q->in = r;
+        q->out.id = static_cast<std::int64_t>(reqs_.size());
+        q->out.status = AS_REQ_BUILDING;
+        q->out.t_submit = now();
+        q->which = r.precision == AS_FP64 ? 1 : 0;
+        reqs_.push_back(q);
+        build_q_.push_back(q);
+        cv_build_.notify_one();
+        return q->out.id;
+    }
+
+    as_request_result result(std::int64_t id) {
+        std::lock_guard<std::mutex> g(mu_);
+        if (id < 0 || id >= std::int64_t(reqs_.size())) throw Error(AS_ERR_INVALID, "no such request");
+        return reqs_[id]->out;
+    }
+
+    bool wait_all(double timeout_s) {
+        std::unique_lock<std::mutex> g(mu_);
+        return cv_done_.wait_for(g, std::chrono::duration<double>(timeout_s), [&] { return finished_ == reqs_.size(); });
+    }
+
+    as_server_stats stats() {
+        std::lock_guard<std::mutex> g(mu_);
+        as_server_stats s{};
+        s.submitted = reqs_.size();
+        std::vector<double> lat;
+        double first_submit = 1e300, last_done = 0.0;
+        for (auto& r : reqs_) {
+            first_submit = std::min(first_submit, r->out.t_submit);
+            if (r->out.status == AS_REQ_DONE) {
+                lat.push_back(r->out.t_done - r->out.t_submit);
+                last_done = std::max(last_done, r->out.t_done);
+            }
+            if (r->out.status == AS_REQ_REJECTED) ++s.rejected;
+        }
+        s.completed = lat.size();
+        if (!lat.empty()) {
+            std::sort(lat.begin(), lat.end());
+            auto pct = [&](double p) {
+                const double k = p * (lat.size() - 1);
+                const std::size_t i = static_cast<std::size_t>(k);
+                const double f = k - i;
+                return i + 1 < lat.size() ? lat[i] * (1 - f) + lat[i + 1] * f : lat[i];
+            };
+            s.p50_latency_s = pct(0.50);
+            s.p99_latency_s = pct(0.99);
+            s.mean_latency_s = 0;
+            for (double v : lat) s.mean_latency_s += v;
+            s.mean_latency_s /= lat.size();
+            s.sims_per_s = last_done > first_submit ? lat.size() / (last_done - first_submit) : 0.0;
+        }
+        s.device_bytes = batch_[0]->device_bytes() + batch_[1]->device_bytes();
+        s.peak_active = peak_active_;
+        s.peak_resident_cells = peak_cells_;
+        s.capacity_cells = cap_[0].neurons + cap_[0].astros + cap_[1].neurons + cap_[1].astros;
+        s.gpu_busy_s = gpu_busy_s_;
+        s.cell_steps = cell_steps_;
+        return s;
+    }
+
+private:
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0_).count(); }
+
+    void build_loop() {
+        for (;;) {
+            std::shared_ptr<Req> q;
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_build_.wait(g, [&] { return stop_ || !build_q_.empty(); });
+                if (stop_) return;
+                q = build_q_.front();
+                build_q_.pop_front();
+            }
+            auto net = std::make_unique<Network>();
+            bool ok = true;
+            try {
+                build_network(q->in.n_total, q->in.seed, *net);
+            } catch (...) {
+                ok = false;
+            }
+            std::lock_guard<std::mutex> g(mu_);
+            q->out.n_neurons = ok ? net->cfg.n_neurons() : 0;
+            q->out.n_astro = ok ? net->cfg.n_astro : 0;
+            q->out.t_built = now();
+            if (!ok) {
+                q->out.status = AS_REQ_REJECTED;
+                ++finished_;
+                cv_done_.notify_all();
+                continue;
+            }
+            q->net = std::move(net);
+            q->out.status = AS_REQ_QUEUED;
+            ready_[q->which].push_back(q);
+            cv_ready_.notify_one();
+        }
+    }
+
+    // Admits queued requests in arrival order while they fit. A request that
+    // can never fit the empty batch is rejected.
+    void admit() {
+        for (int b = 0; b < 2; ++b) {
+            for (;;) {
+                std::shared_ptr<Req> q;
+                {
+                    std::lock_guard<std::mutex> g(mu_);
+                    if (ready_[b].empty()) break;
+                    q = ready_[b].front();
+                }
+                Batch& B = *batch_[b];
+                if (!B.fits(*q->net)) {
+                    const Network* p = q->net.get();
+                    const BatchCapacity need = capacity_for(&p, 1);
+                    const bool never = need.neurons > cap_[b].neurons || need.astros > cap_[b].astros ||
+                                       need.edges[0] > cap_[b].edges[0] || need.edges[1] > cap_[b].edges[1] ||
+                                       need.edges[2] > cap_[b].edges[2] || need.edges[3] > cap_[b].edges[3];
+                    if (!never) break;  // wait for ranges to free up (FIFO: no overtaking)
+                    std::lock_guard<std::mutex> g(mu_);
+                    ready_[b].pop_front();
+                    q->out.status = AS_REQ_REJECTED;
+                    q->net.reset();
+                    ++finished_;
+                    cv_done_.notify_all();
+                    continue;
+                }
+                q->member = B.add(*q->net, q->in.n_steps);
+                std::lock_guard<std::mutex> g(mu_);
+                ready_[b].pop_front();
+                q->net.reset();  // resident in HBM now
+                q->out.status = AS_REQ_RUNNING;
+                q->out.t_admit = now();
+                running_[b].push_back(q);
+            }
+        }
+    }
+
+    void retire() {
+        for (int b = 0; b < 2; ++b) {
+            Batch& B = *batch_[b];
+            std::vector<std::shared_ptr<Req>> still;
+            for (auto& q : running_[b]) {
+                if (B.member_steps(q->member) < q->in.n_steps) {
+                    still.push_back(q);
+                    continue;
+                }
+                const std::uint64_t spikes = B.spike_count(q->member);
+                std::vector<double> ca(q->out.n_astro);
+                if (!ca.empty()) B.read_state(q->member, 0, ca.data());
+                double mean = 0.0;
+                std::uint32_t active = 0;
+                for (double c : ca) {
+                    mean += c;
+                    if ((c - rp_[b].SIC_th) * 1000.0 > 1.0) ++active;
+                }
+                B.remove(q->member);
+                std::lock_guard<std::mutex> g(mu_);
+                q->out.spikes = spikes;
+                q->out.mean_ca = ca.empty() ? 0.0 : mean / ca.size();
+                q->out.active_astro = active;
+                q->out.status = AS_REQ_DONE;
+                q->out.t_done = now();
+                ++finished_;
+                cv_done_.notify_all();
+            }
+            running_[b].swap(still);
+        }
+    }
+
+    void drive_loop() {
+        const int blocks = std::max(1, cfg_.blocks_per_round);
+        try {
+            for (;;) {
+                {
+                    std::unique_lock<std::mutex> g(mu_);
+                    cv_ready_.wait_for(g, std::chrono::milliseconds(2), [&] {
+                        return stop_ || !ready_[0].empty() || !ready_[1].empty() || !running_[0].empty() ||
+                               !running_[1].empty();
+                    });
+                    if (stop_) return;
+                }
+                admit();
+                std::uint64_t cells = 0;
+                int active = 0;
+                const auto g0 = Clock::now();
+                bool any = false;
+                for (int b = 0; b < 2; ++b) {
+                    if (running_[b].empty()) continue;
+                    any = true;
+                    for (auto& q : running_[b]) cells += q->out.n_neurons + q->out.n_astro;
+                    active += int(running_[b].size());
+                    batch_[b]->enqueue_steps(std::int64_t(blocks) * batch_[b]->block_len());  // async
+                }
+                if (!any) continue;
+                for (int b = 0; b < 2; ++b) AS_HIP_CHECK(hipStreamSynchronize(stream_[b]));
+                gpu_busy_s_ += std::chrono::duration<double>(Clock::now() - g0).count();
+                cell_steps_ += double(cells) * blocks * batch_[0]->block_len();
+                peak_active_ = std::max<std::uint64_t>(peak_active_, active);
+                peak_cells_ = std::max<std::uint64_t>(peak_cells_, cells);
+                retire();
+            }
+        } catch (const std::exception& e) {
+            std::lock_guard<std::mutex> g(mu_);
+            error_ = e.what();
+            stop_ = true;
+            // fail every unfinished request so waiters return
+            for (auto& r : reqs_)
+                if (r->out.status != AS_REQ_DONE && r->out.status != AS_REQ_REJECTED) {
+                    r->out.status = AS_REQ_REJECTED;
+                    ++finished_;
+                }
+            cv_done_.notify_all();
+        }
+    }
+
+    as_server_config cfg_;
+    Clock::time_point t0_;
+    RunParams rp_[2];
+    BatchCapacity cap_[2];
+    hipStream_t stream_[2] = {nullptr, nullptr};
+    std::unique_ptr<Batch> batch_[2];
+    std::mutex mu_;
+    std::condition_variable cv_build_, cv_ready_, cv_done_;
+    std::vector<std::shared_ptr<Req>> reqs_;
+    std::deque<std::shared_ptr<Req>> build_q_, ready_[2];
+    std::vector<std::shared_ptr<Req>> running_[2];
+    std::size_t finished_ = 0;
+    bool stop_ = false;
+    std::string error_;
+    std::vector<std::thread> builders_;
+    std::thread driver_;
+    std::uint64_t peak_active_ = 0, peak_cells_ = 0;
+    double gpu_busy_s_ = 0.0, cell_steps_ = 0.0;
+};
+
+}  // namespace astrosim
+
+// The C ABI of the server lives with the other entry points' conventions.
+struct as_server {
+    std::unique_ptr<astrosim::Server> s;
+};
+
+namespace {
+template <class F>
+as_status server_guard(F&& f) {
+    try {
+        f();
+        return AS_OK;
+    } catch (const astrosim::Error& e) {
+        astrosim::set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        astrosim::set_last_error(e.what());
+        return AS_ERR_INVALID;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+void as_server_config_default(as_server_config* c) {
+    if (!c) return;
+    c->dt = 0.1;
+    c->block = 128;
+    c->hbm_fraction = 0.5;
+    c->fp64_share = 0.3;
+    c->build_threads = 8;
+    c->blocks_per_round = 4;
+    c->max_members = 512;
+}
+
+as_status as_server_create(const as_server_config* cfg, as_server** out) {
+    return server_guard([&] {
+        if (!cfg || !out) throw astrosim::Error(AS_ERR_INVALID, "NULL argument");
+        *out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            (void)hipGetLastError();
+            throw astrosim::Error(AS_ERR_NODEVICE, "No GPU device visible.");
+        }
+        std::unique_ptr<as_server> s(new as_server);
+        s->s.reset(new astrosim::Server(*cfg));
+        *out = s.release();
+    });
+}
+
+as_status as_server_submit(as_server* s, const as_request* r, int64_t* id) {
+    return server_guard([&] {
+        if (!s || !r || !id) throw astrosim::Error(AS_ERR_INVALID, "NULL argument");
+        *id = s->s->submit(*r);
+    });
+}
+
+as_status as_server_result(as_server* s, int64_t id, as_request_result* out) {
+    return server_guard([&] {
+        if (!s || !out) throw astrosim::Error(AS_ERR_INVALID, "NULL argument");
+        *out = s->s->result(id);
+    });
+}
+
+as_status as_server_wait_all(as_server* s, double timeout_s, int32_t* all_done) {
+    return server_guard([&] {
+        if (!s || !all_done) throw astrosim::Error(AS_ERR_INVALID, "NULL argument");
+        *all_done = s->s->wait_all(timeout_s) ? 1 : 0;
+    });
+}
+
+as_status as_server_get_stats(as_server* s, as_server_stats* out) {
+    return server_guard([&] {
+        if (!s || !out) throw astrosim::Error(AS_ERR_INVALID, "NULL argument");
+        *out = s->s->stats();
+    });
+}
+
+void as_server_destroy(as_server* s) { delete s; }
+
+}  // extern "C"

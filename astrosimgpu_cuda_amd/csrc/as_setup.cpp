@@ -288,7 +288,8 @@ void build_network(index_t n_total, std::uint64_t seed, Network& net) {
     net.seed = seed;
     const index_t n_exc = cfg.n_exc, n_inh = cfg.n_inh, n_astro = cfg.n_astro;
     const index_t n_neurons = n_exc + n_inh;
-    const unsigned threads = worker_count();
+    // threads for this build: small networks are usually built many at a time
+    const unsigned threads = std::max(1u, std::min(worker_count(), static_cast<unsigned>(n_total / 65536u) + 1u));
     const auto t0 = std::chrono::steady_clock::now();
 
     // ---- connectivity: one sequential setup stream (src/astrosim_fused.cu:631-735) ----

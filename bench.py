@@ -154,6 +154,37 @@ def ensemble_leg(pre_ms: float = 100.0, sim_ms: float = 500.0, n_sample: int = 8
     }
 
 
+def serving_leg(duration_s: float) -> dict:
+    """Serving load (40 req/s Poisson arrivals, N_total log-uniform in
+    [1e3, 1e6], sim time uniform in [200, 2000] ms, 20% FP64, own seeds)
+    replayed in real time for `duration_s` seconds of arrivals against the
+    continuous-batching server; latency is submit -> result."""
+    from astrosimgpu_cuda_amd import Server, serving
+    load = serving.poisson_load(duration_s=duration_s)
+    with Server(hbm_fraction=0.5, fp64_share=0.35, build_threads=12, blocks_per_round=4) as srv:
+        out = serving.replay(srv, load, timeout_s=900.0)
+    st = out["stats"]
+    log(f"serving: {st['completed']}/{len(load)} done, p50 {st['p50_latency_s']:.2f} s, "
+        f"p99 {st['p99_latency_s']:.2f} s, {st['sims_per_s']:.2f} sims/s")
+    res = out["results"]
+    build = [r["t_built"] - r["t_submit"] for r in res if r["status"] == "done"]
+    queue = [r["t_admit"] - r["t_built"] for r in res if r["status"] == "done"]
+    run = [r["t_done"] - r["t_admit"] for r in res if r["status"] == "done"]
+    q = lambda v, p: float(sorted(v)[min(len(v) - 1, int(p * len(v)))]) if v else None
+    return {
+        "arrival_rate_hz": 40.0, "arrival_window_s": duration_s, "requests": len(load),
+        "completed": st["completed"], "rejected": st["rejected"], "all_done": st["all_done"],
+        "p50_latency_s": st["p50_latency_s"], "p99_latency_s": st["p99_latency_s"],
+        "mean_latency_s": st["mean_latency_s"], "sims_per_s": st["sims_per_s"],
+        "hbm_bytes": st["device_bytes"], "peak_active": st["peak_active"],
+        "peak_resident_cells": st["peak_resident_cells"], "capacity_cells": st["capacity_cells"],
+        "hbm_occupancy_peak": st["peak_resident_cells"] / max(1, st["capacity_cells"]),
+        "gpu_busy_s": st["gpu_busy_s"], "cell_steps_per_s": st["cell_steps"] / max(1e-9, st["gpu_busy_s"]),
+        "p50_build_s": q(build, 0.5), "p50_queue_s": q(queue, 0.5), "p50_run_s": q(run, 0.5),
+        "wall_s": st["wall_s"],
+    }
+
+
 def load_traffic(workload: str) -> float | None:
     p = ROOT / "profiles" / f"traffic_{workload}.json"
     if p.exists():
@@ -175,6 +206,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=10.0,
+                    help="arrival window of the serving load (0 skips it; the full load is 120 s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -205,6 +238,7 @@ def main() -> int:
     ens = None
     if not args.no_ensemble:
         ens = ensemble_leg()
+    srv = serving_leg(args.serving_seconds) if args.serving_seconds > 0 else None
     line = {
         "metric": "steps/s at N=10M FP32 vs FP64 (1 GPU); ensemble sims/s + p50/p99 latency",
         "value": head["steps_per_s"],
@@ -234,6 +268,7 @@ def main() -> int:
                      "note": "the blocked integrate kernel is FP64/FP32 vector-ALU bound; see DESIGN.md"},
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "serving": srv,
     }
     print(json.dumps(line), flush=True)
     return 0

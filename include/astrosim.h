@@ -188,6 +188,60 @@ as_status as_batch_set_timing(as_batch* b, int enable);
 as_status as_batch_kernel_time(as_batch* b, double* total_ms, int64_t* launches);
 void as_batch_destroy(as_batch* b);
 
+/* ---- continuous-batching server ----
+ *
+ * No counterpart in the reference. Requests are built on host worker
+ * threads, admitted into resident FP32c / FP64 batches (capacities sized from
+ * a fraction of free HBM) as soon as their footprint fits, advanced a few
+ * blocks at a time with both batches' streams overlapping on the device, and
+ * retired when they reach their step count. Results equal a solo run. */
+typedef struct as_server as_server;
+typedef struct as_server_config {
+    double dt;                /* ms */
+    int32_t block;            /* workgroup size */
+    double hbm_fraction;      /* share of free HBM given to the two batches */
+    double fp64_share;        /* share of that budget given to the FP64 batch */
+    int32_t build_threads;    /* host network builders */
+    int32_t blocks_per_round; /* blocks advanced between admission checks */
+    int32_t max_members;      /* member slots per batch */
+} as_server_config;
+typedef struct as_request {
+    uint32_t n_total;   /* --n */
+    uint64_t seed;      /* --seed */
+    int64_t n_steps;    /* steps to simulate */
+    int32_t precision;  /* AS_FP32C or AS_FP64 */
+} as_request;
+enum { AS_REQ_BUILDING = 0, AS_REQ_QUEUED = 1, AS_REQ_RUNNING = 2, AS_REQ_DONE = 3, AS_REQ_REJECTED = 4 };
+typedef struct as_request_result {
+    int64_t id;
+    int32_t status;               /* AS_REQ_* */
+    uint32_t n_neurons, n_astro;
+    double t_submit, t_built, t_admit, t_done;  /* seconds since the server started */
+    uint64_t spikes;              /* spikes over the whole run */
+    double mean_ca;               /* final mean calcium */
+    uint32_t active_astro;        /* astrocytes above the SIC threshold at the end */
+} as_request_result;
+typedef struct as_server_stats {
+    uint64_t submitted, completed, rejected;
+    double p50_latency_s, p99_latency_s, mean_latency_s;  /* submit -> done */
+    double sims_per_s;            /* completed / (last done - first submit) */
+    uint64_t device_bytes;        /* HBM held by the resident batches */
+    uint64_t peak_active;         /* most requests resident at once */
+    uint64_t peak_resident_cells, capacity_cells;
+    double gpu_busy_s;            /* wall time of the device rounds */
+    double cell_steps;            /* cells x steps advanced */
+} as_server_stats;
+
+void as_server_config_default(as_server_config* cfg);
+as_status as_server_create(const as_server_config* cfg, as_server** out);
+as_status as_server_submit(as_server* s, const as_request* req, int64_t* id);
+as_status as_server_result(as_server* s, int64_t id, as_request_result* out);
+/* Waits until every submitted request is done or rejected (*all_done = 1) or
+ * the timeout passes (*all_done = 0). */
+as_status as_server_wait_all(as_server* s, double timeout_s, int32_t* all_done);
+as_status as_server_get_stats(as_server* s, as_server_stats* out);
+void as_server_destroy(as_server* s);
+
 /* ---- misc ---- */
 /* Name of device 0 and its GCN arch ("gfx950"); AS_ERR_NODEVICE without a GPU. */
 as_status as_device_info(char* name, int name_len, char* arch, int arch_len);

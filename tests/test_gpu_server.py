@@ -1,0 +1,42 @@
+"""The continuous-batching server: requests arriving while others run are
+admitted as capacity frees up, and each result is bit-identical to running
+the same network alone (FP32c and FP64)."""
+import numpy as np
+import pytest
+
+import astrosimgpu_cuda_amd as A
+from astrosimgpu_cuda_amd import serving
+
+pytestmark = pytest.mark.gpu
+
+
+def solo(n_total, seed, steps, precision):
+    net = A.Network(n_total, seed)
+    sim = A.Simulation(net, method="blocked", precision=precision)
+    sim.step(steps)
+    ca = sim.state("Ca")
+    out = (sim.spike_count(), ca.mean(), int(np.sum((ca - 0.19669) * 1000.0 > 1.0)))
+    sim.close()
+    return out
+
+
+def test_server_results_equal_solo_runs():
+    load = serving.poisson_load(rate_hz=20.0, duration_s=1.0, n_lo=1000, n_hi=30000, sim_lo_ms=20.0,
+                                sim_hi_ms=80.0, fp64_fraction=0.3, seed=9)
+    print(f"[server] {len(load)} requests", flush=True)
+    assert len(load) >= 8
+    # a small arena so that later requests must wait for earlier ones to leave
+    with A.Server(hbm_fraction=0.002, fp64_share=0.3, build_threads=4, blocks_per_round=2) as srv:
+        out = serving.replay(srv, load, timeout_s=240.0)
+    st = out["stats"]
+    print(f"[server] stats {st}", flush=True)
+    assert st["all_done"] and st["completed"] == len(load) and st["rejected"] == 0
+    for req, res in zip(load, out["results"]):
+        assert res["status"] == "done"
+        assert res["t_submit"] <= res["t_built"] <= res["t_admit"] <= res["t_done"]
+        spikes, mean_ca, active = solo(req["n_total"], req["seed"], req["n_steps"], req["precision"])
+        assert res["spikes"] == spikes, (req, res)
+        # the calcium values are bit-identical; only the summation of the mean differs
+        assert res["mean_ca"] == pytest.approx(mean_ca, rel=1e-12), (req, res)
+        assert res["active_astro"] == active, (req, res)
+    assert st["peak_active"] >= 2  # requests overlapped on the device
