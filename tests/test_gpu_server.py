@@ -21,8 +21,9 @@ def solo(n_total, seed, steps, precision):
 
 
 def test_server_results_equal_solo_runs():
-    load = serving.poisson_load(rate_hz=20.0, duration_s=1.0, n_lo=1000, n_hi=30000, sim_lo_ms=20.0,
-                                sim_hi_ms=80.0, fp64_fraction=0.3, seed=9)
+    # a near-simultaneous burst, so requests arrive while others run
+    load = serving.poisson_load(rate_hz=400.0, duration_s=0.05, n_lo=1000, n_hi=30000, sim_lo_ms=100.0,
+                                sim_hi_ms=300.0, fp64_fraction=0.3, seed=9)
     print(f"[server] {len(load)} requests", flush=True)
     assert len(load) >= 8
     # a small arena so that later requests must wait for earlier ones to leave
