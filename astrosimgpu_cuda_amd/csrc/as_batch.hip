@@ -51,8 +51,11 @@ struct Member {
     std::uint32_t pad;
 };
 
+// A workgroup's cells: local ids [first, end) of one member's astrocytes
+// (kind 0), excitatory neurons (kind 1) or inhibitory neurons (kind 2). One
+// population per tile keeps the population's constants wave-uniform.
 struct Tile {
-    std::uint32_t member, first, kind, pad;  // kind 0: astrocytes, 1: neurons
+    std::uint32_t member, first, kind, end;
 };
 
 // Everything that does not depend on the state precision.
@@ -287,10 +290,9 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
 template <class R>
 __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
                                 const Member& M, const BlkLaunch& B, index_t gj, index_t j, std::int64_t lo,
-                                std::int64_t hi, bool* emit) {
+                                std::int64_t hi, bool exc, bool* emit) {
     const std::size_t rN = std::size_t(B.rslot) * D.ncap;
     const std::size_t rA = std::size_t(B.rslot) * D.acap;
-    const bool exc = j < M.ne;
     const unsigned ce = D.bcnt[T_EXC][gj], ci = D.bcnt[T_INH][gj], cs = D.bcnt[T_SIC][gj];
     const uint2* ee = D.bent[T_EXC] + std::size_t(gj) * kBucketCap;
     const uint2* ei = D.bent[T_INH] + std::size_t(gj) * kBucketCap;
@@ -407,9 +409,23 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     return nspikes;
 }
 
+// Everything an integrate launch reads besides the block descriptor, kept in
+// device memory: uniform loads go through the scalar cache on demand instead
+// of occupying (and spilling) scalar registers as kernel arguments.
+template <class R>
+struct IntegrateCtx {
+    BatchDev D;
+    BlkState<R> S;
+    RunParams P;
+    Params32 Q;
+};
+
 template <int MAXT, class R>
-__global__ __launch_bounds__(MAXT) void integrate_kernel(BatchDev D, BlkState<R> S, RunParams P, Params32 Q,
-                                                         BlkLaunch B) {
+__global__ __launch_bounds__(MAXT) void integrate_kernel(const IntegrateCtx<R>* __restrict__ ctx, BlkLaunch B) {
+    const BatchDev& D = ctx->D;
+    const BlkState<R>& S = ctx->S;
+    const RunParams& P = ctx->P;
+    const Params32& Q = ctx->Q;
     resolve_launch(B, D.clock);
     const Tile tile = D.tiles[blockIdx.x];
     const Member M = D.members[tile.member];
@@ -419,15 +435,15 @@ __global__ __launch_bounds__(MAXT) void integrate_kernel(BatchDev D, BlkState<R>
     const std::int64_t hi = end < M.step_end ? end : M.step_end;
     const index_t local = tile.first + threadIdx.x;
     if (tile.kind == 0) {
-        const bool run = local < M.na && lo < hi;
+        const bool run = local < tile.end && lo < hi;
         const bool emit = run ? astro_cell<R>(D, S, P, Q, M, B, M.ab + local, local, lo, hi) : false;
         wave_append(emit, M.ab + local, D.alist + std::size_t(B.wslot) * D.acap, &D.lcount[B.wslot * 2 + 1]);
         return;
     }
-    const bool run = local < M.nn && lo < hi;
+    const bool run = local < tile.end && lo < hi;
     bool emit = false;
     unsigned nspikes = 0;
-    if (run) nspikes = neuron_cell<R>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, &emit);
+    if (run) nspikes = neuron_cell<R>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, tile.kind == 1, &emit);
     wave_append(emit, M.nb + local, D.nlist + std::size_t(B.wslot) * D.ncap, &D.lcount[B.wslot * 2 + 0]);
     // diagnostic spike total of the member: one atomic per wave with spikes
     unsigned long long c = nspikes;
@@ -558,7 +574,7 @@ struct BatchLayout {
     std::size_t in_rs[4], in_src[4], out_rs[4], out_ent[4], bcnt[4], bent[4];
     std::size_t astro[10], ever, neuron[12], refr;
     std::size_t spk_w, spk_v, sic_w, sic_v, nlist, alist, lcount, counters, overflow;
-    std::size_t members, tiles, clock, scratch0, scratch1, scan_tmp;
+    std::size_t members, tiles, clock, ctx, scratch0, scratch1, scan_tmp;
     std::size_t scan_tmp_bytes, tiles_cap;
     std::size_t total;
 };
@@ -602,9 +618,10 @@ BatchLayout batch_layout(const BatchCapacity& c, const RunParams& rp) {
     o.counters = L.add(std::size_t(c.members) * kCountersPerMember * sizeof(unsigned long long));
     o.overflow = L.add(sizeof(unsigned));
     o.members = L.add(std::size_t(c.members) * sizeof(Member));
-    o.tiles_cap = (nn + 63) / 64 + (na + 63) / 64 + 2 * std::size_t(c.members);
+    o.tiles_cap = (nn + 63) / 64 + (na + 63) / 64 + 3 * std::size_t(c.members);
     o.tiles = L.add(o.tiles_cap * sizeof(Tile));
     o.clock = L.add(sizeof(std::int64_t));
+    o.ctx = L.add(2 * sizeof(IntegrateCtx<double>));  // eager and graph-replay contexts
     const std::uint64_t sm = std::max(nn, na) + 1;
     o.scratch0 = L.add(sm * sizeof(index_t));
     o.scratch1 = L.add(sm * sizeof(index_t));
@@ -676,6 +693,15 @@ public:
         scratch0_ = (index_t*)dptr(lay_.scratch0);
         scratch1_ = (index_t*)dptr(lay_.scratch1);
         scan_tmp_ = dptr(lay_.scan_tmp);
+        ctx_eager_ = reinterpret_cast<IntegrateCtx<R>*>(dptr(lay_.ctx));
+        ctx_graph_ = ctx_eager_ + 1;
+        {
+            IntegrateCtx<R> host_ctx[2];
+            host_ctx[0] = {D_, S_, P_, Q_};
+            host_ctx[1] = host_ctx[0];
+            host_ctx[1].D.clock = clock_;
+            AS_HIP_CHECK(hipMemcpy(ctx_eager_, host_ctx, sizeof host_ctx, hipMemcpyHostToDevice));
+        }
         AS_HIP_CHECK(hipMemsetAsync(D_.lcount, 0, std::size_t(geo_.H) * 2 * 4, stream_));
         AS_HIP_CHECK(hipMemsetAsync(D_.overflow, 0, sizeof(unsigned), stream_));
         AS_HIP_CHECK(hipMemsetAsync(D_.counters, 0, std::size_t(cap.members) * kCountersPerMember * 8, stream_));
@@ -965,10 +991,11 @@ private:
 
     void launch_integrate(const BatchDev& D, const BlkLaunch& B) {
         if (n_tiles_ == 0) return;
+        const IntegrateCtx<R>* ctx = D.clock ? ctx_graph_ : ctx_eager_;
         if (threads_ <= 256)
-            hipLaunchKernelGGL((integrate_kernel<256, R>), dim3(n_tiles_), dim3(threads_), 0, stream_, D, S_, P_, Q_, B);
+            hipLaunchKernelGGL((integrate_kernel<256, R>), dim3(n_tiles_), dim3(threads_), 0, stream_, ctx, B);
         else
-            hipLaunchKernelGGL((integrate_kernel<1024, R>), dim3(n_tiles_), dim3(threads_), 0, stream_, D, S_, P_, Q_, B);
+            hipLaunchKernelGGL((integrate_kernel<1024, R>), dim3(n_tiles_), dim3(threads_), 0, stream_, ctx, B);
     }
 
     void timing_begin() {
@@ -999,8 +1026,10 @@ private:
             const MemberHost& h = hosts_[i];
             ms[i] = h.m;
             if (!h.active) continue;
-            for (index_t f = 0; f < h.m.na; f += threads_) tiles.push_back({std::uint32_t(i), f, 0u, 0u});
-            for (index_t f = 0; f < h.m.nn; f += threads_) tiles.push_back({std::uint32_t(i), f, 1u, 0u});
+            const std::uint32_t mi = std::uint32_t(i);
+            for (index_t f = 0; f < h.m.na; f += threads_) tiles.push_back({mi, f, 0u, h.m.na});
+            for (index_t f = 0; f < h.m.ne; f += threads_) tiles.push_back({mi, f, 1u, h.m.ne});
+            for (index_t f = h.m.ne; f < h.m.nn; f += threads_) tiles.push_back({mi, f, 2u, h.m.nn});
         }
         if (tiles.size() > lay_.tiles_cap) throw Error(AS_ERR_NOMEM, "tile table capacity exceeded");
         n_tiles_ = static_cast<unsigned>(tiles.size());
@@ -1160,6 +1189,8 @@ private:
     BatchDev D_{};
     BlkState<R> S_{};
     std::int64_t* clock_ = nullptr;
+    IntegrateCtx<R>* ctx_eager_ = nullptr;
+    IntegrateCtx<R>* ctx_graph_ = nullptr;
     index_t *scratch0_ = nullptr, *scratch1_ = nullptr;
     void* scan_tmp_ = nullptr;
     std::int64_t step_ = 0;
