@@ -5,10 +5,14 @@
  * library on the GPU) each round their results a little differently, and in
  * a spiking network a one-ulp difference in one exp() eventually moves a
  * spike. This header therefore supplies the three transcendentals the
- * per-step arithmetic needs, written only with IEEE-754 +, -, *, / (each
+ * per-step arithmetic needs, written only with IEEE-754 +, -, *, /, fma (each
  * correctly rounded on both sides), exact bit manipulation and floor(), so
  * that any build with floating-point contraction disabled evaluates them to
  * identical bits on every target.
+ *
+ * Besides + - * /, the methods use fused multiply-add, which IEEE-754 also
+ * rounds exactly once on every conforming target (the device's v_fma_f64,
+ * the host's FMA instruction or C library fma), so results stay identical.
  *
  * Methods (textbook, from the mathematics):
  *   exp : Cody-Waite reduction x = k*ln2 + r, |r| <= ln2/2, with ln2 split
@@ -73,20 +77,21 @@ AS_DM_FN double as_exp(double x) {
     if (x < -745.2) return 0.0;
     const double kd = __builtin_floor(x * AS_DM_INV_LN2 + 0.5);
     const double r = (x - kd * AS_DM_LN2_HI) - kd * AS_DM_LN2_LO;
-    /* exp(r) = 1 + r + r^2 * q(r), q = sum_{n>=2} r^(n-2)/n! */
-    double q = 1.0 / 6227020800.0;           /* 1/13! */
-    q = 1.0 / 479001600.0 + r * q;            /* 1/12! */
-    q = 1.0 / 39916800.0 + r * q;             /* 1/11! */
-    q = 1.0 / 3628800.0 + r * q;              /* 1/10! */
-    q = 1.0 / 362880.0 + r * q;               /* 1/9!  */
-    q = 1.0 / 40320.0 + r * q;                /* 1/8!  */
-    q = 1.0 / 5040.0 + r * q;                 /* 1/7!  */
-    q = 1.0 / 720.0 + r * q;                  /* 1/6!  */
-    q = 1.0 / 120.0 + r * q;                  /* 1/5!  */
-    q = 1.0 / 24.0 + r * q;                   /* 1/4!  */
-    q = 1.0 / 6.0 + r * q;                    /* 1/3!  */
-    q = 0.5 + r * q;                          /* 1/2!  */
-    const double p = 1.0 + (r + (r * r) * q);
+    /* exp(r) = 1 + r + r^2 * q(r), q = sum_{n>=2} r^(n-2)/n!, Horner with
+     * fused multiply-adds (one rounding per step on every IEEE target) */
+    double q = 1.0 / 6227020800.0;                 /* 1/13! */
+    q = __builtin_fma(r, q, 1.0 / 479001600.0);     /* 1/12! */
+    q = __builtin_fma(r, q, 1.0 / 39916800.0);      /* 1/11! */
+    q = __builtin_fma(r, q, 1.0 / 3628800.0);       /* 1/10! */
+    q = __builtin_fma(r, q, 1.0 / 362880.0);        /* 1/9!  */
+    q = __builtin_fma(r, q, 1.0 / 40320.0);         /* 1/8!  */
+    q = __builtin_fma(r, q, 1.0 / 5040.0);          /* 1/7!  */
+    q = __builtin_fma(r, q, 1.0 / 720.0);           /* 1/6!  */
+    q = __builtin_fma(r, q, 1.0 / 120.0);           /* 1/5!  */
+    q = __builtin_fma(r, q, 1.0 / 24.0);            /* 1/4!  */
+    q = __builtin_fma(r, q, 1.0 / 6.0);             /* 1/3!  */
+    q = __builtin_fma(r, q, 0.5);                   /* 1/2!  */
+    const double p = 1.0 + __builtin_fma(r * r, q, r);
     const int k = (int)kd;
     const int k1 = k / 2;
     const int k2 = k - k1;
@@ -116,16 +121,16 @@ AS_DM_FN double as_log(double x) {
     const double z = s * s;
     /* R = sum_{n=1..11} 2 z^n / (2n+1) */
     double R = 2.0 / 23.0;
-    R = 2.0 / 21.0 + z * R;
-    R = 2.0 / 19.0 + z * R;
-    R = 2.0 / 17.0 + z * R;
-    R = 2.0 / 15.0 + z * R;
-    R = 2.0 / 13.0 + z * R;
-    R = 2.0 / 11.0 + z * R;
-    R = 2.0 / 9.0 + z * R;
-    R = 2.0 / 7.0 + z * R;
-    R = 2.0 / 5.0 + z * R;
-    R = 2.0 / 3.0 + z * R;
+    R = __builtin_fma(z, R, 2.0 / 21.0);
+    R = __builtin_fma(z, R, 2.0 / 19.0);
+    R = __builtin_fma(z, R, 2.0 / 17.0);
+    R = __builtin_fma(z, R, 2.0 / 15.0);
+    R = __builtin_fma(z, R, 2.0 / 13.0);
+    R = __builtin_fma(z, R, 2.0 / 11.0);
+    R = __builtin_fma(z, R, 2.0 / 9.0);
+    R = __builtin_fma(z, R, 2.0 / 7.0);
+    R = __builtin_fma(z, R, 2.0 / 5.0);
+    R = __builtin_fma(z, R, 2.0 / 3.0);
     R = z * R;
     const double hfsq = 0.5 * f * f;
     const double kd = (double)k;
@@ -135,13 +140,13 @@ AS_DM_FN double as_log(double x) {
 /* cos(r) on |r| <= pi/4 (+ a few ulps). */
 AS_DM_FN double as_dm_cos_kernel(double r) {
     const double z = r * r;
-    double c = 1.0 / 20922789888000.0;       /* +1/16! */
-    c = -1.0 / 87178291200.0 + z * c;         /* -1/14! */
-    c = 1.0 / 479001600.0 + z * c;            /* +1/12! */
-    c = -1.0 / 3628800.0 + z * c;             /* -1/10! */
-    c = 1.0 / 40320.0 + z * c;                /* +1/8!  */
-    c = -1.0 / 720.0 + z * c;                 /* -1/6!  */
-    c = 1.0 / 24.0 + z * c;                   /* +1/4!  */
+    double c = 1.0 / 20922789888000.0;           /* +1/16! */
+    c = __builtin_fma(z, c, -1.0 / 87178291200.0); /* -1/14! */
+    c = __builtin_fma(z, c, 1.0 / 479001600.0);    /* +1/12! */
+    c = __builtin_fma(z, c, -1.0 / 3628800.0);     /* -1/10! */
+    c = __builtin_fma(z, c, 1.0 / 40320.0);        /* +1/8!  */
+    c = __builtin_fma(z, c, -1.0 / 720.0);         /* -1/6!  */
+    c = __builtin_fma(z, c, 1.0 / 24.0);           /* +1/4!  */
     const double hz = 0.5 * z;
     const double w = 1.0 - hz;
     return w + (((1.0 - w) - hz) + z * (z * c));
@@ -150,15 +155,15 @@ AS_DM_FN double as_dm_cos_kernel(double r) {
 /* sin(r) on |r| <= pi/4 (+ a few ulps). */
 AS_DM_FN double as_dm_sin_kernel(double r) {
     const double z = r * r;
-    double s = 1.0 / 355687428096000.0;      /* +1/17! */
-    s = -1.0 / 1307674368000.0 + z * s;       /* -1/15! */
-    s = 1.0 / 6227020800.0 + z * s;           /* +1/13! */
-    s = -1.0 / 39916800.0 + z * s;            /* -1/11! */
-    s = 1.0 / 362880.0 + z * s;               /* +1/9!  */
-    s = -1.0 / 5040.0 + z * s;                /* -1/7!  */
-    s = 1.0 / 120.0 + z * s;                  /* +1/5!  */
-    s = -1.0 / 6.0 + z * s;                   /* -1/3!  */
-    return r + (r * z) * s;
+    double s = 1.0 / 355687428096000.0;          /* +1/17! */
+    s = __builtin_fma(z, s, -1.0 / 1307674368000.0); /* -1/15! */
+    s = __builtin_fma(z, s, 1.0 / 6227020800.0);   /* +1/13! */
+    s = __builtin_fma(z, s, -1.0 / 39916800.0);    /* -1/11! */
+    s = __builtin_fma(z, s, 1.0 / 362880.0);       /* +1/9!  */
+    s = __builtin_fma(z, s, -1.0 / 5040.0);        /* -1/7!  */
+    s = __builtin_fma(z, s, 1.0 / 120.0);          /* +1/5!  */
+    s = __builtin_fma(z, s, -1.0 / 6.0);           /* -1/3!  */
+    return __builtin_fma(r * z, s, r);
 }
 
 AS_DM_FN double as_cos(double x) {

@@ -94,7 +94,8 @@ inline Params32 make_params32(const RunParams& P) {
 
 __device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// hi+lo += x, error-free (TwoSum) then renormalised (Fast2Sum).
+// hi+lo += x, error-free (TwoSum) then renormalised (Fast2Sum). Every
+// operation here must round on its own: this function is never contracted.
 __device__ __forceinline__ void comp_add(float& hi, float& lo, float x) {
     const float s = hi + x;
     const float bp = s - hi;
@@ -105,10 +106,33 @@ __device__ __forceinline__ void comp_add(float& hi, float& lo, float x) {
     hi = h2;
 }
 
+// exp(x) in FP64 for the AdEx spike current near threshold, |x| < 3.1:
+// reduction by ln2 (|k| <= 5) and a degree-10 Taylor polynomial with fused
+// multiply-adds (relative error ~1e-13, far below the FP32 state's).
+__device__ __forceinline__ double exp_near_threshold(double x) {
+    const double kd = __builtin_rint(x * 0x1.71547652b82fep+0);
+    double r = __builtin_fma(-kd, 0x1.62e42fee00000p-1, x);
+    r = __builtin_fma(-kd, 0x1.a39ef35793c76p-33, r);
+    double p = 1.0 / 3628800.0;
+    p = __builtin_fma(r, p, 1.0 / 362880.0);
+    p = __builtin_fma(r, p, 1.0 / 40320.0);
+    p = __builtin_fma(r, p, 1.0 / 5040.0);
+    p = __builtin_fma(r, p, 1.0 / 720.0);
+    p = __builtin_fma(r, p, 1.0 / 120.0);
+    p = __builtin_fma(r, p, 1.0 / 24.0);
+    p = __builtin_fma(r, p, 1.0 / 6.0);
+    p = __builtin_fma(r, p, 0.5);
+    p = __builtin_fma(r, p, 1.0);
+    p = __builtin_fma(r, p, 1.0);
+    return p * as_dm_pow2(static_cast<int>(kd));
+}
+
 // Li-Rinzel right-hand side in FP32 (src/astrosim_fused.cu:236-253).
+// The FP32 right-hand sides let the compiler fuse multiply-adds.
 __device__ __forceinline__ void astro_rhs32(const AstroConst32& c, float Ca_tot, float IP3_0, float inv_tau,
                                             float Ca, float IP3, float h, float noise, float& dCa, float& dIP3,
                                             float& dh) {
+#pragma clang fp contract(fast)
     const float Ca_ER = (Ca_tot - Ca) * c.inv_ratio_ER_cyt;
     const float m_inf = IP3 * rcp32(IP3 + c.Kd_IP3_1);
     const float n_inf = Ca * rcp32(Ca + c.Kd_act);
@@ -132,6 +156,7 @@ __device__ __forceinline__ void astro_advance32(const AstroConst32& c, const Par
                                                 float IP3_0, float inv_tau, float delta_IP3, float ip3_input,
                                                 float noise, int substeps, float& Ca, float& Ca_lo, float& IP3,
                                                 float& IP3_lo, float& h, float& h_lo) {
+#pragma clang fp contract(fast)
     if (ip3_input != 0.0f) comp_add(IP3, IP3_lo, delta_IP3 * ip3_input);
     const float hs = q.h_step, hh = q.h_half;
     for (int s = 0; s < substeps; ++s) {
@@ -160,11 +185,13 @@ __device__ __forceinline__ float sic_factor32(const AstroConst32& c, float Ca, f
 // (src/astrosim_fused.cu:281-290).
 __device__ __forceinline__ void neuron_rhs32(const NeuronConst32& p, float V, float w, float g_ex, float g_in,
                                              float I_ext, float& dV, float& dw) {
+#pragma clang fp contract(fast)
     const float Vc = V < p.V_peak ? V : p.V_peak;
     const float x = Vc - p.V_th;
     float I_spike;
     if (x > -p.near_band && x < p.near_band)
-        I_spike = static_cast<float>(p.g_L_Delta_T_d * as_exp((static_cast<double>(Vc) - p.V_th_d) * p.inv_Delta_T_d));
+        I_spike = static_cast<float>(p.g_L_Delta_T_d *
+                                     exp_near_threshold((static_cast<double>(Vc) - p.V_th_d) * p.inv_Delta_T_d));
     else
         I_spike = p.g_L_Delta_T * __expf(x * p.inv_Delta_T);
     dV = (-p.g_L * (Vc - p.E_L) + I_spike - g_ex * (Vc - p.E_ex) - g_in * (Vc - p.E_in) - w + p.I_e + I_ext) *
@@ -182,6 +209,7 @@ struct NeuronRegs32 {
 __device__ __forceinline__ bool neuron_substeps32(const NeuronConst32& p, const Params32& q, NeuronRegs32& r,
                                                   float I_ext, float V_reset, float b, int substeps,
                                                   float ex_decay, float in_decay) {
+#pragma clang fp contract(fast)
     bool spiked = false;
     const float hs = q.h_step, hh = q.h_half;
     for (int sub = 0; sub < substeps; ++sub) {
