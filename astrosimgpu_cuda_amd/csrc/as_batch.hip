@@ -119,6 +119,18 @@ __device__ __forceinline__ void resolve_launch(BlkLaunch& B, const std::int64_t*
 
 __device__ __forceinline__ unsigned lane_id() { return threadIdx.x & 63u; }
 
+// Element at a 32-bit byte offset from a wave-uniform base pointer. One
+// offset register per cell then addresses every state array of its type
+// (scalar base + vector offset), instead of a 64-bit address per array.
+template <class T>
+__device__ __forceinline__ T& at(T* base, std::uint32_t byte_off) {
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off);
+}
+template <class T>
+__device__ __forceinline__ const T& at(const T* base, std::uint32_t byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
 // Appends `id` (when `want`) to a list with one atomic per wave.
 __device__ __forceinline__ void wave_append(bool want, std::uint32_t id, std::uint32_t* list,
                                             std::uint32_t* count) {
@@ -131,42 +143,131 @@ __device__ __forceinline__ void wave_append(bool want, std::uint32_t id, std::ui
     if (want) list[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = id;
 }
 
-// Delayed-input sum of one target for block step k: the reference's in-row
-// sum restricted to its nonzero terms, in ascending in-CSR slot order.
-template <class R, class WordF, class ValF>
-__device__ R input_sum(const uint2* ent, unsigned cnt, uint2 row, const index_t* src, unsigned k, WordF word,
-                       ValF val) {
-    R s = R(0);
-    if (cnt > kBucketCap) {  // overflow: walk the whole row
-        for (index_t e = row.x; e < row.y; ++e) {
-            const index_t q = src[e];
-            if ((word(q) >> k) & 1u) s += val(q);
-        }
-        return s;
-    }
-    unsigned last = 0;
-    for (unsigned r = 0; r < cnt; ++r) {
-        unsigned best = 0xFFFFFFFFu, bsrc = 0;
-        for (unsigned q = 0; q < cnt; ++q) {
-            const uint2 x = ent[q];
-            if ((r == 0 || x.x > last) && x.x < best) {
-                best = x.x;
-                bsrc = x.y;
-            }
-        }
-        last = best;
-        if ((word(bsrc) >> k) & 1u) s += val(bsrc);
-    }
-    return s;
+// ---------------------------------------------------------------------------
+// Delayed inputs of one target and connection type over a block.
+//
+// A target's bucket holds (in-CSR slot, source) for every source that emitted
+// in the block d steps earlier, in arrival order. The reference sums a
+// target's in-row in CSR order (src/astrosim_fused.cu:384-389, 450-463,
+// 482-490); summing the bucket in ascending slot order reproduces that sum
+// bit for bit, since the omitted terms are exact zeros. The bucket is sorted
+// once per block: up to NR entries go to registers with their sources'
+// emission words, and the values of step k+1 are loaded during step k, so no
+// step waits on memory. A larger bucket (rare) is sorted in place and walked
+// from memory each step; an overflowed bucket walks the whole in-row.
+// ---------------------------------------------------------------------------
+
+// Compare-exchange on (slot, source) pairs, ascending by slot.
+__device__ __forceinline__ void cas(uint2& a, uint2& b) {
+    const bool swap = b.x < a.x;
+    const uint2 lo = swap ? b : a, hi = swap ? a : b;
+    a = lo;
+    b = hi;
 }
 
-// Union of the emission bits of a target's inputs (all ones on overflow).
-template <class WordF>
-__device__ std::uint32_t input_mask(const uint2* ent, unsigned cnt, WordF word) {
-    if (cnt > kBucketCap) return 0xFFFFFFFFu;
-    std::uint32_t m = 0;
-    for (unsigned q = 0; q < cnt; ++q) m |= word(ent[q].y);
-    return m;
+// Sorting networks (Batcher) for 2, 4 and 8 entries.
+template <int N>
+__device__ __forceinline__ void sort_entries(uint2 (&e)[N]) {
+    static_assert(N == 2 || N == 4 || N == 8, "network size");
+    if constexpr (N == 2) {
+        cas(e[0], e[1]);
+    } else if constexpr (N == 4) {
+        cas(e[0], e[1]); cas(e[2], e[3]);
+        cas(e[0], e[2]); cas(e[1], e[3]);
+        cas(e[1], e[2]);
+    } else {
+        cas(e[0], e[1]); cas(e[2], e[3]); cas(e[4], e[5]); cas(e[6], e[7]);
+        cas(e[0], e[2]); cas(e[1], e[3]); cas(e[4], e[6]); cas(e[5], e[7]);
+        cas(e[1], e[2]); cas(e[5], e[6]);
+        cas(e[0], e[4]); cas(e[1], e[5]); cas(e[2], e[6]); cas(e[3], e[7]);
+        cas(e[2], e[4]); cas(e[3], e[5]);
+        cas(e[1], e[2]); cas(e[3], e[4]); cas(e[5], e[6]);
+    }
+}
+
+// Empty network slots sort last: no in-CSR slot reaches 2^32 - 1.
+constexpr std::uint32_t kNoSlot = 0xFFFFFFFFu;
+
+template <class R, int NR>
+struct InputStream {
+    std::uint32_t off[NR];   // source id * sizeof(R): the source's value column
+    std::uint32_t word[NR];  // the source's emission word (bit k: emitted at block step k)
+    R v[NR];                 // prefetched values of one step (zero where not emitted)
+    unsigned cnt;            // bucket entries; more than NR: the in-memory path
+};
+
+// words: emission words of the source population for block b - d/L.
+template <class R, int NR>
+__device__ __forceinline__ void input_init(InputStream<R, NR>& s, uint2* ent, unsigned cnt,
+                                           const std::uint32_t* words) {
+    s.cnt = cnt;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        s.off[r] = 0u;
+        s.word[r] = 0u;
+        s.v[r] = R(0);
+    }
+    if (cnt == 0u) return;
+    if (cnt <= unsigned(NR)) {
+        uint2 e[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) e[q] = unsigned(q) < cnt ? ent[q] : make_uint2(kNoSlot, 0u);
+        sort_entries<NR>(e);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (e[r].x != kNoSlot) {
+                s.off[r] = e[r].y * std::uint32_t(sizeof(R));
+                s.word[r] = words[e[r].y];
+            }
+        }
+    } else if (cnt <= kBucketCap) {
+        uint2 e[kBucketCap];
+#pragma unroll
+        for (int q = 0; q < int(kBucketCap); ++q) e[q] = unsigned(q) < cnt ? ent[q] : make_uint2(kNoSlot, 0u);
+        sort_entries<kBucketCap>(e);
+#pragma unroll
+        for (int q = 0; q < int(kBucketCap); ++q)
+            if (unsigned(q) < cnt) ent[q] = e[q];
+    }
+}
+
+// Loads the values of block step k from value-ring row `vals` (one row of
+// the source population's ring) for the register-held entries.
+template <class R, int NR>
+__device__ __forceinline__ void input_prefetch(InputStream<R, NR>& s, unsigned k, const R* vals) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        R x = R(0);
+        if ((s.word[r] >> k) & 1u) x = at(vals, s.off[r]);
+        s.v[r] = x;
+    }
+}
+
+// The input sum of block step k: the prefetched values in slot order, or the
+// in-memory path. words/vals as for input_init and input_prefetch; row()
+// and src give the target's in-row for the overflow walk.
+template <class R, int NR, class RowF>
+__device__ __forceinline__ R input_sum(const InputStream<R, NR>& s, unsigned k, const uint2* ent,
+                                       const std::uint32_t* words, const R* vals, RowF row,
+                                       const index_t* src) {
+    R sum = R(0);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) sum += s.v[r];
+    if (s.cnt > unsigned(NR)) {
+        if (s.cnt <= kBucketCap) {
+            for (unsigned r = 0; r < s.cnt; ++r) {
+                const index_t q = ent[r].y;
+                if ((words[q] >> k) & 1u) sum += vals[q];
+            }
+        } else {
+            const uint2 rw = row();
+            for (index_t e = rw.x; e < rw.y; ++e) {
+                const index_t q = src[e];
+                if ((words[q] >> k) & 1u) sum += vals[q];
+            }
+        }
+    }
+    return sum;
 }
 
 // ---------------------------------------------------------------------------
@@ -211,11 +312,15 @@ __global__ void tick_kernel(std::int64_t* clock) {
 // INTEGRATE: advance every cell over steps [t0, t0 + n) of one block.
 // ---------------------------------------------------------------------------
 
-// Value-ring row of the delayed step t - d (dereferenced only when a source's
-// bit is set, which implies t >= d).
-__device__ __forceinline__ std::size_t value_row(std::int64_t t, const BlkLaunch& B) {
-    const std::int64_t tv = t - B.d;
-    return tv >= 0 ? std::size_t(tv % B.Rv) : 0;
+// Row of step t in a ring of depth Rv (t may be negative: the value row of
+// t - d before step d is never dereferenced, but stays in range).
+__device__ __forceinline__ std::uint32_t ring_row(std::int64_t t, int Rv) {
+    const std::int64_t r = t % Rv;
+    return static_cast<std::uint32_t>(r < 0 ? r + Rv : r);
+}
+
+__device__ __forceinline__ std::uint32_t ring_next(std::uint32_t r, int Rv) {
+    return r + 1u == std::uint32_t(Rv) ? 0u : r + 1u;
 }
 
 // One astrocyte (global id ga, local id a) over global steps [lo, hi)
@@ -224,31 +329,31 @@ template <class R>
 __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
                            const Member& M, const BlkLaunch& B, index_t ga, index_t a, std::int64_t lo,
                            std::int64_t hi) {
-    const std::size_t rN = std::size_t(B.rslot) * D.ncap;
-    const unsigned cnt = D.bcnt[T_E2A][ga];
-    const uint2* ent = D.bent[T_E2A] + std::size_t(ga) * kBucketCap;
-    auto word = [&](index_t q) { return D.spk_w[rN + q]; };
-    const std::uint32_t any = cnt ? input_mask(ent, cnt, word) : 0u;
-    R ca = S.Ca[ga], ip3 = S.IP3[ga], hh = S.h[ga];
+    const std::uint32_t oa = ga * std::uint32_t(sizeof(R)), oa4 = ga * 4u;
+    const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
+    uint2* ent = D.bent[T_E2A] + std::size_t(ga) * kBucketCap;
+    InputStream<R, 4> in;
+    input_init(in, ent, at(D.bcnt[T_E2A], oa4), nwords);
+    std::uint32_t vr = ring_row(lo - B.d, B.Rv), pr = ring_row(lo, B.Rv);
+    input_prefetch(in, static_cast<unsigned>(lo - B.bstart), S.spk_v + std::size_t(vr) * D.ncap);
+    R ca = at(S.Ca, oa), ip3 = at(S.IP3, oa), hh = at(S.h, oa);
     R ca_lo = R(0), ip3_lo = R(0), hh_lo = R(0);
     if constexpr (sizeof(R) == 4) {
-        ca_lo = S.Ca_lo[ga];
-        ip3_lo = S.IP3_lo[ga];
-        hh_lo = S.h_lo[ga];
+        ca_lo = at(S.Ca_lo, oa);
+        ip3_lo = at(S.IP3_lo, oa);
+        hh_lo = at(S.h_lo, oa);
     }
-    const R Ca_tot = S.Ca_tot[ga], IP3_0 = S.IP3_0[ga], tau = S.tau_IP3[ga], dIP3 = S.delta_IP3[ga];
-    std::uint32_t wout = B.first ? 0u : D.sic_w[std::size_t(B.wslot) * D.acap + ga];
+    const R Ca_tot = at(S.Ca_tot, oa), IP3_0 = at(S.IP3_0, oa), tau = at(S.tau_IP3, oa), dIP3 = at(S.delta_IP3, oa);
+    std::uint32_t wout = B.first ? 0u : at(D.sic_w + std::size_t(B.wslot) * D.acap, oa4);
     bool ever = false;
+    // random streams of the member's step tl = t - step0 and cell a
+    std::uint64_t stream = std::uint64_t(lo - M.step0) * kStepStride + a;
     for (std::int64_t t = lo; t < hi; ++t) {
         const unsigned k = static_cast<unsigned>(t - B.bstart);
-        const std::int64_t tl = t - M.step0;  // the member's own step
-        R gathered = R(0);
-        if ((any >> k) & 1u) {
-            const std::size_t vrow = value_row(t, B) * D.ncap;
-            gathered = input_sum<R>(ent, cnt, D.in_rs[T_E2A][ga], D.in_src[T_E2A], k, word,
-                                    [&](index_t q) { return S.spk_v[vrow + q]; });
-        }
-        const std::uint64_t stream = std::uint64_t(tl) * kStepStride + a;
+        const R* vals = S.spk_v + std::size_t(vr) * D.ncap;
+        const R gathered = input_sum(in, k, ent, nwords, vals, [&] { return D.in_rs[T_E2A][ga]; }, D.in_src[T_E2A]);
+        const std::uint32_t vn = ring_next(vr, B.Rv);
+        if (t + 1 < hi) input_prefetch(in, k + 1u, S.spk_v + std::size_t(vn) * D.ncap);
         int ev = 0;
         if (P.poiss_astro) ev = rng_poisson(M.seed ^ kKeyAstroPoisson, stream, P.lam_astro, P.lim_astro);
         const double noise = P.noise_astro > 0.0 ? P.noise_astro * rng_normal(M.seed ^ kKeyAstroNoise, stream) : 0.0;
@@ -268,20 +373,23 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
         if (factor > R(0)) {
             ever = true;
             wout |= 1u << k;
-            S.sic_v[std::size_t(t % B.Rv) * D.acap + ga] = factor;
+            at(S.sic_v + std::size_t(pr) * D.acap, oa) = factor;
         }
+        vr = vn;
+        pr = ring_next(pr, B.Rv);
+        stream += kStepStride;
     }
-    S.Ca[ga] = ca;
-    S.IP3[ga] = ip3;
-    S.h[ga] = hh;
+    at(S.Ca, oa) = ca;
+    at(S.IP3, oa) = ip3;
+    at(S.h, oa) = hh;
     if constexpr (sizeof(R) == 4) {
-        S.Ca_lo[ga] = ca_lo;
-        S.IP3_lo[ga] = ip3_lo;
-        S.h_lo[ga] = hh_lo;
+        at(S.Ca_lo, oa) = ca_lo;
+        at(S.IP3_lo, oa) = ip3_lo;
+        at(S.h_lo, oa) = hh_lo;
     }
-    if (ever) D.ever[ga] = 1;
-    D.sic_w[std::size_t(B.wslot) * D.acap + ga] = wout;
-    if (B.last) D.bcnt[T_E2A][ga] = 0;
+    if (ever) at(D.ever, ga) = 1;
+    at(D.sic_w + std::size_t(B.wslot) * D.acap, oa4) = wout;
+    if (B.last) at(D.bcnt[T_E2A], oa4) = 0;
     return B.last && wout != 0u;
 }
 
@@ -291,55 +399,76 @@ template <class R>
 __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
                                 const Member& M, const BlkLaunch& B, index_t gj, index_t j, std::int64_t lo,
                                 std::int64_t hi, bool exc, bool* emit) {
-    const std::size_t rN = std::size_t(B.rslot) * D.ncap;
-    const std::size_t rA = std::size_t(B.rslot) * D.acap;
-    const unsigned ce = D.bcnt[T_EXC][gj], ci = D.bcnt[T_INH][gj], cs = D.bcnt[T_SIC][gj];
-    const uint2* ee = D.bent[T_EXC] + std::size_t(gj) * kBucketCap;
-    const uint2* ei = D.bent[T_INH] + std::size_t(gj) * kBucketCap;
-    const uint2* es = D.bent[T_SIC] + std::size_t(gj) * kBucketCap;
-    auto nword = [&](index_t q) { return D.spk_w[rN + q]; };
-    auto aword = [&](index_t q) { return D.sic_w[rA + q]; };
-    const std::uint32_t any_e = ce ? input_mask(ee, ce, nword) : 0u;
-    const std::uint32_t any_i = ci ? input_mask(ei, ci, nword) : 0u;
-    const std::uint32_t any_s = cs ? input_mask(es, cs, aword) : 0u;
-    R V = S.V[gj], w = S.w[gj], g_ex = S.g_ex[gj], dg_ex = S.dg_ex[gj], g_in = S.g_in[gj], dg_in = S.dg_in[gj];
-    int refr = D.refr[gj];
-    R I_sic = S.I_sic[gj];
-    const R V_reset = S.V_reset[gj], bb = S.b[gj];
+    const std::uint32_t on = gj * std::uint32_t(sizeof(R)), o4 = gj * 4u;
+    const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
+    const std::uint32_t* awords = D.sic_w + std::size_t(B.rslot) * D.acap;
+    uint2* ee = D.bent[T_EXC] + std::size_t(gj) * kBucketCap;
+    uint2* ei = D.bent[T_INH] + std::size_t(gj) * kBucketCap;
+    uint2* es = D.bent[T_SIC] + std::size_t(gj) * kBucketCap;
+    // inhibitory source ids are local indices used as neuron ids
+    // (src/astrosim_fused.cu:459-462); the scatter has already resolved them
+    InputStream<R, 4> ie;
+    InputStream<R, 2> ii, is;
+    input_init(ie, ee, at(D.bcnt[T_EXC], o4), nwords);
+    input_init(ii, ei, at(D.bcnt[T_INH], o4), nwords);
+    input_init(is, es, at(D.bcnt[T_SIC], o4), awords);
+    std::uint32_t vr = ring_row(lo - B.d, B.Rv), pr = ring_row(lo, B.Rv);
+    {
+        const unsigned k0 = static_cast<unsigned>(lo - B.bstart);
+        input_prefetch(ie, k0, S.spk_v + std::size_t(vr) * D.ncap);
+        input_prefetch(ii, k0, S.spk_v + std::size_t(vr) * D.ncap);
+        input_prefetch(is, k0, S.sic_v + std::size_t(vr) * D.acap);
+    }
+    R V = at(S.V, on), w = at(S.w, on), g_ex = at(S.g_ex, on), dg_ex = at(S.dg_ex, on), g_in = at(S.g_in, on),
+      dg_in = at(S.dg_in, on);
+    int refr = at(D.refr, o4);
+    R I_sic = at(S.I_sic, on);
+    const R V_reset = at(S.V_reset, on), bb = at(S.b, on);
     const double nstd = exc ? P.noise_exc : P.noise_inh;
-    std::uint64_t noise_at = ~std::uint64_t(0);
     R noise_val = R(0);
     unsigned nspikes = 0;
-    std::uint32_t wout = B.first ? 0u : D.spk_w[std::size_t(B.wslot) * D.ncap + gj];
+    std::uint32_t wout = B.first ? 0u : at(D.spk_w + std::size_t(B.wslot) * D.ncap, o4);
+    // Counters of the member's step tl = t - step0, advanced per step:
+    // the Poisson stream tl*stride + j, the SIC-update phase (an update at
+    // tl >= d_a2n with (tl - d_a2n) % sic_interval == 0), and the noise
+    // index tl / steps_per_noise with its remainder.
+    const std::int64_t tl0 = lo - M.step0;
+    std::uint64_t pstream = std::uint64_t(tl0) * kStepStride + j;
+    int sic_wait;
+    {
+        const std::int64_t u = tl0 - P.d_a2n;
+        sic_wait = u < 0 ? static_cast<int>(-u) : static_cast<int>((P.sic_interval - u % P.sic_interval) % P.sic_interval);
+    }
+    std::uint64_t nidx = static_cast<std::uint64_t>(tl0 / P.steps_per_noise);
+    int nrem = static_cast<int>(tl0 % P.steps_per_noise);
     for (std::int64_t t = lo; t < hi; ++t) {
         const unsigned k = static_cast<unsigned>(t - B.bstart);
         const std::int64_t tl = t - M.step0;  // the member's own step
-        const std::size_t vrow = value_row(t, B);
-        auto nval = [&](index_t q) { return S.spk_v[vrow * D.ncap + q]; };
-        R exc_sum = R(0), inh_sum = R(0);
-        if ((any_e >> k) & 1u) exc_sum = input_sum<R>(ee, ce, D.in_rs[T_EXC][gj], D.in_src[T_EXC], k, nword, nval);
-        // inhibitory source ids are local indices used as neuron ids (src/astrosim_fused.cu:459-462)
-        if ((any_i >> k) & 1u) inh_sum = input_sum<R>(ei, ci, D.in_rs[T_INH][gj], D.in_src[T_INH], k, nword, nval);
-        int ev = 0;
-        if (exc ? P.poiss_exc : P.poiss_inh)
-            ev = rng_poisson(M.seed ^ kKeyNeuronPoisson, std::uint64_t(tl) * kStepStride + j,
-                             exc ? P.lam_exc : P.lam_inh, exc ? P.lim_exc : P.lim_inh);
-        if ((tl - P.d_a2n >= 0) && ((tl - P.d_a2n) % P.sic_interval == 0)) {
-            R sic_sum = R(0);
-            if ((any_s >> k) & 1u)
-                sic_sum = input_sum<R>(es, cs, D.in_rs[T_SIC][gj], D.in_src[T_SIC], k, aword,
-                                       [&](index_t q) { return S.sic_v[vrow * D.acap + q]; });
+        const R* nvals = S.spk_v + std::size_t(vr) * D.ncap;
+        const R exc_sum =
+            input_sum(ie, k, ee, nwords, nvals, [&] { return D.in_rs[T_EXC][gj]; }, D.in_src[T_EXC]);
+        const R inh_sum =
+            input_sum(ii, k, ei, nwords, nvals, [&] { return D.in_rs[T_INH][gj]; }, D.in_src[T_INH]);
+        const bool sic_step = sic_wait == 0;
+        sic_wait = sic_step ? P.sic_interval - 1 : sic_wait - 1;
+        if (sic_step) {
+            const R sic_sum = input_sum(is, k, es, awords, S.sic_v + std::size_t(vr) * D.acap,
+                                        [&] { return D.in_rs[T_SIC][gj]; }, D.in_src[T_SIC]);
             if constexpr (sizeof(R) == 8) I_sic = P.w_a2n * sic_sum;
             else I_sic = Q.w_a2n * sic_sum;
         }
-        if (nstd > 0.0) {
-            // the neuron noise stream changes every steps_per_noise steps
-            const std::uint64_t nidx = static_cast<std::uint64_t>(tl / P.steps_per_noise);
-            if (nidx != noise_at) {
-                noise_at = nidx;
-                noise_val = static_cast<R>(nstd * rng_normal(M.seed ^ kKeyNeuronNoise, nidx * kNoiseStride + std::uint64_t(j)));
-            }
+        const std::uint32_t vn = ring_next(vr, B.Rv);
+        if (t + 1 < hi) {
+            input_prefetch(ie, k + 1u, S.spk_v + std::size_t(vn) * D.ncap);
+            input_prefetch(ii, k + 1u, S.spk_v + std::size_t(vn) * D.ncap);
+            input_prefetch(is, k + 1u, S.sic_v + std::size_t(vn) * D.acap);
         }
+        int ev = 0;
+        if (exc ? P.poiss_exc : P.poiss_inh)
+            ev = rng_poisson(M.seed ^ kKeyNeuronPoisson, pstream, exc ? P.lam_exc : P.lam_inh,
+                             exc ? P.lim_exc : P.lim_inh);
+        if (nstd > 0.0 && (t == lo || nrem == 0))  // the noise stream changes every steps_per_noise steps
+            noise_val = static_cast<R>(nstd * rng_normal(M.seed ^ kKeyNeuronNoise, nidx * kNoiseStride + std::uint64_t(j)));
         bool spiked;
         if constexpr (sizeof(R) == 8) {
             const NeuronConst& p = exc ? P.ne : P.ni;
@@ -366,7 +495,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         if (spiked) {
             R out = R(1);
             if (P.stp_enabled) {
-                R x = S.stp_x[gj], u = S.stp_u[gj], tlast = S.stp_t[gj];
+                R x = at(S.stp_x, on), u = at(S.stp_u, on), tlast = at(S.stp_t, on);
                 if constexpr (sizeof(R) == 8) {
                     out = stp_on_spike(x, u, tlast, static_cast<double>(tl) * P.dt, P.stp_U, P.stp_tau_rec,
                                        P.stp_tau_fac);
@@ -382,28 +511,35 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
                     tlast = t_now;
                     out = x * u;
                 }
-                S.stp_x[gj] = x;
-                S.stp_u[gj] = u;
-                S.stp_t[gj] = tlast;
+                at(S.stp_x, on) = x;
+                at(S.stp_u, on) = u;
+                at(S.stp_t, on) = tlast;
             }
-            S.spk_v[std::size_t(t % B.Rv) * D.ncap + gj] = out;
+            at(S.spk_v + std::size_t(pr) * D.ncap, on) = out;
             wout |= 1u << k;
             ++nspikes;
         }
+        vr = vn;
+        pr = ring_next(pr, B.Rv);
+        pstream += kStepStride;
+        if (++nrem == P.steps_per_noise) {
+            nrem = 0;
+            ++nidx;
+        }
     }
-    S.V[gj] = V;
-    S.w[gj] = w;
-    S.g_ex[gj] = g_ex;
-    S.dg_ex[gj] = dg_ex;
-    S.g_in[gj] = g_in;
-    S.dg_in[gj] = dg_in;
-    D.refr[gj] = refr;
-    S.I_sic[gj] = I_sic;
-    D.spk_w[std::size_t(B.wslot) * D.ncap + gj] = wout;
+    at(S.V, on) = V;
+    at(S.w, on) = w;
+    at(S.g_ex, on) = g_ex;
+    at(S.dg_ex, on) = dg_ex;
+    at(S.g_in, on) = g_in;
+    at(S.dg_in, on) = dg_in;
+    at(D.refr, o4) = refr;
+    at(S.I_sic, on) = I_sic;
+    at(D.spk_w + std::size_t(B.wslot) * D.ncap, o4) = wout;
     if (B.last) {
-        D.bcnt[T_EXC][gj] = 0;
-        D.bcnt[T_INH][gj] = 0;
-        D.bcnt[T_SIC][gj] = 0;
+        at(D.bcnt[T_EXC], o4) = 0;
+        at(D.bcnt[T_INH], o4) = 0;
+        at(D.bcnt[T_SIC], o4) = 0;
     }
     *emit = B.last && wout != 0u;
     return nspikes;
@@ -420,8 +556,13 @@ struct IntegrateCtx {
     Params32 Q;
 };
 
+// Minimum waves per SIMD the integrate kernel's register budget targets.
+#ifndef AS_INTEGRATE_MIN_WAVES
+#define AS_INTEGRATE_MIN_WAVES 2
+#endif
+
 template <int MAXT, class R>
-__global__ __launch_bounds__(MAXT) void integrate_kernel(const IntegrateCtx<R>* __restrict__ ctx, BlkLaunch B) {
+__global__ __launch_bounds__(MAXT, AS_INTEGRATE_MIN_WAVES) void integrate_kernel(const IntegrateCtx<R>* __restrict__ ctx, BlkLaunch B) {
     const BatchDev& D = ctx->D;
     const BlkState<R>& S = ctx->S;
     const RunParams& P = ctx->P;
