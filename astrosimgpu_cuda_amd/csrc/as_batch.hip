@@ -65,7 +65,7 @@ struct BatchDev {
     const uint2* out_rs[4];  // (begin, end) per source: neurons (exc, inh, e2a), astrocytes (sic)
     const uint2* out_ent[4];  // (global target, global in-CSR slot)
     unsigned* bcnt[4];
-    uint2* bent[4];
+    uint4* bent[4];  // [target][kBucketCap] bucket entries (see entry layout below)
     std::uint8_t* ever;
     std::int32_t* refr;
     std::uint32_t* spk_w;  // [H][ncap]
@@ -131,6 +131,17 @@ __device__ __forceinline__ const T& at(const T* base, std::uint32_t byte_off) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
 }
 
+// Row of step t in a ring of depth Rv (t may be negative: the value row of
+// t - d before step d is never dereferenced, but stays in range).
+__device__ __forceinline__ std::uint32_t ring_row(std::int64_t t, int Rv) {
+    const std::int64_t r = t % Rv;
+    return static_cast<std::uint32_t>(r < 0 ? r + Rv : r);
+}
+
+__device__ __forceinline__ std::uint32_t ring_next(std::uint32_t r, int Rv) {
+    return r + 1u == std::uint32_t(Rv) ? 0u : r + 1u;
+}
+
 // Appends `id` (when `want`) to a list with one atomic per wave.
 __device__ __forceinline__ void wave_append(bool want, std::uint32_t id, std::uint32_t* list,
                                             std::uint32_t* count) {
@@ -146,28 +157,42 @@ __device__ __forceinline__ void wave_append(bool want, std::uint32_t id, std::ui
 // ---------------------------------------------------------------------------
 // Delayed inputs of one target and connection type over a block.
 //
-// A target's bucket holds (in-CSR slot, source) for every source that emitted
-// in the block d steps earlier, in arrival order. The reference sums a
-// target's in-row in CSR order (src/astrosim_fused.cu:384-389, 450-463,
-// 482-490); summing the bucket in ascending slot order reproduces that sum
-// bit for bit, since the omitted terms are exact zeros. The bucket is sorted
-// once per block: up to NR entries go to registers with their sources'
-// emission words, and the values of step k+1 are loaded during step k, so no
-// step waits on memory. A larger bucket (rare) is sorted in place and walked
-// from memory each step; an overflowed bucket walks the whole in-row.
+// A target's bucket holds one entry per source that emitted in the block d
+// steps earlier, in arrival order:
+//   x  the edge's in-CSR slot;
+//   y  the source's global id (bits 0-25), and, when the source emitted at a
+//      single step k of its block (a neuron almost always does), k in bits
+//      26-30 and kAttached;
+//   zw with kAttached, the emitted value's bits (FP32: z only).
+// The reference sums a target's in-row in CSR order (src/astrosim_fused.cu:
+// 384-389, 450-463, 482-490); summing the bucket in ascending slot order
+// reproduces that sum bit for bit, since the omitted terms are exact zeros.
+// Each bucket is sorted once per block. Neuron-sourced inputs keep up to NR
+// attached (step, value) pairs in registers, so their step loop touches no
+// memory; SIC inputs (astrocytes emit at every step while active) keep
+// (source, word) pairs and load the values of step k+1 during step k. A
+// target with more entries, or an entry without an attached value (rare),
+// takes the in-memory path: entries sorted in place and walked each step,
+// or the whole in-row when the bucket overflowed.
 // ---------------------------------------------------------------------------
 
-// Compare-exchange on (slot, source) pairs, ascending by slot.
-__device__ __forceinline__ void cas(uint2& a, uint2& b) {
+constexpr std::uint32_t kSrcMask = (1u << 26) - 1u;
+constexpr std::uint32_t kAttached = 1u << 31;
+constexpr int kStepShift = 26;
+// Empty network slots sort last: no in-CSR slot reaches 2^32 - 1.
+constexpr std::uint32_t kNoSlot = 0xFFFFFFFFu;
+
+// Compare-exchange on bucket entries, ascending by slot.
+__device__ __forceinline__ void cas(uint4& a, uint4& b) {
     const bool swap = b.x < a.x;
-    const uint2 lo = swap ? b : a, hi = swap ? a : b;
+    const uint4 lo = swap ? b : a, hi = swap ? a : b;
     a = lo;
     b = hi;
 }
 
 // Sorting networks (Batcher) for 2, 4 and 8 entries.
 template <int N>
-__device__ __forceinline__ void sort_entries(uint2 (&e)[N]) {
+__device__ __forceinline__ void sort_entries(uint4 (&e)[N]) {
     static_assert(N == 2 || N == 4 || N == 8, "network size");
     if constexpr (N == 2) {
         cas(e[0], e[1]);
@@ -185,9 +210,117 @@ __device__ __forceinline__ void sort_entries(uint2 (&e)[N]) {
     }
 }
 
-// Empty network slots sort last: no in-CSR slot reaches 2^32 - 1.
-constexpr std::uint32_t kNoSlot = 0xFFFFFFFFu;
+// Loads the first N entries of a bucket (empty slots: kNoSlot) and sorts them.
+template <int N>
+__device__ __forceinline__ void load_sorted(uint4 (&e)[N], const uint4* ent, unsigned cnt) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) e[q] = unsigned(q) < cnt ? ent[q] : make_uint4(kNoSlot, 0u, 0u, 0u);
+    sort_entries<N>(e);
+}
 
+// Sorts a bucket of at most kBucketCap entries in place.
+__device__ __forceinline__ void sort_in_place(uint4* ent, unsigned cnt) {
+    uint4 e[kBucketCap];
+    load_sorted<kBucketCap>(e, ent, cnt);
+#pragma unroll
+    for (int q = 0; q < int(kBucketCap); ++q)
+        if (unsigned(q) < cnt) ent[q] = e[q];
+}
+
+template <class R>
+__device__ __forceinline__ R attached_value(const uint4& e) {
+    if constexpr (sizeof(R) == 8) {
+        return __hiloint2double(static_cast<int>(e.w), static_cast<int>(e.z));
+    } else {
+        return __uint_as_float(e.z);
+    }
+}
+
+__device__ __forceinline__ unsigned attached_step(const uint4& e) { return (e.y >> kStepShift) & 31u; }
+
+// The in-memory path: the sum over a sorted bucket of at most kBucketCap
+// entries, or over the whole in-row after an overflow. words/vals: emission
+// words and block-step-k values of the source population.
+template <class R, class RowF>
+__device__ R memory_sum(const uint4* ent, unsigned cnt, unsigned k, const std::uint32_t* words, const R* vals,
+                        RowF row, const index_t* src) {
+    R sum = R(0);
+    if (cnt <= kBucketCap) {
+        for (unsigned r = 0; r < cnt; ++r) {
+            const uint4 e = ent[r];
+            if (e.y & kAttached) {
+                if (attached_step(e) == k) sum += attached_value<R>(e);
+            } else {
+                const index_t q = e.y & kSrcMask;
+                if ((words[q] >> k) & 1u) sum += vals[q];
+            }
+        }
+    } else {
+        const uint2 rw = row();
+        for (index_t e = rw.x; e < rw.y; ++e) {
+            const index_t q = src[e];
+            if ((words[q] >> k) & 1u) sum += vals[q];
+        }
+    }
+    return sum;
+}
+
+// Neuron-sourced inputs: attached (step, value) pairs in slot order.
+template <class R, int NR>
+struct DirectInputs {
+    R v[NR];                 // values (zero where empty)
+    std::uint32_t k[NR];     // their block steps (32: empty)
+    unsigned cnt;            // bucket entries
+    bool mem;                // the in-memory path
+};
+
+template <class R, int NR>
+__device__ __forceinline__ void direct_init(DirectInputs<R, NR>& s, uint4* ent, unsigned cnt) {
+    s.cnt = cnt;
+    s.mem = cnt > unsigned(NR);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        s.v[r] = R(0);
+        s.k[r] = 32u;
+    }
+    if (cnt == 0u) return;
+    if (cnt <= unsigned(NR)) {
+        uint4 e[NR];
+        load_sorted<NR>(e, ent, cnt);
+        bool all = true;
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (e[r].x != kNoSlot && !(e[r].y & kAttached)) all = false;
+        if (all) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                if (e[r].x != kNoSlot) {
+                    s.k[r] = attached_step(e[r]);
+                    s.v[r] = attached_value<R>(e[r]);
+                }
+            }
+        } else {
+            s.mem = true;
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (unsigned(r) < cnt) ent[r] = e[r];
+        }
+    } else if (cnt <= kBucketCap) {
+        sort_in_place(ent, cnt);
+    }
+}
+
+template <class R, int NR, class RowF>
+__device__ __forceinline__ R direct_sum(const DirectInputs<R, NR>& s, unsigned k, const uint4* ent,
+                                        const std::uint32_t* words, const R* vals, RowF row, const index_t* src) {
+    R sum = R(0);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) sum += s.k[r] == k ? s.v[r] : R(0);
+    if (s.mem) sum = memory_sum<R>(ent, s.cnt, k, words, vals, row, src);
+    return sum;
+}
+
+// SIC inputs: (source, word) pairs in slot order, values prefetched a step ahead.
 template <class R, int NR>
 struct InputStream {
     std::uint32_t off[NR];   // source id * sizeof(R): the source's value column
@@ -196,9 +329,8 @@ struct InputStream {
     unsigned cnt;            // bucket entries; more than NR: the in-memory path
 };
 
-// words: emission words of the source population for block b - d/L.
 template <class R, int NR>
-__device__ __forceinline__ void input_init(InputStream<R, NR>& s, uint2* ent, unsigned cnt,
+__device__ __forceinline__ void input_init(InputStream<R, NR>& s, uint4* ent, unsigned cnt,
                                            const std::uint32_t* words) {
     s.cnt = cnt;
 #pragma unroll
@@ -209,30 +341,22 @@ __device__ __forceinline__ void input_init(InputStream<R, NR>& s, uint2* ent, un
     }
     if (cnt == 0u) return;
     if (cnt <= unsigned(NR)) {
-        uint2 e[NR];
-#pragma unroll
-        for (int q = 0; q < NR; ++q) e[q] = unsigned(q) < cnt ? ent[q] : make_uint2(kNoSlot, 0u);
-        sort_entries<NR>(e);
+        uint4 e[NR];
+        load_sorted<NR>(e, ent, cnt);
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             if (e[r].x != kNoSlot) {
-                s.off[r] = e[r].y * std::uint32_t(sizeof(R));
-                s.word[r] = words[e[r].y];
+                const std::uint32_t q = e[r].y & kSrcMask;
+                s.off[r] = q * std::uint32_t(sizeof(R));
+                s.word[r] = words[q];
             }
         }
     } else if (cnt <= kBucketCap) {
-        uint2 e[kBucketCap];
-#pragma unroll
-        for (int q = 0; q < int(kBucketCap); ++q) e[q] = unsigned(q) < cnt ? ent[q] : make_uint2(kNoSlot, 0u);
-        sort_entries<kBucketCap>(e);
-#pragma unroll
-        for (int q = 0; q < int(kBucketCap); ++q)
-            if (unsigned(q) < cnt) ent[q] = e[q];
+        sort_in_place(ent, cnt);
     }
 }
 
-// Loads the values of block step k from value-ring row `vals` (one row of
-// the source population's ring) for the register-held entries.
+// Loads the values of block step k from value-ring row `vals`.
 template <class R, int NR>
 __device__ __forceinline__ void input_prefetch(InputStream<R, NR>& s, unsigned k, const R* vals) {
 #pragma unroll
@@ -244,29 +368,15 @@ __device__ __forceinline__ void input_prefetch(InputStream<R, NR>& s, unsigned k
 }
 
 // The input sum of block step k: the prefetched values in slot order, or the
-// in-memory path. words/vals as for input_init and input_prefetch; row()
-// and src give the target's in-row for the overflow walk.
+// in-memory path.
 template <class R, int NR, class RowF>
-__device__ __forceinline__ R input_sum(const InputStream<R, NR>& s, unsigned k, const uint2* ent,
+__device__ __forceinline__ R input_sum(const InputStream<R, NR>& s, unsigned k, const uint4* ent,
                                        const std::uint32_t* words, const R* vals, RowF row,
                                        const index_t* src) {
     R sum = R(0);
 #pragma unroll
     for (int r = 0; r < NR; ++r) sum += s.v[r];
-    if (s.cnt > unsigned(NR)) {
-        if (s.cnt <= kBucketCap) {
-            for (unsigned r = 0; r < s.cnt; ++r) {
-                const index_t q = ent[r].y;
-                if ((words[q] >> k) & 1u) sum += vals[q];
-            }
-        } else {
-            const uint2 rw = row();
-            for (index_t e = rw.x; e < rw.y; ++e) {
-                const index_t q = src[e];
-                if ((words[q] >> k) & 1u) sum += vals[q];
-            }
-        }
-    }
+    if (s.cnt > unsigned(NR)) sum = memory_sum<R>(ent, s.cnt, k, words, vals, row, src);
     return sum;
 }
 
@@ -274,7 +384,8 @@ __device__ __forceinline__ R input_sum(const InputStream<R, NR>& s, unsigned k, 
 // SCATTER: emitters of block b - d/L append to their targets' buckets.
 // One wave per emitter, lanes over its out-edges.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void scatter_kernel(BatchDev D, BlkLaunch B) {
+template <class R>
+__global__ __launch_bounds__(256) void scatter_kernel(BatchDev D, BlkLaunch B, const R* __restrict__ spk_v) {
     resolve_launch(B, D.clock);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         D.lcount[B.wslot * 2 + 0] = 0;
@@ -288,6 +399,23 @@ __global__ __launch_bounds__(256) void scatter_kernel(BatchDev D, BlkLaunch B) {
         const bool is_neuron = i < nN;
         const std::uint32_t src = is_neuron ? D.nlist[std::size_t(B.rslot) * D.ncap + i]
                                             : D.alist[std::size_t(B.rslot) * D.acap + (i - nN)];
+        // A neuron that spiked at one step k of its block hands (k, value)
+        // to its targets with the entry.
+        std::uint32_t tag = src, vlo = 0u, vhi = 0u;
+        if (is_neuron) {
+            const std::uint32_t w = D.spk_w[std::size_t(B.rslot) * D.ncap + src];
+            if (__popc(w) == 1) {
+                const unsigned k = __ffs(static_cast<int>(w)) - 1;
+                const R v = spk_v[std::size_t(ring_row(B.bstart - B.d + k, B.Rv)) * D.ncap + src];
+                tag |= kAttached | (k << kStepShift);
+                if constexpr (sizeof(R) == 8) {
+                    vlo = static_cast<std::uint32_t>(__double2loint(v));
+                    vhi = static_cast<std::uint32_t>(__double2hiint(v));
+                } else {
+                    vlo = __float_as_uint(v);
+                }
+            }
+        }
         // A neuron walks its exc and n2a out-edges and the inhibitory rows
         // whose stored LOCAL id equals its own local id (src/astrosim_fused.cu:
         // 459-462); an astrocyte walks its a2n out-edges.
@@ -297,8 +425,10 @@ __global__ __launch_bounds__(256) void scatter_kernel(BatchDev D, BlkLaunch B) {
             for (index_t e = r.x + lane_id(); e < r.y; e += 64) {
                 const uint2 te = D.out_ent[T][e];  // (target, in-CSR slot)
                 const unsigned slot = atomicAdd(&D.bcnt[T][te.x], 1u);
-                if (slot < kBucketCap) D.bent[T][std::size_t(te.x) * kBucketCap + slot] = make_uint2(te.y, src);
-                else if (slot == kBucketCap) atomicAdd(D.overflow, 1u);
+                if (slot < kBucketCap)
+                    D.bent[T][std::size_t(te.x) * kBucketCap + slot] = make_uint4(te.y, tag, vlo, vhi);
+                else if (slot == kBucketCap)
+                    atomicAdd(D.overflow, 1u);
             }
         }
     }
@@ -312,17 +442,6 @@ __global__ void tick_kernel(std::int64_t* clock) {
 // INTEGRATE: advance every cell over steps [t0, t0 + n) of one block.
 // ---------------------------------------------------------------------------
 
-// Row of step t in a ring of depth Rv (t may be negative: the value row of
-// t - d before step d is never dereferenced, but stays in range).
-__device__ __forceinline__ std::uint32_t ring_row(std::int64_t t, int Rv) {
-    const std::int64_t r = t % Rv;
-    return static_cast<std::uint32_t>(r < 0 ? r + Rv : r);
-}
-
-__device__ __forceinline__ std::uint32_t ring_next(std::uint32_t r, int Rv) {
-    return r + 1u == std::uint32_t(Rv) ? 0u : r + 1u;
-}
-
 // One astrocyte (global id ga, local id a) over global steps [lo, hi)
 // (src/astrosim_fused.cu:374-427). Returns whether it joins the emitter list.
 template <class R>
@@ -331,11 +450,10 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
                            std::int64_t hi) {
     const std::uint32_t oa = ga * std::uint32_t(sizeof(R)), oa4 = ga * 4u;
     const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
-    uint2* ent = D.bent[T_E2A] + std::size_t(ga) * kBucketCap;
-    InputStream<R, 4> in;
-    input_init(in, ent, at(D.bcnt[T_E2A], oa4), nwords);
+    uint4* ent = D.bent[T_E2A] + std::size_t(ga) * kBucketCap;
+    DirectInputs<R, 4> in;
+    direct_init(in, ent, at(D.bcnt[T_E2A], oa4));
     std::uint32_t vr = ring_row(lo - B.d, B.Rv), pr = ring_row(lo, B.Rv);
-    input_prefetch(in, static_cast<unsigned>(lo - B.bstart), S.spk_v + std::size_t(vr) * D.ncap);
     R ca = at(S.Ca, oa), ip3 = at(S.IP3, oa), hh = at(S.h, oa);
     R ca_lo = R(0), ip3_lo = R(0), hh_lo = R(0);
     if constexpr (sizeof(R) == 4) {
@@ -351,9 +469,7 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
     for (std::int64_t t = lo; t < hi; ++t) {
         const unsigned k = static_cast<unsigned>(t - B.bstart);
         const R* vals = S.spk_v + std::size_t(vr) * D.ncap;
-        const R gathered = input_sum(in, k, ent, nwords, vals, [&] { return D.in_rs[T_E2A][ga]; }, D.in_src[T_E2A]);
-        const std::uint32_t vn = ring_next(vr, B.Rv);
-        if (t + 1 < hi) input_prefetch(in, k + 1u, S.spk_v + std::size_t(vn) * D.ncap);
+        const R gathered = direct_sum(in, k, ent, nwords, vals, [&] { return D.in_rs[T_E2A][ga]; }, D.in_src[T_E2A]);
         int ev = 0;
         if (P.poiss_astro) ev = rng_poisson(M.seed ^ kKeyAstroPoisson, stream, P.lam_astro, P.lim_astro);
         const double noise = P.noise_astro > 0.0 ? P.noise_astro * rng_normal(M.seed ^ kKeyAstroNoise, stream) : 0.0;
@@ -375,7 +491,7 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
             wout |= 1u << k;
             at(S.sic_v + std::size_t(pr) * D.acap, oa) = factor;
         }
-        vr = vn;
+        vr = ring_next(vr, B.Rv);
         pr = ring_next(pr, B.Rv);
         stream += kStepStride;
     }
@@ -402,23 +518,19 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     const std::uint32_t on = gj * std::uint32_t(sizeof(R)), o4 = gj * 4u;
     const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
     const std::uint32_t* awords = D.sic_w + std::size_t(B.rslot) * D.acap;
-    uint2* ee = D.bent[T_EXC] + std::size_t(gj) * kBucketCap;
-    uint2* ei = D.bent[T_INH] + std::size_t(gj) * kBucketCap;
-    uint2* es = D.bent[T_SIC] + std::size_t(gj) * kBucketCap;
+    uint4* ee = D.bent[T_EXC] + std::size_t(gj) * kBucketCap;
+    uint4* ei = D.bent[T_INH] + std::size_t(gj) * kBucketCap;
+    uint4* es = D.bent[T_SIC] + std::size_t(gj) * kBucketCap;
     // inhibitory source ids are local indices used as neuron ids
     // (src/astrosim_fused.cu:459-462); the scatter has already resolved them
-    InputStream<R, 4> ie;
-    InputStream<R, 2> ii, is;
-    input_init(ie, ee, at(D.bcnt[T_EXC], o4), nwords);
-    input_init(ii, ei, at(D.bcnt[T_INH], o4), nwords);
+    DirectInputs<R, 4> ie;
+    DirectInputs<R, 2> ii;
+    InputStream<R, 2> is;
+    direct_init(ie, ee, at(D.bcnt[T_EXC], o4));
+    direct_init(ii, ei, at(D.bcnt[T_INH], o4));
     input_init(is, es, at(D.bcnt[T_SIC], o4), awords);
     std::uint32_t vr = ring_row(lo - B.d, B.Rv), pr = ring_row(lo, B.Rv);
-    {
-        const unsigned k0 = static_cast<unsigned>(lo - B.bstart);
-        input_prefetch(ie, k0, S.spk_v + std::size_t(vr) * D.ncap);
-        input_prefetch(ii, k0, S.spk_v + std::size_t(vr) * D.ncap);
-        input_prefetch(is, k0, S.sic_v + std::size_t(vr) * D.acap);
-    }
+    input_prefetch(is, static_cast<unsigned>(lo - B.bstart), S.sic_v + std::size_t(vr) * D.acap);
     R V = at(S.V, on), w = at(S.w, on), g_ex = at(S.g_ex, on), dg_ex = at(S.dg_ex, on), g_in = at(S.g_in, on),
       dg_in = at(S.dg_in, on);
     int refr = at(D.refr, o4);
@@ -446,9 +558,9 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         const std::int64_t tl = t - M.step0;  // the member's own step
         const R* nvals = S.spk_v + std::size_t(vr) * D.ncap;
         const R exc_sum =
-            input_sum(ie, k, ee, nwords, nvals, [&] { return D.in_rs[T_EXC][gj]; }, D.in_src[T_EXC]);
+            direct_sum(ie, k, ee, nwords, nvals, [&] { return D.in_rs[T_EXC][gj]; }, D.in_src[T_EXC]);
         const R inh_sum =
-            input_sum(ii, k, ei, nwords, nvals, [&] { return D.in_rs[T_INH][gj]; }, D.in_src[T_INH]);
+            direct_sum(ii, k, ei, nwords, nvals, [&] { return D.in_rs[T_INH][gj]; }, D.in_src[T_INH]);
         const bool sic_step = sic_wait == 0;
         sic_wait = sic_step ? P.sic_interval - 1 : sic_wait - 1;
         if (sic_step) {
@@ -458,11 +570,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
             else I_sic = Q.w_a2n * sic_sum;
         }
         const std::uint32_t vn = ring_next(vr, B.Rv);
-        if (t + 1 < hi) {
-            input_prefetch(ie, k + 1u, S.spk_v + std::size_t(vn) * D.ncap);
-            input_prefetch(ii, k + 1u, S.spk_v + std::size_t(vn) * D.ncap);
-            input_prefetch(is, k + 1u, S.sic_v + std::size_t(vn) * D.acap);
-        }
+        if (t + 1 < hi) input_prefetch(is, k + 1u, S.sic_v + std::size_t(vn) * D.acap);
         int ev = 0;
         if (exc ? P.poiss_exc : P.poiss_inh)
             ev = rng_poisson(M.seed ^ kKeyNeuronPoisson, pstream, exc ? P.lam_exc : P.lam_inh,
@@ -742,7 +850,7 @@ BatchLayout batch_layout(const BatchCapacity& c, const RunParams& rp) {
         o.out_rs[i] = L.add(ns * sizeof(uint2));
         o.out_ent[i] = L.add(ne * sizeof(uint2));
         o.bcnt[i] = L.add(nt * sizeof(unsigned));
-        o.bent[i] = L.add(nt * kBucketCap * sizeof(uint2));
+        o.bent[i] = L.add(nt * kBucketCap * sizeof(uint4));
     }
     const int n_astro_arrays = rsize == 4 ? 10 : 7;
     for (int i = 0; i < n_astro_arrays; ++i) o.astro[i] = L.add(na * rsize);
@@ -803,7 +911,7 @@ public:
             D_.out_rs[i] = (const uint2*)dptr(lay_.out_rs[i]);
             D_.out_ent[i] = (const uint2*)dptr(lay_.out_ent[i]);
             D_.bcnt[i] = (unsigned*)dptr(lay_.bcnt[i]);
-            D_.bent[i] = (uint2*)dptr(lay_.bent[i]);
+            D_.bent[i] = (uint4*)dptr(lay_.bent[i]);
         }
         R* A[10] = {};
         for (int i = 0; i < (sizeof(R) == 4 ? 10 : 7); ++i) A[i] = (R*)dptr(lay_.astro[i]);
@@ -918,7 +1026,7 @@ public:
             B.last = step_ + B.n == B.bstart + geo_.L;
             BatchDev D = D_;
             D.clock = nullptr;
-            if (B.first) hipLaunchKernelGGL(scatter_kernel, dim3(kScatterBlocks), dim3(256), 0, stream_, D, B);
+            if (B.first) hipLaunchKernelGGL(scatter_kernel<R>, dim3(kScatterBlocks), dim3(256), 0, stream_, D, B, S_.spk_v);
             timing_begin();
             launch_integrate(D, B);
             timing_end();
@@ -1212,7 +1320,7 @@ private:
         const BlkLaunch B = launch_for(0);  // the kernels take the block index from the clock
         AS_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         for (std::int64_t i = 0; i < blocks; ++i) {
-            hipLaunchKernelGGL(scatter_kernel, dim3(kScatterBlocks), dim3(256), 0, stream_, D, B);
+            hipLaunchKernelGGL(scatter_kernel<R>, dim3(kScatterBlocks), dim3(256), 0, stream_, D, B, S_.spk_v);
             launch_integrate(D, B);
             hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(64), 0, stream_, clock_);
         }
@@ -1399,6 +1507,8 @@ std::unique_ptr<Batch> make_batch(const RunParams& rp, const BatchCapacity& cap,
     if (cap.neurons + cap.astros >= (std::uint64_t(1) << 32) ||
         std::max({cap.edges[0], cap.edges[1], cap.edges[2], cap.edges[3]}) >= (std::uint64_t(1) << 32))
         throw Error(AS_ERR_INVALID, "batch capacity exceeds 32-bit ids");
+    if (std::max(cap.neurons, cap.astros) > kSrcMask)
+        throw Error(AS_ERR_INVALID, "batch capacity exceeds 2^26 cells of one kind (bucket entry source ids)");
     if (rp.precision == AS_FP64) return std::unique_ptr<Batch>(new BatchImpl<double>(rp, cap, s));
     return std::unique_ptr<Batch>(new BatchImpl<float>(rp, cap, s));
 }

@@ -195,6 +195,30 @@ def load_traffic(workload: str) -> float | None:
     return None
 
 
+def copy_calibration(nbytes: int) -> int:
+    """Three device-to-device copies of `nbytes`: the known-size reference
+    that FETCH_SIZE / WRITE_SIZE readings are scaled by (run under
+    rocprofv3 --pmc; each copy reads and writes nbytes)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            print("bench.py: hipMalloc failed", file=sys.stderr)
+            return 1
+    if hip.hipMemset(a, 1, C.c_size_t(nbytes)) != 0:
+        return 1
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            return 1
+    if hip.hipDeviceSynchronize() != 0:
+        return 1
+    log(f"copy calibration: {nbytes} bytes copied 3 times")
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,7 +232,11 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=10.0,
                     help="arrival window of the serving load (0 skips it; the full load is 120 s)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the known-size copy that HBM counter readings are scaled by")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
               "multi-GPU sharding is a design note, not built", file=sys.stderr)
