@@ -190,11 +190,12 @@ __device__ __forceinline__ void cas(uint4& a, uint4& b) {
     b = hi;
 }
 
-// Sorting networks (Batcher) for 2, 4 and 8 entries.
+// Sorting networks (Batcher) for 1, 2, 4 and 8 entries.
 template <int N>
 __device__ __forceinline__ void sort_entries(uint4 (&e)[N]) {
-    static_assert(N == 2 || N == 4 || N == 8, "network size");
-    if constexpr (N == 2) {
+    static_assert(N == 1 || N == 2 || N == 4 || N == 8, "network size");
+    if constexpr (N == 1) {
+    } else if constexpr (N == 2) {
         cas(e[0], e[1]);
     } else if constexpr (N == 4) {
         cas(e[0], e[1]); cas(e[2], e[3]);
@@ -268,8 +269,9 @@ __device__ R memory_sum(const uint4* ent, unsigned cnt, unsigned k, const std::u
 // Neuron-sourced inputs: attached (step, value) pairs in slot order.
 template <class R, int NR>
 struct DirectInputs {
+    static_assert(NR <= 4, "steps are packed in bytes");
     R v[NR];                 // values (zero where empty)
-    std::uint32_t k[NR];     // their block steps (32: empty)
+    std::uint32_t ks;        // byte r: block step of v[r] (0xFF: empty)
     unsigned cnt;            // bucket entries
     bool mem;                // the in-memory path
 };
@@ -278,11 +280,9 @@ template <class R, int NR>
 __device__ __forceinline__ void direct_init(DirectInputs<R, NR>& s, uint4* ent, unsigned cnt) {
     s.cnt = cnt;
     s.mem = cnt > unsigned(NR);
+    s.ks = 0xFFFFFFFFu;
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        s.v[r] = R(0);
-        s.k[r] = 32u;
-    }
+    for (int r = 0; r < NR; ++r) s.v[r] = R(0);
     if (cnt == 0u) return;
     if (cnt <= unsigned(NR)) {
         uint4 e[NR];
@@ -295,7 +295,7 @@ __device__ __forceinline__ void direct_init(DirectInputs<R, NR>& s, uint4* ent, 
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 if (e[r].x != kNoSlot) {
-                    s.k[r] = attached_step(e[r]);
+                    s.ks = (s.ks & ~(0xFFu << (8 * r))) | (attached_step(e[r]) << (8 * r));
                     s.v[r] = attached_value<R>(e[r]);
                 }
             }
@@ -310,13 +310,14 @@ __device__ __forceinline__ void direct_init(DirectInputs<R, NR>& s, uint4* ent, 
     }
 }
 
-template <class R, int NR, class RowF>
-__device__ __forceinline__ R direct_sum(const DirectInputs<R, NR>& s, unsigned k, const uint4* ent,
+// ent(): the target's bucket (computed on the in-memory path only).
+template <class R, int NR, class EntF, class RowF>
+__device__ __forceinline__ R direct_sum(const DirectInputs<R, NR>& s, unsigned k, EntF ent,
                                         const std::uint32_t* words, const R* vals, RowF row, const index_t* src) {
     R sum = R(0);
 #pragma unroll
-    for (int r = 0; r < NR; ++r) sum += s.k[r] == k ? s.v[r] : R(0);
-    if (s.mem) sum = memory_sum<R>(ent, s.cnt, k, words, vals, row, src);
+    for (int r = 0; r < NR; ++r) sum += ((s.ks >> (8 * r)) & 0xFFu) == k ? s.v[r] : R(0);
+    if (s.mem) sum = memory_sum<R>(ent(), s.cnt, k, words, vals, row, src);
     return sum;
 }
 
@@ -369,14 +370,14 @@ __device__ __forceinline__ void input_prefetch(InputStream<R, NR>& s, unsigned k
 
 // The input sum of block step k: the prefetched values in slot order, or the
 // in-memory path.
-template <class R, int NR, class RowF>
-__device__ __forceinline__ R input_sum(const InputStream<R, NR>& s, unsigned k, const uint4* ent,
+template <class R, int NR, class EntF, class RowF>
+__device__ __forceinline__ R input_sum(const InputStream<R, NR>& s, unsigned k, EntF ent,
                                        const std::uint32_t* words, const R* vals, RowF row,
                                        const index_t* src) {
     R sum = R(0);
 #pragma unroll
     for (int r = 0; r < NR; ++r) sum += s.v[r];
-    if (s.cnt > unsigned(NR)) sum = memory_sum<R>(ent, s.cnt, k, words, vals, row, src);
+    if (s.cnt > unsigned(NR)) sum = memory_sum<R>(ent(), s.cnt, k, words, vals, row, src);
     return sum;
 }
 
@@ -450,9 +451,9 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
                            std::int64_t hi) {
     const std::uint32_t oa = ga * std::uint32_t(sizeof(R)), oa4 = ga * 4u;
     const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
-    uint4* ent = D.bent[T_E2A] + std::size_t(ga) * kBucketCap;
+    auto ent = [&] { return D.bent[T_E2A] + std::size_t(ga) * kBucketCap; };
     DirectInputs<R, 4> in;
-    direct_init(in, ent, at(D.bcnt[T_E2A], oa4));
+    direct_init(in, ent(), at(D.bcnt[T_E2A], oa4));
     std::uint32_t vr = ring_row(lo - B.d, B.Rv), pr = ring_row(lo, B.Rv);
     R ca = at(S.Ca, oa), ip3 = at(S.IP3, oa), hh = at(S.h, oa);
     R ca_lo = R(0), ip3_lo = R(0), hh_lo = R(0);
@@ -465,11 +466,12 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
     std::uint32_t wout = B.first ? 0u : at(D.sic_w + std::size_t(B.wslot) * D.acap, oa4);
     bool ever = false;
     // random streams of the member's step tl = t - step0 and cell a
-    std::uint64_t stream = std::uint64_t(lo - M.step0) * kStepStride + a;
+    std::uint64_t tl_stride = std::uint64_t(lo - M.step0) * kStepStride;  // wave-uniform
     for (std::int64_t t = lo; t < hi; ++t) {
         const unsigned k = static_cast<unsigned>(t - B.bstart);
         const R* vals = S.spk_v + std::size_t(vr) * D.ncap;
         const R gathered = direct_sum(in, k, ent, nwords, vals, [&] { return D.in_rs[T_E2A][ga]; }, D.in_src[T_E2A]);
+        const std::uint64_t stream = tl_stride + a;
         int ev = 0;
         if (P.poiss_astro) ev = rng_poisson(M.seed ^ kKeyAstroPoisson, stream, P.lam_astro, P.lim_astro);
         const double noise = P.noise_astro > 0.0 ? P.noise_astro * rng_normal(M.seed ^ kKeyAstroNoise, stream) : 0.0;
@@ -493,7 +495,7 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
         }
         vr = ring_next(vr, B.Rv);
         pr = ring_next(pr, B.Rv);
-        stream += kStepStride;
+        tl_stride += kStepStride;
     }
     at(S.Ca, oa) = ca;
     at(S.IP3, oa) = ip3;
@@ -511,24 +513,24 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
 
 // One neuron (global id gj, local id j) over global steps [lo, hi)
 // (src/astrosim_fused.cu:433-587). Returns the number of spikes.
-template <class R>
+template <class R, bool exc>
 __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
                                 const Member& M, const BlkLaunch& B, index_t gj, index_t j, std::int64_t lo,
-                                std::int64_t hi, bool exc, bool* emit) {
+                                std::int64_t hi, bool* emit) {
     const std::uint32_t on = gj * std::uint32_t(sizeof(R)), o4 = gj * 4u;
     const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
     const std::uint32_t* awords = D.sic_w + std::size_t(B.rslot) * D.acap;
-    uint4* ee = D.bent[T_EXC] + std::size_t(gj) * kBucketCap;
-    uint4* ei = D.bent[T_INH] + std::size_t(gj) * kBucketCap;
-    uint4* es = D.bent[T_SIC] + std::size_t(gj) * kBucketCap;
+    auto ee = [&] { return D.bent[T_EXC] + std::size_t(gj) * kBucketCap; };
+    auto ei = [&] { return D.bent[T_INH] + std::size_t(gj) * kBucketCap; };
+    auto es = [&] { return D.bent[T_SIC] + std::size_t(gj) * kBucketCap; };
     // inhibitory source ids are local indices used as neuron ids
     // (src/astrosim_fused.cu:459-462); the scatter has already resolved them
     DirectInputs<R, 4> ie;
     DirectInputs<R, 2> ii;
-    InputStream<R, 2> is;
-    direct_init(ie, ee, at(D.bcnt[T_EXC], o4));
-    direct_init(ii, ei, at(D.bcnt[T_INH], o4));
-    input_init(is, es, at(D.bcnt[T_SIC], o4), awords);
+    InputStream<R, 1> is;
+    direct_init(ie, ee(), at(D.bcnt[T_EXC], o4));
+    direct_init(ii, ei(), at(D.bcnt[T_INH], o4));
+    input_init(is, es(), at(D.bcnt[T_SIC], o4), awords);
     std::uint32_t vr = ring_row(lo - B.d, B.Rv), pr = ring_row(lo, B.Rv);
     input_prefetch(is, static_cast<unsigned>(lo - B.bstart), S.sic_v + std::size_t(vr) * D.acap);
     R V = at(S.V, on), w = at(S.w, on), g_ex = at(S.g_ex, on), dg_ex = at(S.dg_ex, on), g_in = at(S.g_in, on),
@@ -545,7 +547,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     // tl >= d_a2n with (tl - d_a2n) % sic_interval == 0), and the noise
     // index tl / steps_per_noise with its remainder.
     const std::int64_t tl0 = lo - M.step0;
-    std::uint64_t pstream = std::uint64_t(tl0) * kStepStride + j;
+    std::uint64_t tl_stride = std::uint64_t(tl0) * kStepStride;  // wave-uniform
     int sic_wait;
     {
         const std::int64_t u = tl0 - P.d_a2n;
@@ -573,7 +575,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         if (t + 1 < hi) input_prefetch(is, k + 1u, S.sic_v + std::size_t(vn) * D.acap);
         int ev = 0;
         if (exc ? P.poiss_exc : P.poiss_inh)
-            ev = rng_poisson(M.seed ^ kKeyNeuronPoisson, pstream, exc ? P.lam_exc : P.lam_inh,
+            ev = rng_poisson(M.seed ^ kKeyNeuronPoisson, tl_stride + j, exc ? P.lam_exc : P.lam_inh,
                              exc ? P.lim_exc : P.lim_inh);
         if (nstd > 0.0 && (t == lo || nrem == 0))  // the noise stream changes every steps_per_noise steps
             noise_val = static_cast<R>(nstd * rng_normal(M.seed ^ kKeyNeuronNoise, nidx * kNoiseStride + std::uint64_t(j)));
@@ -629,7 +631,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         }
         vr = vn;
         pr = ring_next(pr, B.Rv);
-        pstream += kStepStride;
+        tl_stride += kStepStride;
         if (++nrem == P.steps_per_noise) {
             nrem = 0;
             ++nidx;
@@ -664,13 +666,20 @@ struct IntegrateCtx {
     Params32 Q;
 };
 
-// Minimum waves per SIMD the integrate kernel's register budget targets.
-#ifndef AS_INTEGRATE_MIN_WAVES
-#define AS_INTEGRATE_MIN_WAVES 2
+// Minimum waves per SIMD the integrate kernel's register budget targets, per
+// state precision (chosen by measurement: past these, spills cost more than
+// the extra waves hide).
+#ifndef AS_INTEGRATE_WAVES_F64
+#define AS_INTEGRATE_WAVES_F64 2
 #endif
+#ifndef AS_INTEGRATE_WAVES_F32
+#define AS_INTEGRATE_WAVES_F32 3
+#endif
+template <class R>
+constexpr int kIntegrateWaves = sizeof(R) == 8 ? AS_INTEGRATE_WAVES_F64 : AS_INTEGRATE_WAVES_F32;
 
 template <int MAXT, class R>
-__global__ __launch_bounds__(MAXT, AS_INTEGRATE_MIN_WAVES) void integrate_kernel(const IntegrateCtx<R>* __restrict__ ctx, BlkLaunch B) {
+__global__ __launch_bounds__(MAXT, kIntegrateWaves<R>) void integrate_kernel(const IntegrateCtx<R>* __restrict__ ctx, BlkLaunch B) {
     const BatchDev& D = ctx->D;
     const BlkState<R>& S = ctx->S;
     const RunParams& P = ctx->P;
@@ -692,7 +701,13 @@ __global__ __launch_bounds__(MAXT, AS_INTEGRATE_MIN_WAVES) void integrate_kernel
     const bool run = local < tile.end && lo < hi;
     bool emit = false;
     unsigned nspikes = 0;
-    if (run) nspikes = neuron_cell<R>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, tile.kind == 1, &emit);
+    if (run) {
+        // one instantiation per population keeps only its constants live
+        if (tile.kind == 1)
+            nspikes = neuron_cell<R, true>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, &emit);
+        else
+            nspikes = neuron_cell<R, false>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, &emit);
+    }
     wave_append(emit, M.nb + local, D.nlist + std::size_t(B.wslot) * D.ncap, &D.lcount[B.wslot * 2 + 0]);
     // diagnostic spike total of the member: one atomic per wave with spikes
     unsigned long long c = nspikes;
