@@ -670,7 +670,7 @@ struct IntegrateCtx {
 // state precision (chosen by measurement: past these, spills cost more than
 // the extra waves hide).
 #ifndef AS_INTEGRATE_WAVES_F64
-#define AS_INTEGRATE_WAVES_F64 2
+#define AS_INTEGRATE_WAVES_F64 3
 #endif
 #ifndef AS_INTEGRATE_WAVES_F32
 #define AS_INTEGRATE_WAVES_F32 3

@@ -85,6 +85,11 @@ struct NeuronConst {
     double C_m, g_L, E_L, V_th, Delta_T, a, tau_w;
     double V_peak, t_ref, E_ex, E_in, tau_syn_ex, tau_syn_in, I_e;
     double psc_init_ex, psc_init_in;  // e / tau_syn
+    // Delta_T a power of two: x / Delta_T is the exact scaling x * inv_Delta_T,
+    // bit for bit (a division by 2^k and a multiplication by 2^-k round the
+    // same real number), so the spike-current argument needs no division.
+    double inv_Delta_T;
+    int Delta_T_pow2;
 };
 
 AstroConst make_astro_const(const Biology& bio);
@@ -148,7 +153,8 @@ AS_HD double sic_factor(const AstroConst& c, double Ca) {
 AS_HD void neuron_rhs(const NeuronConst& p, double V, double w, double g_ex, double g_in,
                       double I_ext, double& dV, double& dw) {
     const double Vc = V < p.V_peak ? V : p.V_peak;
-    const double I_spike = p.g_L * p.Delta_T * as_exp((Vc - p.V_th) / p.Delta_T);
+    const double x = p.Delta_T_pow2 ? (Vc - p.V_th) * p.inv_Delta_T : (Vc - p.V_th) / p.Delta_T;
+    const double I_spike = p.g_L * p.Delta_T * as_exp(x);
     const double I_syn_ex = g_ex * (Vc - p.E_ex);
     const double I_syn_in = g_in * (Vc - p.E_in);
     dV = (-p.g_L * (Vc - p.E_L) + I_spike - I_syn_ex - I_syn_in - w + p.I_e + I_ext) / p.C_m;

@@ -58,8 +58,13 @@ AS_HD int rng_poisson(std::uint64_t seed, std::uint64_t stream, double lambda, d
         const double v = lambda + __builtin_sqrt(lambda) * z;
         return v < 0.0 ? 0 : static_cast<int>(v + 0.5);
     }
-    std::uint64_t c = 0;
-    double product = rng_uniform(seed, stream, c++);
+    // The first test, u0 > limit, is decided in integers: u0 = m0 * 2^-53
+    // with integer m0, so u0 > limit <=> m0 > floor(limit * 2^53) (exact
+    // scaling; limit is in (0, 1)). Most draws stop here.
+    const std::uint64_t m0 = rng_bits(seed, stream, 0) >> 11;
+    if (m0 <= static_cast<std::uint64_t>(limit * 9007199254740992.0)) return 0;
+    std::uint64_t c = 1;
+    double product = static_cast<double>(m0) * (1.0 / 9007199254740992.0);
     int count = 0;
     while (product > limit) {
         ++count;

@@ -61,6 +61,10 @@ NeuronConst make_neuron_const(const Biology& bio, bool exc) {
     p.I_e = 0.0;
     p.psc_init_ex = 2.7182818284590452353602874713527 / p.tau_syn_ex;
     p.psc_init_in = 2.7182818284590452353602874713527 / p.tau_syn_in;
+    int e = 0;
+    const bool pow2 = p.Delta_T > 0.0 && std::frexp(p.Delta_T, &e) == 0.5;
+    p.inv_Delta_T = 1.0 / p.Delta_T;
+    p.Delta_T_pow2 = pow2 && std::isnormal(p.inv_Delta_T) ? 1 : 0;
     return p;
 }
 
