@@ -79,7 +79,10 @@ def kernel_bytes_per_launch(net, prec: str, method: str) -> float:
 
 def run_leg(A, net, prec: str, method: str, steps: int, warmup: int) -> dict:
     log(f"leg {prec}/{method}: create")
+    t0 = time.perf_counter()
     sim = A.Simulation(net, precision=prec, method=method)
+    sim.sync()
+    create_s = time.perf_counter() - t0  # host-to-device upload of the network + device setup
     sim.step(warmup)
     sim.sync()
     sim.set_timing(True)
@@ -97,6 +100,7 @@ def run_leg(A, net, prec: str, method: str, steps: int, warmup: int) -> dict:
         "kernel_ms_per_launch": kms / max(1, launches),
         "steps_per_launch": steps / max(1, launches),
         "device_bytes": sim.device_bytes(), "spikes_total": spikes,
+        "create_s": create_s,
     }
     sim.close()
     log(f"leg {prec}/{method}: {out['steps_per_s']:.1f} steps/s, {out['kernel_ms_per_launch']:.3f} ms/launch")
