@@ -233,45 +233,62 @@ private:
     std::unique_ptr<SkipWindows> skips_;
 };
 
-// Target-partitioned stable inversion. Edges are visited in generation
-// order by `visit(lo, hi, emit)`, which calls emit(src, dst) for every edge
-// whose dst lies in [lo, hi); each thread owns a contiguous target range,
-// counts its in-degrees, then scatters sources in generation order, so every
-// row lists its sources exactly as the reference's counting sort does
-// (src/astrosim_fused.cu:307-314).
+// Stable inversion of a generation-ordered edge sequence into a
+// target-major CSR, as the reference's counting sort (count, prefix,
+// scatter in generation order; src/astrosim_fused.cu:307-314), in O(edges)
+// work over the threads. `visit(lo, hi, emit)` calls emit(src, dst) for the
+// sequence positions [lo, hi) in order. The sequence is cut into one
+// contiguous chunk per thread; each thread counts its chunk's targets, the
+// per-(chunk, target) counts become write cursors (chunks in order inside
+// each row), and each thread scatters its chunk in order, so every row lists
+// its sources in generation order.
 template <class Visit>
-void invert_parallel(index_t n_targets, std::size_t n_edges, unsigned threads, Visit&& visit, HostCSR& out) {
+void invert_stable(index_t n_targets, std::size_t n_edges, unsigned threads, Visit&& visit, HostCSR& out) {
     out.row_start.assign(std::size_t(n_targets) + 1, 0);
     out.src.assign(n_edges, 0);
-    threads = std::max(1u, std::min<unsigned>(threads, std::max<index_t>(1, n_targets / 1024)));
-    std::vector<std::uint64_t> totals(threads, 0);
-    auto lo_of = [&](unsigned t) { return static_cast<index_t>(std::uint64_t(n_targets) * t / threads); };
-    // pass 1: in-degrees of each thread's range
-    parallel_for_threads(threads, [&](unsigned t) {
-        const index_t lo = lo_of(t), hi = lo_of(t + 1);
-        index_t* cnt = &out.row_start[std::size_t(lo) + 1];
-        visit(lo, hi, [&](index_t, index_t d) { cnt[d - lo]++; });
-        std::uint64_t s = 0;
-        for (index_t i = lo; i < hi; ++i) s += out.row_start[std::size_t(i) + 1];
-        totals[t] = s;
+    if (n_targets == 0) return;
+    // at most 2^28 counters (1 GiB), and chunks of at least 2^16 edges
+    const std::uint64_t by_mem = std::max<std::uint64_t>(1, (std::uint64_t(1) << 28) / n_targets);
+    const std::uint64_t by_work = std::max<std::uint64_t>(1, n_edges >> 16);
+    const unsigned T = static_cast<unsigned>(std::max<std::uint64_t>(1, std::min({std::uint64_t(threads), by_mem, by_work})));
+    const std::size_t n = n_targets;
+    std::vector<index_t> cnt(std::size_t(T) * n, 0);  // [chunk][target]
+    auto chunk_lo = [&](unsigned t) { return static_cast<std::size_t>(std::uint64_t(n_edges) * t / T); };
+    parallel_for_threads(T, [&](unsigned t) {
+        index_t* c = &cnt[std::size_t(t) * n];
+        visit(chunk_lo(t), chunk_lo(t + 1), [&](index_t, index_t d) { ++c[d]; });
     });
-    // exclusive prefix over ranges, then within each range
-    std::vector<std::uint64_t> base(threads, 0);
-    for (unsigned t = 1; t < threads; ++t) base[t] = base[t - 1] + totals[t - 1];
-    parallel_for_threads(threads, [&](unsigned t) {
-        const index_t lo = lo_of(t), hi = lo_of(t + 1);
-        std::uint64_t run = base[t];
-        for (index_t i = lo; i < hi; ++i) {
-            run += out.row_start[std::size_t(i) + 1];
-            out.row_start[std::size_t(i) + 1] = static_cast<index_t>(run);
+    // per target: chunk cursors relative to the row start, and the in-degree
+    auto tgt_lo = [&](unsigned t) { return static_cast<std::size_t>(std::uint64_t(n) * t / T); };
+    std::vector<std::uint64_t> range_total(T, 0);
+    parallel_for_threads(T, [&](unsigned r) {
+        std::uint64_t total = 0;
+        for (std::size_t d = tgt_lo(r); d < tgt_lo(r + 1); ++d) {
+            index_t run = 0;
+            for (unsigned t = 0; t < T; ++t) {
+                const index_t c = cnt[std::size_t(t) * n + d];
+                cnt[std::size_t(t) * n + d] = run;
+                run += c;
+            }
+            out.row_start[d + 1] = run;
+            total += run;
+        }
+        range_total[r] = total;
+    });
+    std::vector<std::uint64_t> range_base(T, 0);
+    for (unsigned r = 1; r < T; ++r) range_base[r] = range_base[r - 1] + range_total[r - 1];
+    parallel_for_threads(T, [&](unsigned r) {
+        std::uint64_t run = range_base[r];
+        for (std::size_t d = tgt_lo(r); d < tgt_lo(r + 1); ++d) {
+            run += out.row_start[d + 1];
+            out.row_start[d + 1] = static_cast<index_t>(run);
         }
     });
-    // pass 2: scatter in generation order
-    parallel_for_threads(threads, [&](unsigned t) {
-        const index_t lo = lo_of(t), hi = lo_of(t + 1);
-        std::vector<index_t> cursor(out.row_start.begin() + lo, out.row_start.begin() + hi);
+    parallel_for_threads(T, [&](unsigned t) {
+        index_t* c = &cnt[std::size_t(t) * n];
         index_t* src = out.src.data();
-        visit(lo, hi, [&](index_t s, index_t d) { src[cursor[d - lo]++] = s; });
+        const index_t* rs = out.row_start.data();
+        visit(chunk_lo(t), chunk_lo(t + 1), [&](index_t s, index_t d) { src[rs[d] + c[d]++] = s; });
     });
 }
 
@@ -413,38 +430,36 @@ void build_network(index_t n_total, std::uint64_t seed, Network& net) {
     tf_pre.resize(kept); tf_post.resize(kept); tf_astro.resize(kept);
     const auto t_gen = std::chrono::steady_clock::now();
 
-    // Four stable inversions, each target-partitioned over the worker threads.
-    auto visit_source_major = [](const SourceMajor& pass) {
-        return [&pass](index_t lo, index_t hi, auto&& emit) {
-            const std::size_t n_src = pass.off.size() - 1;
-            for (std::size_t s = 0; s < n_src; ++s)
-                for (std::uint64_t k = pass.off[s]; k < pass.off[s + 1]; ++k) {
-                    const index_t d = pass.dst[k];
-                    if (d >= lo && d < hi) emit(static_cast<index_t>(s), d);
-                }
-        };
+    // Four stable inversions over the worker threads.
+    // Sequence positions of a source-major pass: source s owns [off[s], off[s+1]).
+    auto visit_pass = [](const SourceMajor& pass, std::uint64_t lo, std::uint64_t hi, auto&& emit) {
+        if (lo >= hi) return;
+        std::size_t s = static_cast<std::size_t>(
+            std::upper_bound(pass.off.begin(), pass.off.end(), lo) - pass.off.begin()) - 1;
+        for (std::uint64_t k = lo; k < hi; ++k) {
+            while (pass.off[s + 1] <= k) ++s;
+            emit(static_cast<index_t>(s), pass.dst[k]);
+        }
     };
     {
-        auto v0 = visit_source_major(exc_pass[0]);
-        auto v1 = visit_source_major(exc_pass[1]);
-        // pass 0 holds targets < n_exc, pass 1 targets >= n_exc
-        invert_parallel(n_neurons, exc_pass[0].dst.size() + exc_pass[1].dst.size(), threads,
-                        [&](index_t lo, index_t hi, auto&& emit) {
-                            if (lo < n_exc) v0(lo, std::min(hi, n_exc), emit);
-                            if (hi > n_exc) v1(std::max(lo, n_exc), hi, emit);
-                        }, net.inc_exc);
+        // pass 0 (targets < n_exc) then pass 1 (targets >= n_exc): no target in both
+        const std::uint64_t e0 = exc_pass[0].dst.size(), e1 = exc_pass[1].dst.size();
+        invert_stable(n_neurons, e0 + e1, threads, [&](std::uint64_t lo, std::uint64_t hi, auto&& emit) {
+            visit_pass(exc_pass[0], lo, std::min(hi, e0), emit);
+            visit_pass(exc_pass[1], std::max(lo, e0) - e0, std::max(hi, e0) - e0, emit);
+        }, net.inc_exc);
     }
     exc_pass[0] = SourceMajor();
     exc_pass[1] = SourceMajor();
-    invert_parallel(n_neurons, inh_pass.dst.size(), threads, visit_source_major(inh_pass), net.inc_inh);
+    invert_stable(n_neurons, inh_pass.dst.size(), threads, [&](std::uint64_t lo, std::uint64_t hi, auto&& emit) {
+        visit_pass(inh_pass, lo, hi, emit);
+    }, net.inc_inh);
     inh_pass = SourceMajor();
-    invert_parallel(n_neurons, kept, threads, [&](index_t lo, index_t hi, auto&& emit) {
-        for (std::size_t i = 0; i < kept; ++i)
-            if (tf_post[i] >= lo && tf_post[i] < hi) emit(tf_astro[i], tf_post[i]);
+    invert_stable(n_neurons, kept, threads, [&](std::uint64_t lo, std::uint64_t hi, auto&& emit) {
+        for (std::uint64_t i = lo; i < hi; ++i) emit(tf_astro[i], tf_post[i]);
     }, net.inc_sic);
-    invert_parallel(n_astro, kept, threads, [&](index_t lo, index_t hi, auto&& emit) {
-        for (std::size_t i = 0; i < kept; ++i)
-            if (tf_astro[i] >= lo && tf_astro[i] < hi) emit(tf_pre[i], tf_astro[i]);
+    invert_stable(n_astro, kept, threads, [&](std::uint64_t lo, std::uint64_t hi, auto&& emit) {
+        for (std::uint64_t i = lo; i < hi; ++i) emit(tf_pre[i], tf_astro[i]);
     }, net.inc_e2a);
     const auto t1 = std::chrono::steady_clock::now();
     net.build_seconds = std::chrono::duration<double>(t1 - t0).count();

@@ -91,6 +91,16 @@ def test_blocked_larger_network_bit_exact():
 
 
 @pytest.mark.parametrize("method", METHODS)
+def test_high_activity_bit_exact(method):
+    # Strong n2a/a2n weights: normal activity for ~150 ms, then SIC-driven
+    # firing at up to one spike per step. The blocked method crosses every
+    # input path: attached single-spike entries, sources that spiked several
+    # times in a block (values read from the ring), more entries than fit in
+    # registers, and overflowed buckets (the whole in-row).
+    run_pair(2000, 7, 2000, w_n2a=100.0, w_a2n=5000.0, method=method)
+
+
+@pytest.mark.parametrize("method", METHODS)
 def test_config1_full_run_summary_matches(method):
     # pre 1000 ms + sim 5000 ms as the reference's default run
     net, onet = A.Network(500, 1), O.OracleNet(500, 1)

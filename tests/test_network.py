@@ -9,7 +9,10 @@ import pytest
 import astrosimgpu_cuda_amd as A
 import pyoracle as O
 
-CASES = [(1, 1), (2, 3), (7, 1), (20, 5), (99, 2), (100, 1), (101, 4), (500, 1), (5000, 7), (40000, 11)]
+# 300,001 cells build on several host threads (one per 65,536 cells), so the
+# chunked inversion and the skip windows run in parallel there.
+CASES = [(1, 1), (2, 3), (7, 1), (20, 5), (99, 2), (100, 1), (101, 4), (500, 1), (5000, 7), (40000, 11),
+         (300001, 13)]
 
 
 @pytest.mark.parametrize("n,seed", CASES)
