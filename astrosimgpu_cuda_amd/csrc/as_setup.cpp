@@ -161,39 +161,68 @@ void parallel_for_threads(unsigned n, F&& f) {
 class SkipWindows {
 public:
     // window: counters per window (sized to the network so small builds do
-    // not precompute far past their last draw)
-    SkipWindows(std::uint64_t seed, std::uint64_t stream, double log1m_p, std::uint64_t first,
+    // not precompute far past their last draw). take: astrocyte pool size.
+    SkipWindows(std::uint64_t seed, std::uint64_t stream, double log1m_p, int take, std::uint64_t first,
                 unsigned threads, std::uint64_t window)
-        : kWindow(window), seed_(seed), stream_(stream), log1m_p_(log1m_p), threads_(threads) {
+        : kWindow(window), seed_(seed), stream_(stream), log1m_p_(log1m_p), take_(take), threads_(threads) {
         cur_.resize(kWindow);
         nxt_.resize(kWindow);
+        cur_aux_.resize(kWindow);
+        nxt_aux_.resize(kWindow);
         cur_base_ = first;
-        fill(cur_, cur_base_);
-        pending_ = std::async(std::launch::async, [this] { fill(nxt_, cur_base_ + kWindow); });
+        fill(cur_, cur_aux_, cur_base_);
+        pending_ = std::async(std::launch::async, [this] { fill(nxt_, nxt_aux_, cur_base_ + kWindow); });
     }
     ~SkipWindows() {
         if (pending_.valid()) pending_.wait();
     }
 
-    std::uint32_t at(std::uint64_t c) {
+    // The draw at counter c read as a geometric skip.
+    std::uint32_t skip(std::uint64_t c) { return cur_[slide(c)]; }
+    // The draw at counter c read as the third-factor coin (bit 0) and as the
+    // pool pick (bits 1-3).
+    std::uint8_t aux(std::uint64_t c) { return cur_aux_[slide(c)]; }
+
+private:
+    std::size_t slide(std::uint64_t c) {
         if (c >= cur_base_ + kWindow) {
             pending_.wait();
             std::swap(cur_, nxt_);
+            std::swap(cur_aux_, nxt_aux_);
             cur_base_ += kWindow;
-            pending_ = std::async(std::launch::async, [this] { fill(nxt_, cur_base_ + kWindow); });
+            pending_ = std::async(std::launch::async, [this] { fill(nxt_, nxt_aux_, cur_base_ + kWindow); });
         }
-        return cur_[c - cur_base_];
+        return static_cast<std::size_t>(c - cur_base_);
     }
 
-private:
-    void fill(std::vector<std::uint32_t>& buf, std::uint64_t base) {
+    // g = floor(log1p(-u) / log1p(-p)) with glibc's log1p, as the reference.
+    // Fast path: u lies on the 2^-53 grid, so 1 - u is exact and log1p(-u) =
+    // log(1 - u) exactly; as_log evaluates it within a few ulps and glibc's
+    // log1p within one, so the two quotients differ by less than 2^-46 of
+    // their size. When the fast quotient is farther than that from an
+    // integer, both floors agree; otherwise glibc decides.
+    static std::uint32_t skip_of(double u, double log1m_p) {
+        const double q = as_log(1.0 - u) / log1m_p;
+        const double margin = std::fabs(q) * 0x1p-46 + 0x1p-1000;
+        double g = std::floor(q - margin);
+        if (g != std::floor(q + margin)) g = std::floor(std::log1p(-u) / log1m_p);
+        return g >= 4294967295.0 ? 0xFFFFFFFFu : static_cast<std::uint32_t>(g);
+    }
+
+    // Every counter is decoded all three ways the walk may read it; the
+    // expressions are those of the sequential draws (src/astrosim_fused.cu:
+    // 617-624 for the skip, 703-706 for the coin and the pool pick).
+    void fill(std::vector<std::uint32_t>& buf, std::vector<std::uint8_t>& aux, std::uint64_t base) {
         parallel_for_threads(threads_, [&](unsigned t) {
             const std::uint64_t lo = kWindow * t / threads_, hi = kWindow * (t + 1) / threads_;
             for (std::uint64_t i = lo; i < hi; ++i) {
-                double u = rng_uniform(seed_, stream_, base + i);
-                if (u >= 1.0) u = std::nextafter(1.0, 0.0);
-                const double g = std::floor(std::log1p(-u) / log1m_p_);
-                buf[i] = g >= 4294967295.0 ? 0xFFFFFFFFu : static_cast<std::uint32_t>(g);
+                const double r = rng_uniform(seed_, stream_, base + i);
+                const double u = r >= 1.0 ? std::nextafter(1.0, 0.0) : r;
+                buf[i] = skip_of(u, log1m_p_);
+                const unsigned coin = r < ScalingConfig::kPThird ? 1u : 0u;
+                const unsigned slot = static_cast<unsigned>(static_cast<std::size_t>(r * take_) %
+                                                            static_cast<std::size_t>(take_));
+                aux[i] = static_cast<std::uint8_t>(coin | (slot << 1));
             }
         });
     }
@@ -201,35 +230,50 @@ private:
     const std::uint64_t kWindow;
     std::uint64_t seed_, stream_;
     double log1m_p_;
+    int take_;
     unsigned threads_;
     std::vector<std::uint32_t> cur_, nxt_;
+    std::vector<std::uint8_t> cur_aux_, nxt_aux_;
     std::uint64_t cur_base_ = 0;
     std::future<void> pending_;
 };
 
-// Sequential draws on the setup stream: uniforms are computed directly,
-// geometric skips come from SkipWindows. Mirrors HostRng::uniform and
-// geometric_skip (src/astrosim_fused.cu:183, 617-624) draw for draw.
+// Sequential draws on the setup stream, decoded from SkipWindows (or computed
+// directly when p is 0 or 1). Mirrors HostRng::uniform and geometric_skip
+// (src/astrosim_fused.cu:183, 617-624) draw for draw.
 class SetupDraws {
 public:
-    SetupDraws(std::uint64_t seed, std::uint64_t counter, double p, unsigned threads, std::uint64_t expected_draws)
-        : seed_(seed), counter_(counter), p_(p) {
+    SetupDraws(std::uint64_t seed, std::uint64_t counter, double p, int take, unsigned threads,
+               std::uint64_t expected_draws)
+        : seed_(seed), counter_(counter), p_(p), take_(take) {
         std::uint64_t window = std::uint64_t(1) << 16;
         while (window < (std::uint64_t(1) << 22) && window * 8 < expected_draws) window <<= 1;
         if (p > 0.0 && p < 1.0)
-            skips_.reset(new SkipWindows(seed, kSetupStream, std::log1p(-p), counter, threads, window));
+            skips_.reset(new SkipWindows(seed, kSetupStream, std::log1p(-p), take, counter, threads, window));
     }
-    double uniform() { return rng_uniform(seed_, kSetupStream, counter_++); }
     index_t skip(index_t remaining) {
         if (remaining == 0 || p_ >= 1.0) return 0;
         if (p_ <= 0.0) return remaining;
-        const std::uint32_t g = skips_->at(counter_++);
+        const std::uint32_t g = skips_->skip(counter_++);
         return g >= remaining ? remaining : g;
+    }
+    // uniform() < kPThird
+    bool third_factor() {
+        if (skips_) return (skips_->aux(counter_++) & 1u) != 0;
+        return uniform() < ScalingConfig::kPThird;
+    }
+    // static_cast<size_t>(uniform() * take) % take
+    std::size_t pool_pick() {
+        if (skips_) return skips_->aux(counter_++) >> 1;
+        return static_cast<std::size_t>(uniform() * take_) % static_cast<std::size_t>(take_);
     }
 
 private:
+    double uniform() { return rng_uniform(seed_, kSetupStream, counter_++); }
+
     std::uint64_t seed_, counter_;
     double p_;
+    int take_;
     std::unique_ptr<SkipWindows> skips_;
 };
 
@@ -356,7 +400,8 @@ void build_network(index_t n_total, std::uint64_t seed, Network& net) {
     // as (pre, post, pool slot) and resolved after the sequential pass.
     const double p = cfg.p_primary;
     // about 2.3 draws per synapse (skip, third-factor coin, pool pick on 20%)
-    SetupDraws dr(seed, rng.counter(), p, threads, static_cast<std::uint64_t>(2.5 * ScalingConfig::kSyn * n_neurons));
+    SetupDraws dr(seed, rng.counter(), p, take, threads,
+                  static_cast<std::uint64_t>(2.5 * ScalingConfig::kSyn * n_neurons));
     SourceMajor exc_pass[2], inh_pass;
     std::vector<index_t> tf_pre, tf_post;
     std::vector<std::uint8_t> tf_slot;
@@ -379,8 +424,8 @@ void build_network(index_t n_total, std::uint64_t seed, Network& net) {
                 ++pos;
                 if (pre == post) continue;  // no autapses
                 pass.dst.push_back(post);
-                if (dr.uniform() >= ScalingConfig::kPThird) continue;  // n_astro >= 1 always
-                const std::size_t slot = static_cast<std::size_t>(dr.uniform() * take) % static_cast<std::size_t>(take);
+                if (!dr.third_factor()) continue;  // n_astro >= 1 always
+                const std::size_t slot = dr.pool_pick();
                 tf_pre.push_back(pre);
                 tf_post.push_back(post);
                 tf_slot.push_back(static_cast<std::uint8_t>(slot));
