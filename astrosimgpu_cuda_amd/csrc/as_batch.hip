@@ -101,6 +101,8 @@ struct BlkLaunch {
     int d, L, H, Rv;   // delay, block length, history blocks, value-ring depth
     std::uint32_t wslot, rslot;  // history slots of block b and block b - d/L
     int first, last;   // the launch starts / completes its block
+    int chunk;         // most steps one integrate launch advances (LDS staging)
+    int sub;           // graph replays: first step of the launch within its block
 };
 
 // Graph replays read the block index from device memory instead of the
@@ -109,10 +111,10 @@ __device__ __forceinline__ void resolve_launch(BlkLaunch& B, const std::int64_t*
     if (!clock) return;
     const std::int64_t b = *clock;
     B.bstart = b * B.L;
-    B.t0 = B.bstart;
-    B.n = B.L;
-    B.first = 1;
-    B.last = 1;
+    B.t0 = B.bstart + B.sub;
+    B.n = B.L - B.sub < B.chunk ? B.L - B.sub : B.chunk;
+    B.first = B.sub == 0;
+    B.last = B.sub + B.n == B.L;
     B.wslot = static_cast<std::uint32_t>(b % B.H);
     B.rslot = static_cast<std::uint32_t>(((b - B.d / B.L) % B.H + B.H) % B.H);
 }
@@ -239,22 +241,75 @@ __device__ __forceinline__ R attached_value(const uint4& e) {
 
 __device__ __forceinline__ unsigned attached_step(const uint4& e) { return (e.y >> kStepShift) & 31u; }
 
-// The in-memory path: the sum over a sorted bucket of at most kBucketCap
-// entries, or over the whole in-row after an overflow. words/vals: emission
-// words and block-step-k values of the source population.
+// Sorts a bucket of up to kBucketCap entries in place (nothing to do for
+// fewer than two).
+__device__ __forceinline__ void sort_bucket(uint4* ent, unsigned cnt) {
+    if (cnt < 2u || cnt > kBucketCap) return;
+    if (cnt <= 4u) {
+        uint4 e[4];
+        load_sorted<4>(e, ent, cnt);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (unsigned(q) < cnt) ent[q] = e[q];
+    } else {
+        sort_in_place(ent, cnt);
+    }
+}
+
+// Bits [c0, c1) of a block word.
+__device__ __forceinline__ std::uint32_t step_mask(unsigned c0, unsigned c1) {
+    const std::uint32_t hi = c1 >= 32u ? 0xFFFFFFFFu : ((1u << c1) - 1u);
+    return hi & ~((1u << c0) - 1u);
+}
+
+// Input sums of one target and connection type for block steps [c0, c1),
+// into the lane's LDS column col[(k - c0) * stride], in ascending in-CSR
+// slot order: the bucket is sorted; an attached entry adds its value at its
+// step, an unattached one the source's ring values at its emitted steps, and
+// an overflowed bucket walks the whole in-row. ring(k): the source
+// population's value-ring row of block step k.
+template <class R, class RingF, class RowF>
+__device__ void gather_chunk(R* col, unsigned stride, unsigned c0, unsigned c1, const uint4* ent, unsigned cnt,
+                             const std::uint32_t* words, RingF ring, RowF row, const index_t* src) {
+    for (unsigned k = c0; k < c1; ++k) col[(k - c0) * stride] = R(0);
+    if (cnt == 0u) return;
+    const std::uint32_t mask = step_mask(c0, c1);
+    if (cnt <= kBucketCap) {
+        for (unsigned r = 0; r < cnt; ++r) {
+            const uint4 e = ent[r];
+            if (e.y & kAttached) {
+                const unsigned k = attached_step(e);
+                if ((mask >> k) & 1u) col[(k - c0) * stride] += attached_value<R>(e);
+            } else {
+                const index_t q = e.y & kSrcMask;
+                for (std::uint32_t w = words[q] & mask; w != 0u; w &= w - 1u) {
+                    const unsigned k = __ffs(static_cast<int>(w)) - 1;
+                    col[(k - c0) * stride] += ring(k)[q];
+                }
+            }
+        }
+    } else {
+        const uint2 rw = row();
+        for (index_t e = rw.x; e < rw.y; ++e) {
+            const index_t q = src[e];
+            for (std::uint32_t w = words[q] & mask; w != 0u; w &= w - 1u) {
+                const unsigned k = __ffs(static_cast<int>(w)) - 1;
+                col[(k - c0) * stride] += ring(k)[q];
+            }
+        }
+    }
+}
+
+// The in-memory path of the SIC stream: the sum of block step k over a
+// sorted bucket of at most kBucketCap entries, or the whole in-row.
 template <class R, class RowF>
 __device__ R memory_sum(const uint4* ent, unsigned cnt, unsigned k, const std::uint32_t* words, const R* vals,
                         RowF row, const index_t* src) {
     R sum = R(0);
     if (cnt <= kBucketCap) {
         for (unsigned r = 0; r < cnt; ++r) {
-            const uint4 e = ent[r];
-            if (e.y & kAttached) {
-                if (attached_step(e) == k) sum += attached_value<R>(e);
-            } else {
-                const index_t q = e.y & kSrcMask;
-                if ((words[q] >> k) & 1u) sum += vals[q];
-            }
+            const index_t q = ent[r].y & kSrcMask;
+            if ((words[q] >> k) & 1u) sum += vals[q];
         }
     } else {
         const uint2 rw = row();
@@ -263,61 +318,6 @@ __device__ R memory_sum(const uint4* ent, unsigned cnt, unsigned k, const std::u
             if ((words[q] >> k) & 1u) sum += vals[q];
         }
     }
-    return sum;
-}
-
-// Neuron-sourced inputs: attached (step, value) pairs in slot order.
-template <class R, int NR>
-struct DirectInputs {
-    static_assert(NR <= 4, "steps are packed in bytes");
-    R v[NR];                 // values (zero where empty)
-    std::uint32_t ks;        // byte r: block step of v[r] (0xFF: empty)
-    unsigned cnt;            // bucket entries
-    bool mem;                // the in-memory path
-};
-
-template <class R, int NR>
-__device__ __forceinline__ void direct_init(DirectInputs<R, NR>& s, uint4* ent, unsigned cnt) {
-    s.cnt = cnt;
-    s.mem = cnt > unsigned(NR);
-    s.ks = 0xFFFFFFFFu;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) s.v[r] = R(0);
-    if (cnt == 0u) return;
-    if (cnt <= unsigned(NR)) {
-        uint4 e[NR];
-        load_sorted<NR>(e, ent, cnt);
-        bool all = true;
-#pragma unroll
-        for (int r = 0; r < NR; ++r)
-            if (e[r].x != kNoSlot && !(e[r].y & kAttached)) all = false;
-        if (all) {
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                if (e[r].x != kNoSlot) {
-                    s.ks = (s.ks & ~(0xFFu << (8 * r))) | (attached_step(e[r]) << (8 * r));
-                    s.v[r] = attached_value<R>(e[r]);
-                }
-            }
-        } else {
-            s.mem = true;
-#pragma unroll
-            for (int r = 0; r < NR; ++r)
-                if (unsigned(r) < cnt) ent[r] = e[r];
-        }
-    } else if (cnt <= kBucketCap) {
-        sort_in_place(ent, cnt);
-    }
-}
-
-// ent(): the target's bucket (computed on the in-memory path only).
-template <class R, int NR, class EntF, class RowF>
-__device__ __forceinline__ R direct_sum(const DirectInputs<R, NR>& s, unsigned k, EntF ent,
-                                        const std::uint32_t* words, const R* vals, RowF row, const index_t* src) {
-    R sum = R(0);
-#pragma unroll
-    for (int r = 0; r < NR; ++r) sum += ((s.ks >> (8 * r)) & 0xFFu) == k ? s.v[r] : R(0);
-    if (s.mem) sum = memory_sum<R>(ent(), s.cnt, k, words, vals, row, src);
     return sum;
 }
 
@@ -445,16 +445,56 @@ __global__ void tick_kernel(std::int64_t* clock) {
 
 // One astrocyte (global id ga, local id a) over global steps [lo, hi)
 // (src/astrosim_fused.cu:374-427). Returns whether it joins the emitter list.
+// Both cell functions advance one cell over global steps [lo, hi), at most
+// B.chunk of them, in two phases. The prologue computes everything that does
+// not depend on the cell's own state into the lane's LDS column col[i *
+// stride]: the delayed input sums (in slot order), the Poisson drive folded
+// in with the reference's operation order, and the noise draws. The step
+// loop then reads those and runs the arithmetic that does; keeping the
+// prologue's random-number and gather code out of that loop keeps its
+// constants and temporaries out of the loop's registers.
+
+// One astrocyte (global id ga, local id a) (src/astrosim_fused.cu:374-427).
+// Returns whether it joins the emitter list. col: [0, C) the IP3 input of
+// each step, [C, 2C) its noise term.
 template <class R>
 __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
                            const Member& M, const BlkLaunch& B, index_t ga, index_t a, std::int64_t lo,
-                           std::int64_t hi) {
+                           std::int64_t hi, R* col, unsigned stride) {
     const std::uint32_t oa = ga * std::uint32_t(sizeof(R)), oa4 = ga * 4u;
-    const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
-    auto ent = [&] { return D.bent[T_E2A] + std::size_t(ga) * kBucketCap; };
-    DirectInputs<R, 4> in;
-    direct_init(in, ent(), at(D.bcnt[T_E2A], oa4));
-    std::uint32_t vr = ring_row(lo - B.d, B.Rv), pr = ring_row(lo, B.Rv);
+    const unsigned C = static_cast<unsigned>(B.chunk);
+    {
+        const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
+        uint4* ent = D.bent[T_E2A] + std::size_t(ga) * kBucketCap;
+        const unsigned cnt = at(D.bcnt[T_E2A], oa4);
+        sort_bucket(ent, cnt);
+        auto nring = [&](unsigned k) { return S.spk_v + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.ncap; };
+        gather_chunk<R>(col, stride, static_cast<unsigned>(lo - B.bstart), static_cast<unsigned>(hi - B.bstart), ent,
+                        cnt, nwords, nring, [&] { return D.in_rs[T_E2A][ga]; }, D.in_src[T_E2A]);
+        // random streams of the member's step tl = t - step0 and cell a
+        std::uint64_t tl_stride = std::uint64_t(lo - M.step0) * kStepStride;  // wave-uniform
+        for (std::int64_t t = lo; t < hi; ++t) {
+            const unsigned j = static_cast<unsigned>(t - lo);
+            const std::uint64_t stream = tl_stride + a;
+            int ev = 0;
+            if (P.poiss_astro) ev = rng_poisson(M.seed ^ kKeyAstroPoisson, stream, P.lam_astro, P.lim_astro);
+            const double noise =
+                P.noise_astro > 0.0 ? P.noise_astro * rng_normal(M.seed ^ kKeyAstroNoise, stream) : 0.0;
+            R& in = col[j * stride];
+            if constexpr (sizeof(R) == 8) {
+                double ip3_input = P.w_n2a * in;
+                if (ev > 0) ip3_input += P.w_poiss_astro * static_cast<double>(ev);
+                in = ip3_input;
+            } else {
+                float ip3_input = Q.w_n2a * in;
+                if (ev > 0) ip3_input += Q.w_poiss_astro * static_cast<float>(ev);
+                in = ip3_input;
+            }
+            col[(C + j) * stride] = static_cast<R>(noise);
+            tl_stride += kStepStride;
+        }
+    }
+    std::uint32_t pr = ring_row(lo, B.Rv);
     R ca = at(S.Ca, oa), ip3 = at(S.IP3, oa), hh = at(S.h, oa);
     R ca_lo = R(0), ip3_lo = R(0), hh_lo = R(0);
     if constexpr (sizeof(R) == 4) {
@@ -465,27 +505,16 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
     const R Ca_tot = at(S.Ca_tot, oa), IP3_0 = at(S.IP3_0, oa), tau = at(S.tau_IP3, oa), dIP3 = at(S.delta_IP3, oa);
     std::uint32_t wout = B.first ? 0u : at(D.sic_w + std::size_t(B.wslot) * D.acap, oa4);
     bool ever = false;
-    // random streams of the member's step tl = t - step0 and cell a
-    std::uint64_t tl_stride = std::uint64_t(lo - M.step0) * kStepStride;  // wave-uniform
     for (std::int64_t t = lo; t < hi; ++t) {
-        const unsigned k = static_cast<unsigned>(t - B.bstart);
-        const R* vals = S.spk_v + std::size_t(vr) * D.ncap;
-        const R gathered = direct_sum(in, k, ent, nwords, vals, [&] { return D.in_rs[T_E2A][ga]; }, D.in_src[T_E2A]);
-        const std::uint64_t stream = tl_stride + a;
-        int ev = 0;
-        if (P.poiss_astro) ev = rng_poisson(M.seed ^ kKeyAstroPoisson, stream, P.lam_astro, P.lim_astro);
-        const double noise = P.noise_astro > 0.0 ? P.noise_astro * rng_normal(M.seed ^ kKeyAstroNoise, stream) : 0.0;
+        const unsigned k = static_cast<unsigned>(t - B.bstart), j = static_cast<unsigned>(t - lo);
+        const R ip3_input = col[j * stride], noise = col[(C + j) * stride];
         R factor;
         if constexpr (sizeof(R) == 8) {
-            double ip3_input = P.w_n2a * gathered;
-            if (ev > 0) ip3_input += P.w_poiss_astro * static_cast<double>(ev);
             astro_advance(P.ac, Ca_tot, IP3_0, tau, dIP3, ip3_input, noise, P.h_step, P.substeps, ca, ip3, hh);
             factor = sic_factor(P.ac, ca);
         } else {
-            float ip3_input = Q.w_n2a * gathered;
-            if (ev > 0) ip3_input += Q.w_poiss_astro * static_cast<float>(ev);
-            astro_advance32(Q.ac, Q, Ca_tot, IP3_0, tau, dIP3, ip3_input, static_cast<float>(noise), P.substeps, ca,
-                            ca_lo, ip3, ip3_lo, hh, hh_lo);
+            astro_advance32(Q.ac, Q, Ca_tot, IP3_0, tau, dIP3, ip3_input, noise, P.substeps, ca, ca_lo, ip3, ip3_lo,
+                            hh, hh_lo);
             factor = sic_factor32(Q.ac, ca, ca_lo);
         }
         if (factor > R(0)) {
@@ -493,9 +522,7 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
             wout |= 1u << k;
             at(S.sic_v + std::size_t(pr) * D.acap, oa) = factor;
         }
-        vr = ring_next(vr, B.Rv);
         pr = ring_next(pr, B.Rv);
-        tl_stride += kStepStride;
     }
     at(S.Ca, oa) = ca;
     at(S.IP3, oa) = ip3;
@@ -511,80 +538,128 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
     return B.last && wout != 0u;
 }
 
-// One neuron (global id gj, local id j) over global steps [lo, hi)
-// (src/astrosim_fused.cu:433-587). Returns the number of spikes.
+// One neuron (global id gj, local id j) (src/astrosim_fused.cu:433-587).
+// Returns the number of spikes. col: [0, C) the excitatory input of each
+// step, [C, 2C) the inhibitory input, FP32c only: [2C, 3C) the SIC current
+// of each step, then the noise values of the launch's noise indices.
 template <class R, bool exc>
 __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
                                 const Member& M, const BlkLaunch& B, index_t gj, index_t j, std::int64_t lo,
-                                std::int64_t hi, bool* emit) {
+                                std::int64_t hi, R* col, unsigned stride, bool* emit) {
     const std::uint32_t on = gj * std::uint32_t(sizeof(R)), o4 = gj * 4u;
-    const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
+    const unsigned C = static_cast<unsigned>(B.chunk);
+    constexpr bool kSicInLds = sizeof(R) == 4;
+    const unsigned noise_base = (kSicInLds ? 3u : 2u) * C;
     const std::uint32_t* awords = D.sic_w + std::size_t(B.rslot) * D.acap;
-    auto ee = [&] { return D.bent[T_EXC] + std::size_t(gj) * kBucketCap; };
-    auto ei = [&] { return D.bent[T_INH] + std::size_t(gj) * kBucketCap; };
     auto es = [&] { return D.bent[T_SIC] + std::size_t(gj) * kBucketCap; };
-    // inhibitory source ids are local indices used as neuron ids
-    // (src/astrosim_fused.cu:459-462); the scatter has already resolved them
-    DirectInputs<R, 4> ie;
-    DirectInputs<R, 2> ii;
+    auto sring = [&](unsigned k) { return S.sic_v + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.acap; };
+    const double nstd = exc ? P.noise_exc : P.noise_inh;
+    const std::int64_t tl0 = lo - M.step0;  // the member's own step at lo
+    const std::uint64_t nidx0 = static_cast<std::uint64_t>(tl0 / P.steps_per_noise);
+    {
+        const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
+        uint4* ee = D.bent[T_EXC] + std::size_t(gj) * kBucketCap;
+        uint4* ei = D.bent[T_INH] + std::size_t(gj) * kBucketCap;
+        const unsigned ce = at(D.bcnt[T_EXC], o4), ci = at(D.bcnt[T_INH], o4);
+        sort_bucket(ee, ce);
+        sort_bucket(ei, ci);
+        auto nring = [&](unsigned k) { return S.spk_v + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.ncap; };
+        const unsigned k0 = static_cast<unsigned>(lo - B.bstart), k1 = static_cast<unsigned>(hi - B.bstart);
+        // inhibitory source ids are local indices used as neuron ids
+        // (src/astrosim_fused.cu:459-462); the scatter has already resolved them
+        gather_chunk<R>(col, stride, k0, k1, ee, ce, nwords, nring, [&] { return D.in_rs[T_EXC][gj]; },
+                        D.in_src[T_EXC]);
+        gather_chunk<R>(col + C * stride, stride, k0, k1, ei, ci, nwords, nring, [&] { return D.in_rs[T_INH][gj]; },
+                        D.in_src[T_INH]);
+        if constexpr (kSicInLds) {
+            const unsigned cs = at(D.bcnt[T_SIC], o4);
+            sort_bucket(es(), cs);
+            gather_chunk<R>(col + 2 * C * stride, stride, k0, k1, es(), cs, awords, sring,
+                            [&] { return D.in_rs[T_SIC][gj]; }, D.in_src[T_SIC]);
+        }
+        // the Poisson stream tl*stride + j of each step, folded into the inputs
+        std::uint64_t tl_stride = std::uint64_t(tl0) * kStepStride;  // wave-uniform
+        for (std::int64_t t = lo; t < hi; ++t) {
+            const unsigned i = static_cast<unsigned>(t - lo);
+            int ev = 0;
+            if (exc ? P.poiss_exc : P.poiss_inh)
+                ev = rng_poisson(M.seed ^ kKeyNeuronPoisson, tl_stride + j, exc ? P.lam_exc : P.lam_inh,
+                                 exc ? P.lim_exc : P.lim_inh);
+            R& ein = col[i * stride];
+            R& iin = col[(C + i) * stride];
+            if constexpr (sizeof(R) == 8) {
+                double exc_input = P.w_e * ein;
+                if (ev > 0) exc_input += (exc ? P.w_poiss_exc : P.w_poiss_inh) * static_cast<double>(ev);
+                ein = exc_input;
+                iin = (-P.w_i) * iin;
+            } else {
+                float exc_input = Q.w_e_abs * ein;
+                if (ev > 0) exc_input += (exc ? Q.w_poiss_exc : Q.w_poiss_inh) * static_cast<float>(ev);
+                ein = exc_input;
+                iin = Q.w_i_abs * iin;
+                R& sin_ = col[(2 * C + i) * stride];
+                sin_ = Q.w_a2n * sin_;
+            }
+            tl_stride += kStepStride;
+        }
+        // the noise values of the launch's noise indices tl / steps_per_noise
+        if (nstd > 0.0) {
+            const std::uint64_t nlast = static_cast<std::uint64_t>((tl0 + (hi - lo) - 1) / P.steps_per_noise);
+            for (std::uint64_t n = nidx0; n <= nlast; ++n)
+                col[(noise_base + static_cast<unsigned>(n - nidx0)) * stride] =
+                    static_cast<R>(nstd * rng_normal(M.seed ^ kKeyNeuronNoise, n * kNoiseStride + std::uint64_t(j)));
+        }
+    }
+    // FP64: SIC inputs from (source, word) pairs, values loaded a step ahead
     InputStream<R, 1> is;
-    direct_init(ie, ee(), at(D.bcnt[T_EXC], o4));
-    direct_init(ii, ei(), at(D.bcnt[T_INH], o4));
-    input_init(is, es(), at(D.bcnt[T_SIC], o4), awords);
-    std::uint32_t vr = ring_row(lo - B.d, B.Rv), pr = ring_row(lo, B.Rv);
-    input_prefetch(is, static_cast<unsigned>(lo - B.bstart), S.sic_v + std::size_t(vr) * D.acap);
+    std::uint32_t vr = ring_row(lo - B.d, B.Rv);
+    if constexpr (!kSicInLds) {
+        input_init(is, es(), at(D.bcnt[T_SIC], o4), awords);
+        input_prefetch(is, static_cast<unsigned>(lo - B.bstart), S.sic_v + std::size_t(vr) * D.acap);
+    }
+    std::uint32_t pr = ring_row(lo, B.Rv);
     R V = at(S.V, on), w = at(S.w, on), g_ex = at(S.g_ex, on), dg_ex = at(S.dg_ex, on), g_in = at(S.g_in, on),
       dg_in = at(S.dg_in, on);
     int refr = at(D.refr, o4);
     R I_sic = at(S.I_sic, on);
     const R V_reset = at(S.V_reset, on), bb = at(S.b, on);
-    const double nstd = exc ? P.noise_exc : P.noise_inh;
     R noise_val = R(0);
     unsigned nspikes = 0;
     std::uint32_t wout = B.first ? 0u : at(D.spk_w + std::size_t(B.wslot) * D.ncap, o4);
-    // Counters of the member's step tl = t - step0, advanced per step:
-    // the Poisson stream tl*stride + j, the SIC-update phase (an update at
-    // tl >= d_a2n with (tl - d_a2n) % sic_interval == 0), and the noise
-    // index tl / steps_per_noise with its remainder.
-    const std::int64_t tl0 = lo - M.step0;
-    std::uint64_t tl_stride = std::uint64_t(tl0) * kStepStride;  // wave-uniform
+    // The SIC-update phase (an update at tl >= d_a2n with (tl - d_a2n) %
+    // sic_interval == 0) and the noise index with its remainder, advanced
+    // per step.
     int sic_wait;
     {
         const std::int64_t u = tl0 - P.d_a2n;
         sic_wait = u < 0 ? static_cast<int>(-u) : static_cast<int>((P.sic_interval - u % P.sic_interval) % P.sic_interval);
     }
-    std::uint64_t nidx = static_cast<std::uint64_t>(tl0 / P.steps_per_noise);
+    unsigned nslot = 0;
     int nrem = static_cast<int>(tl0 % P.steps_per_noise);
     for (std::int64_t t = lo; t < hi; ++t) {
-        const unsigned k = static_cast<unsigned>(t - B.bstart);
+        const unsigned k = static_cast<unsigned>(t - B.bstart), i = static_cast<unsigned>(t - lo);
         const std::int64_t tl = t - M.step0;  // the member's own step
-        const R* nvals = S.spk_v + std::size_t(vr) * D.ncap;
-        const R exc_sum =
-            direct_sum(ie, k, ee, nwords, nvals, [&] { return D.in_rs[T_EXC][gj]; }, D.in_src[T_EXC]);
-        const R inh_sum =
-            direct_sum(ii, k, ei, nwords, nvals, [&] { return D.in_rs[T_INH][gj]; }, D.in_src[T_INH]);
+        const R exc_input = col[i * stride];
+        const R inh_input = col[(C + i) * stride];
         const bool sic_step = sic_wait == 0;
         sic_wait = sic_step ? P.sic_interval - 1 : sic_wait - 1;
-        if (sic_step) {
-            const R sic_sum = input_sum(is, k, es, awords, S.sic_v + std::size_t(vr) * D.acap,
-                                        [&] { return D.in_rs[T_SIC][gj]; }, D.in_src[T_SIC]);
-            if constexpr (sizeof(R) == 8) I_sic = P.w_a2n * sic_sum;
-            else I_sic = Q.w_a2n * sic_sum;
+        if constexpr (kSicInLds) {
+            if (sic_step) I_sic = col[(2 * C + i) * stride];
+        } else {
+            if (sic_step) {
+                const R sic_sum = input_sum(is, k, es, awords, S.sic_v + std::size_t(vr) * D.acap,
+                                            [&] { return D.in_rs[T_SIC][gj]; }, D.in_src[T_SIC]);
+                I_sic = P.w_a2n * sic_sum;
+            }
+            const std::uint32_t vn = ring_next(vr, B.Rv);
+            if (t + 1 < hi) input_prefetch(is, k + 1u, S.sic_v + std::size_t(vn) * D.acap);
+            vr = vn;
         }
-        const std::uint32_t vn = ring_next(vr, B.Rv);
-        if (t + 1 < hi) input_prefetch(is, k + 1u, S.sic_v + std::size_t(vn) * D.acap);
-        int ev = 0;
-        if (exc ? P.poiss_exc : P.poiss_inh)
-            ev = rng_poisson(M.seed ^ kKeyNeuronPoisson, tl_stride + j, exc ? P.lam_exc : P.lam_inh,
-                             exc ? P.lim_exc : P.lim_inh);
         if (nstd > 0.0 && (t == lo || nrem == 0))  // the noise stream changes every steps_per_noise steps
-            noise_val = static_cast<R>(nstd * rng_normal(M.seed ^ kKeyNeuronNoise, nidx * kNoiseStride + std::uint64_t(j)));
+            noise_val = col[(noise_base + nslot) * stride];
         bool spiked;
         if constexpr (sizeof(R) == 8) {
             const NeuronConst& p = exc ? P.ne : P.ni;
-            double exc_input = P.w_e * exc_sum;
-            const double inh_input = (-P.w_i) * inh_sum;
-            if (ev > 0) exc_input += (exc ? P.w_poiss_exc : P.w_poiss_inh) * static_cast<double>(ev);
             double I_ext = I_sic;
             if (nstd > 0.0) I_ext += noise_val;
             NeuronRegs r{V, w, g_ex, dg_ex + exc_input * p.psc_init_ex, g_in, dg_in + inh_input * p.psc_init_in, refr};
@@ -593,9 +668,6 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
             V = r.V; w = r.w; g_ex = r.g_ex; dg_ex = r.dg_ex; g_in = r.g_in; dg_in = r.dg_in; refr = r.refractory;
         } else {
             const NeuronConst32& p = exc ? Q.ne : Q.ni;
-            float exc_input = Q.w_e_abs * exc_sum;
-            const float inh_input = Q.w_i_abs * inh_sum;
-            if (ev > 0) exc_input += (exc ? Q.w_poiss_exc : Q.w_poiss_inh) * static_cast<float>(ev);
             const float I_ext = I_sic + noise_val;
             NeuronRegs32 r{V, w, g_ex, dg_ex + exc_input * p.psc_init_ex, g_in, dg_in + inh_input * p.psc_init_in, refr};
             spiked = neuron_substeps32(p, Q, r, I_ext, V_reset, bb, P.substeps, exc ? Q.ex_decay_e : Q.ex_decay_i,
@@ -629,12 +701,10 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
             wout |= 1u << k;
             ++nspikes;
         }
-        vr = vn;
         pr = ring_next(pr, B.Rv);
-        tl_stride += kStepStride;
         if (++nrem == P.steps_per_noise) {
             nrem = 0;
-            ++nidx;
+            ++nslot;
         }
     }
     at(S.V, on) = V;
@@ -680,6 +750,9 @@ constexpr int kIntegrateWaves = sizeof(R) == 8 ? AS_INTEGRATE_WAVES_F64 : AS_INT
 
 template <int MAXT, class R>
 __global__ __launch_bounds__(MAXT, kIntegrateWaves<R>) void integrate_kernel(const IntegrateCtx<R>* __restrict__ ctx, BlkLaunch B) {
+    // each lane's column of staged input sums: [2 types][B.chunk][blockDim.x]
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    R* col = reinterpret_cast<R*>(lds_raw) + threadIdx.x;
     const BatchDev& D = ctx->D;
     const BlkState<R>& S = ctx->S;
     const RunParams& P = ctx->P;
@@ -694,7 +767,7 @@ __global__ __launch_bounds__(MAXT, kIntegrateWaves<R>) void integrate_kernel(con
     const index_t local = tile.first + threadIdx.x;
     if (tile.kind == 0) {
         const bool run = local < tile.end && lo < hi;
-        const bool emit = run ? astro_cell<R>(D, S, P, Q, M, B, M.ab + local, local, lo, hi) : false;
+        const bool emit = run ? astro_cell<R>(D, S, P, Q, M, B, M.ab + local, local, lo, hi, col, blockDim.x) : false;
         wave_append(emit, M.ab + local, D.alist + std::size_t(B.wslot) * D.acap, &D.lcount[B.wslot * 2 + 1]);
         return;
     }
@@ -704,9 +777,9 @@ __global__ __launch_bounds__(MAXT, kIntegrateWaves<R>) void integrate_kernel(con
     if (run) {
         // one instantiation per population keeps only its constants live
         if (tile.kind == 1)
-            nspikes = neuron_cell<R, true>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, &emit);
+            nspikes = neuron_cell<R, true>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, col, blockDim.x, &emit);
         else
-            nspikes = neuron_cell<R, false>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, &emit);
+            nspikes = neuron_cell<R, false>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, col, blockDim.x, &emit);
     }
     wave_append(emit, M.nb + local, D.nlist + std::size_t(B.wslot) * D.ncap, &D.lcount[B.wslot * 2 + 0]);
     // diagnostic spike total of the member: one atomic per wave with spikes
@@ -1036,7 +1109,7 @@ public:
             const std::int64_t bidx = step_ / geo_.L;
             BlkLaunch B = launch_for(bidx);
             B.t0 = step_;
-            B.n = static_cast<int>(std::min<std::int64_t>(k, B.bstart + geo_.L - step_));
+            B.n = static_cast<int>(std::min<std::int64_t>({k, B.bstart + geo_.L - step_, std::int64_t(B.chunk)}));
             B.first = step_ == B.bstart;
             B.last = step_ + B.n == B.bstart + geo_.L;
             BatchDev D = D_;
@@ -1250,16 +1323,33 @@ private:
         B.rslot = static_cast<std::uint32_t>(((bidx - geo_.d / geo_.L) % geo_.H + geo_.H) % geo_.H);
         B.first = 1;
         B.last = 1;
+        B.chunk = chunk_len();
         return B;
+    }
+
+    // LDS words (of R) per lane that an integrate launch of c steps stages:
+    // excitatory and inhibitory inputs (FP32c also SIC currents) per step and
+    // the noise values of up to (c-1)/steps_per_noise + 2 noise indices.
+    int lds_words(int c) const {
+        return (sizeof(R) == 4 ? 3 : 2) * c + (c - 1) / P_.steps_per_noise + 2;
+    }
+    // Most steps per integrate launch: the staged words fit 208 bytes per
+    // lane (3 resident waves per SIMD share 160 KB) and 64 KB per workgroup.
+    int chunk_len() const {
+        const int per_lane = std::min(208, 65536 / threads_);
+        int c = geo_.L;
+        while (c > 1 && lds_words(c) * int(sizeof(R)) > per_lane) --c;
+        return c;
     }
 
     void launch_integrate(const BatchDev& D, const BlkLaunch& B) {
         if (n_tiles_ == 0) return;
         const IntegrateCtx<R>* ctx = D.clock ? ctx_graph_ : ctx_eager_;
+        const std::size_t lds = std::size_t(lds_words(B.chunk)) * std::size_t(threads_) * sizeof(R);
         if (threads_ <= 256)
-            hipLaunchKernelGGL((integrate_kernel<256, R>), dim3(n_tiles_), dim3(threads_), 0, stream_, ctx, B);
+            hipLaunchKernelGGL((integrate_kernel<256, R>), dim3(n_tiles_), dim3(threads_), lds, stream_, ctx, B);
         else
-            hipLaunchKernelGGL((integrate_kernel<1024, R>), dim3(n_tiles_), dim3(threads_), 0, stream_, ctx, B);
+            hipLaunchKernelGGL((integrate_kernel<1024, R>), dim3(n_tiles_), dim3(threads_), lds, stream_, ctx, B);
     }
 
     void timing_begin() {
@@ -1336,7 +1426,11 @@ private:
         AS_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         for (std::int64_t i = 0; i < blocks; ++i) {
             hipLaunchKernelGGL(scatter_kernel<R>, dim3(kScatterBlocks), dim3(256), 0, stream_, D, B, S_.spk_v);
-            launch_integrate(D, B);
+            for (int sub = 0; sub < geo_.L; sub += B.chunk) {
+                BlkLaunch Bs = B;
+                Bs.sub = sub;
+                launch_integrate(D, Bs);
+            }
             hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(64), 0, stream_, clock_);
         }
         hipGraph_t g = nullptr;

@@ -77,10 +77,10 @@ def kernel_bytes_per_launch(net, prec: str, method: str) -> float:
     return float(net.n_neurons * neuron + net.n_astro * astro)
 
 
-def run_leg(A, net, prec: str, method: str, steps: int, warmup: int) -> dict:
+def run_leg(A, net, prec: str, method: str, steps: int, warmup: int, block: int = 128) -> dict:
     log(f"leg {prec}/{method}: create")
     t0 = time.perf_counter()
-    sim = A.Simulation(net, precision=prec, method=method)
+    sim = A.Simulation(net, precision=prec, method=method, block=block)
     sim.sync()
     create_s = time.perf_counter() - t0  # host-to-device upload of the network + device setup
     sim.step(warmup)
@@ -94,7 +94,7 @@ def run_leg(A, net, prec: str, method: str, steps: int, warmup: int) -> dict:
     sim.set_timing(False)
     spikes = sim.spike_count()
     out = {
-        "precision": prec, "method": method, "steps": steps, "wall_s": wall,
+        "precision": prec, "method": method, "block": block, "steps": steps, "wall_s": wall,
         "steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
         "kernel_ms_total": kms, "launches": launches,
         "kernel_ms_per_launch": kms / max(1, launches),
@@ -238,6 +238,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--legs", default="fp64:plain,fp64:blocked,fp32c:blocked")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--block", type=int, default=128, help="threads per workgroup (the CLI's --block)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=10.0,
@@ -261,7 +262,7 @@ def main() -> int:
         f"edges {net.n_edges}")
 
     legs = [tuple(x.split(":")) for x in args.legs.split(",")]
-    results = [run_leg(A, net, p, m, args.steps, args.warmup) for p, m in legs]
+    results = [run_leg(A, net, p, m, args.steps, args.warmup, args.block) for p, m in legs]
     head = results[-1]
     workload = f"n{args.n}_seed{args.seed}"
     spl = head["steps_per_launch"]
