@@ -4,7 +4,8 @@
 // within an aligned block of L steps (L divides d, L <= 32) no cell reads
 // anything written in the same block: all inputs of block b come from block
 // b - d/L. One INTEGRATE launch therefore advances every cell of every
-// member network a whole block with its state in registers.
+// member network through a block (or a chunk of it) with its state in
+// registers.
 //
 // Inputs are delivered event-driven instead of by probing every in-edge:
 //   * each cell records the steps of its block at which it emitted (a spike,
@@ -12,13 +13,16 @@
 //     indexed by absolute step, and appends itself to its block's emitter
 //     list when the word is nonzero;
 //   * a SCATTER launch walks only the emitters' out-edges (a transposed copy
-//     of the in-CSR built on the device) and appends (in-CSR slot, source)
-//     to the target's fixed-capacity bucket for that connection type;
-//   * INTEGRATE sums a target's bucket in ascending in-CSR slot order, which
-//     is the reference's CSR-order sum with its exact-zero terms removed
-//     (src/astrosim_fused.cu:384-389, 450-463, 482-490), so FP64 results are
-//     bit-identical to the plain method and to the host oracle. A bucket that
-//     overflowed falls back to walking the whole in-row.
+//     of the in-CSR built on the device) and appends an entry (in-CSR slot,
+//     source, and for a source that emitted once its step and value) to the
+//     target's fixed-capacity bucket for that connection type;
+//   * INTEGRATE sorts each bucket by in-CSR slot and sums it per step in that
+//     order, which is the reference's CSR-order sum with its exact-zero terms
+//     removed (src/astrosim_fused.cu:384-389, 450-463, 482-490), so FP64
+//     results are bit-identical to the plain method and to the host oracle.
+//     A bucket that overflowed falls back to walking the whole in-row. The
+//     per-step sums, the Poisson drive and the noise draws are staged in LDS
+//     by a prologue, so the RK4 step loop carries only the cell's state.
 //
 // Members occupy contiguous ranges of the arena's cell and edge spaces with
 // rebased (global) ids. Workgroups are tiles of one member's astrocytes or

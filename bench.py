@@ -296,7 +296,7 @@ def main() -> int:
         "value": head["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": head["steps"],
         "warmup": args.warmup,
         "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
@@ -321,10 +321,12 @@ def main() -> int:
                      "measured_gbs": None if traffic is None else traffic / launch_s / 1e9,
                      "measured_frac": None if traffic is None else traffic / launch_s / 1e9 / HBM_PEAK_GBS,
                      "kernel_state_bytes_per_launch": own_bpl,
-                     "note": "frac > 1: the blocked, event-driven kernel does not stream the CSR every "
-                             "step, so it exceeds the reference layout's HBM ceiling; its own HBM "
-                             "traffic (measured_gbs) is far below peak because it is bound by "
-                             "dependent vector-ALU latency"},
+                     "note": ("frac > 1: the blocked, event-driven kernel does not stream the CSR every "
+                              "step, so it exceeds the reference layout's HBM ceiling; its own HBM "
+                              "traffic (measured_gbs) is far below peak because it is bound by "
+                              "vector-instruction throughput") if head["method"] == "blocked" else
+                             ("the plain kernel walks ~116 dependent id->value loads per lane and "
+                              "step: latency-bound, far from the HBM ceiling")},
         "cpu_baseline": cpu,
         "ensemble": ens,
         "serving": srv,

@@ -91,6 +91,28 @@ def test_graph_replays_match_eager(nets):
     e.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_chunked_graph_replays_match(nets, precision):
+    # 1024-thread tiles stage fewer steps per launch in LDS, so each block
+    # is several integrate launches (also inside the captured graph); the
+    # results do not depend on that split.
+    print(f"[batch] chunked graph replay {precision}", flush=True)
+    g = A.Batch(nets[:3], precision=precision, block=1024)
+    e = A.Batch(nets[:3], precision=precision)
+    gm = [g.add(n, 10**6) for n in nets[:3]]
+    em = [e.add(n, 10**6) for n in nets[:3]]
+    for rep in range(3):
+        g.run_graph(5)
+        e.step(50)
+        g.sync()
+        for a, b_ in zip(gm, em):
+            for f in FIELDS:
+                assert np.array_equal(g.state(a, f).view(np.uint64), e.state(b_, f).view(np.uint64)), (rep, f)
+            assert g.spike_count(a) == e.spike_count(b_), rep
+    g.close()
+    e.close()
+
+
 def test_autotune_keeps_results(nets):
     print("[batch] autotune", flush=True)
     b = A.Batch(nets[:3])
