@@ -96,6 +96,11 @@ def run_leg(A, net, prec: str, method: str, steps: int, warmup: int, block: int 
     kms, launches = sim.kernel_time()
     sim.set_timing(False)
     spikes = sim.spike_count()
+    # SIC fractions as the reference's summary reports them (src/astrosim_fused.cu:
+    # 1117-1125): Ca above threshold at the final step, and ever above it
+    ca = sim.state("Ca")
+    sic_active = float(((ca - 0.19669) * 1000.0 > 1.0).mean())
+    sic_engaged = float(sim.ever_active().astype(bool).mean())
     out = {
         "precision": prec, "method": method, "block": block, "steps": steps, "wall_s": wall,
         "steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
@@ -103,6 +108,7 @@ def run_leg(A, net, prec: str, method: str, steps: int, warmup: int, block: int 
         "kernel_ms_per_launch": kms / max(1, launches),
         "steps_per_launch": steps / max(1, launches),
         "device_bytes": sim.device_bytes(), "spikes_total": spikes,
+        "sic_active_fraction": sic_active, "sic_engaged_fraction": sic_engaged,
         "create_s": create_s,
     }
     sim.close()
@@ -282,6 +288,18 @@ def main() -> int:
         b = reference_bytes_per_step(net, leg["precision"]) * leg["steps_per_launch"]
         leg["reference_layout_gbs"] = b / (leg["kernel_ms_per_launch"] * 1e-3) / 1e9
         leg["hbm_bytes_per_launch"] = load_traffic(f"{leg['method']}_{leg['precision']}_{workload}")
+    # FP32c against FP64 over the same timed window of the same network
+    # (the FP64 blocked leg is bit-identical to the FP64 oracle)
+    accuracy = None
+    by = {(r["precision"], r["method"]): r for r in results}
+    a64, a32 = by.get(("fp64", "blocked")) or by.get(("fp64", "plain")), by.get(("fp32c", "blocked"))
+    if a64 and a32 and a64["steps"] == a32["steps"] and a64["spikes_total"] > 0:
+        accuracy = {"rate_rel": a32["spikes_total"] / a64["spikes_total"] - 1.0, "rate_tolerance": 0.01,
+                    "sic_active_diff": a32["sic_active_fraction"] - a64["sic_active_fraction"],
+                    "sic_engaged_diff": a32["sic_engaged_fraction"] - a64["sic_engaged_fraction"],
+                    "sic_tolerance": 0.01, "window_steps": a32["steps"] + args.warmup}
+        log(f"fp32c vs fp64: rate {100 * accuracy['rate_rel']:+.4f}%, SIC active "
+            f"{accuracy['sic_active_diff']:+.5f}, engaged {accuracy['sic_engaged_diff']:+.5f}")
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: FP64 host oracle")
@@ -328,6 +346,7 @@ def main() -> int:
                              ("the plain kernel walks ~116 dependent id->value loads per lane and "
                               "step: latency-bound, far from the HBM ceiling")},
         "cpu_baseline": cpu,
+        "fp32c_vs_fp64": accuracy,
         "ensemble": ens,
         "serving": srv,
     }
