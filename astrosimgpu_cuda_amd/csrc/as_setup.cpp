@@ -2,6 +2,7 @@
 #include "as_setup.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -238,6 +239,13 @@ private:
     std::future<void> pending_;
 };
 
+// Window-fill threads beside the sequential walk: leave the walk a hardware
+// thread of its own when the build would otherwise occupy all of them.
+unsigned fill_threads(unsigned threads) {
+    const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
+    return std::max(1u, std::min(threads, hc > 1 ? hc - 1 : 1u));
+}
+
 // Sequential draws on the setup stream, decoded from SkipWindows (or computed
 // directly when p is 0 or 1). Mirrors HostRng::uniform and geometric_skip
 // (src/astrosim_fused.cu:183, 617-624) draw for draw.
@@ -249,7 +257,8 @@ public:
         std::uint64_t window = std::uint64_t(1) << 16;
         while (window < (std::uint64_t(1) << 22) && window * 8 < expected_draws) window <<= 1;
         if (p > 0.0 && p < 1.0)
-            skips_.reset(new SkipWindows(seed, kSetupStream, std::log1p(-p), take, counter, threads, window));
+            skips_.reset(new SkipWindows(seed, kSetupStream, std::log1p(-p), take, counter, fill_threads(threads),
+                                         window));
     }
     index_t skip(index_t remaining) {
         if (remaining == 0 || p_ >= 1.0) return 0;
@@ -281,58 +290,73 @@ private:
 // target-major CSR, as the reference's counting sort (count, prefix,
 // scatter in generation order; src/astrosim_fused.cu:307-314), in O(edges)
 // work over the threads. `visit(lo, hi, emit)` calls emit(src, dst) for the
-// sequence positions [lo, hi) in order. The sequence is cut into one
-// contiguous chunk per thread; each thread counts its chunk's targets, the
-// per-(chunk, target) counts become write cursors (chunks in order inside
-// each row), and each thread scatters its chunk in order, so every row lists
-// its sources in generation order.
+// sequence positions [lo, hi) in order.
+//
+// Two streaming levels keep the writes cache-friendly: the sequence, cut into
+// one contiguous chunk per thread, is first split by target range (a few
+// thousand targets per range) into per-range lists, chunks in order; then
+// each range is counting-sorted on its own, with its counters and its rows of
+// the output in cache. Both levels keep generation order, so every row lists
+// its sources exactly as the reference's counting sort does.
 template <class Visit>
 void invert_stable(index_t n_targets, std::size_t n_edges, unsigned threads, Visit&& visit, HostCSR& out) {
     out.row_start.assign(std::size_t(n_targets) + 1, 0);
-    out.src.assign(n_edges, 0);
-    if (n_targets == 0) return;
-    // at most 2^28 counters (1 GiB), and chunks of at least 2^16 edges
-    const std::uint64_t by_mem = std::max<std::uint64_t>(1, (std::uint64_t(1) << 28) / n_targets);
-    const std::uint64_t by_work = std::max<std::uint64_t>(1, n_edges >> 16);
-    const unsigned T = static_cast<unsigned>(std::max<std::uint64_t>(1, std::min({std::uint64_t(threads), by_mem, by_work})));
-    const std::size_t n = n_targets;
-    std::vector<index_t> cnt(std::size_t(T) * n, 0);  // [chunk][target]
+    out.src.resize(n_edges);  // every element is written below
+    if (n_targets == 0 || n_edges == 0) return;
+    // ranges of 2^shift targets, at most 4096 of them
+    int shift = 12;
+    while ((std::uint64_t(n_targets) >> shift) >= 4096) ++shift;
+    const std::size_t R = (std::size_t(n_targets) + (std::size_t(1) << shift) - 1) >> shift;
+    const unsigned T = static_cast<unsigned>(
+        std::max<std::uint64_t>(1, std::min<std::uint64_t>(threads, std::max<std::uint64_t>(1, n_edges >> 16))));
     auto chunk_lo = [&](unsigned t) { return static_cast<std::size_t>(std::uint64_t(n_edges) * t / T); };
+    // level 1: per (chunk, range) counts, then offsets with chunks in order inside each range
+    std::vector<std::uint64_t> off(std::size_t(T) * R + 1, 0);  // [range][chunk]
     parallel_for_threads(T, [&](unsigned t) {
-        index_t* c = &cnt[std::size_t(t) * n];
-        visit(chunk_lo(t), chunk_lo(t + 1), [&](index_t, index_t d) { ++c[d]; });
+        std::vector<std::uint64_t> c(R, 0);
+        visit(chunk_lo(t), chunk_lo(t + 1), [&](index_t, index_t d) { ++c[d >> shift]; });
+        for (std::size_t r = 0; r < R; ++r) off[r * T + t] = c[r];
     });
-    // per target: chunk cursors relative to the row start, and the in-degree
-    auto tgt_lo = [&](unsigned t) { return static_cast<std::size_t>(std::uint64_t(n) * t / T); };
-    std::vector<std::uint64_t> range_total(T, 0);
-    parallel_for_threads(T, [&](unsigned r) {
-        std::uint64_t total = 0;
-        for (std::size_t d = tgt_lo(r); d < tgt_lo(r + 1); ++d) {
-            index_t run = 0;
-            for (unsigned t = 0; t < T; ++t) {
-                const index_t c = cnt[std::size_t(t) * n + d];
-                cnt[std::size_t(t) * n + d] = run;
+    {
+        std::uint64_t run = 0;
+        for (std::size_t i = 0; i < std::size_t(T) * R; ++i) {
+            const std::uint64_t c = off[i];
+            off[i] = run;
+            run += c;
+        }
+        off[std::size_t(T) * R] = run;
+    }
+    struct Pair { index_t src, dst; };
+    std::unique_ptr<Pair[]> pairs(new Pair[n_edges]);  // grouped by range; first touched by the writers
+    parallel_for_threads(T, [&](unsigned t) {
+        std::vector<std::uint64_t> cur(R);
+        for (std::size_t r = 0; r < R; ++r) cur[r] = off[r * T + t];
+        Pair* p = pairs.get();
+        visit(chunk_lo(t), chunk_lo(t + 1), [&](index_t sv, index_t d) { p[cur[d >> shift]++] = Pair{sv, d}; });
+    });
+    // level 2: each range's in-degrees, the row starts, then a stable scatter
+    std::vector<std::uint64_t> range_edges(R + 1, 0);
+    for (std::size_t r = 0; r < R; ++r) range_edges[r + 1] = off[(r + 1) * T];
+    std::atomic<std::size_t> next{0};
+    parallel_for_threads(T, [&](unsigned) {
+        std::vector<index_t> cur;
+        for (std::size_t r; (r = next.fetch_add(1)) < R;) {
+            const std::size_t d0 = r << shift;
+            const std::size_t d1 = std::min<std::size_t>(std::size_t(n_targets), d0 + (std::size_t(1) << shift));
+            const Pair* p0 = pairs.get() + range_edges[r];
+            const Pair* p1 = pairs.get() + range_edges[r + 1];
+            cur.assign(d1 - d0, 0);
+            for (const Pair* q = p0; q != p1; ++q) ++cur[q->dst - d0];
+            std::uint64_t run = range_edges[r];
+            for (std::size_t i = 0; i < d1 - d0; ++i) {
+                const index_t c = cur[i];
+                cur[i] = static_cast<index_t>(run);
                 run += c;
+                out.row_start[d0 + i + 1] = static_cast<index_t>(run);
             }
-            out.row_start[d + 1] = run;
-            total += run;
+            index_t* src = out.src.data();
+            for (const Pair* q = p0; q != p1; ++q) src[cur[q->dst - d0]++] = q->src;
         }
-        range_total[r] = total;
-    });
-    std::vector<std::uint64_t> range_base(T, 0);
-    for (unsigned r = 1; r < T; ++r) range_base[r] = range_base[r - 1] + range_total[r - 1];
-    parallel_for_threads(T, [&](unsigned r) {
-        std::uint64_t run = range_base[r];
-        for (std::size_t d = tgt_lo(r); d < tgt_lo(r + 1); ++d) {
-            run += out.row_start[d + 1];
-            out.row_start[d + 1] = static_cast<index_t>(run);
-        }
-    });
-    parallel_for_threads(T, [&](unsigned t) {
-        index_t* c = &cnt[std::size_t(t) * n];
-        index_t* src = out.src.data();
-        const index_t* rs = out.row_start.data();
-        visit(chunk_lo(t), chunk_lo(t + 1), [&](index_t s, index_t d) { src[rs[d] + c[d]++] = s; });
     });
 }
 

@@ -4,7 +4,9 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "as_model.h"
@@ -38,11 +40,33 @@ double draw_scaled(double mean, bool enabled, double lower, double upper, double
 // Reference: src/astrosim_fused.cu:617-624. `log1m_p` = log1p(-p).
 index_t geometric_skip(double p, double log1m_p, index_t remaining, HostRng& rng);
 
+// An allocator whose resize() leaves trivial elements uninitialised: large
+// arrays that are about to be overwritten are then neither zero-filled nor
+// first touched by one thread.
+template <class T>
+struct NoInitAllocator : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAllocator<U>;
+    };
+    NoInitAllocator() = default;
+    template <class U>
+    NoInitAllocator(const NoInitAllocator<U>&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... Args>
+    void construct(U* p, Args&&... args) {
+        ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+    }
+};
+
 // Connectivity inverted by target: row t lists the sources of target t in
 // generation order. Reference: src/astrosim_fused.cu:296-315.
 struct HostCSR {
-    std::vector<index_t> row_start;  // n_targets + 1
-    std::vector<index_t> src;        // row_start[n_targets]
+    std::vector<index_t> row_start;                       // n_targets + 1
+    std::vector<index_t, NoInitAllocator<index_t>> src;   // row_start[n_targets]
     index_t n_targets() const { return row_start.empty() ? 0 : index_t(row_start.size() - 1); }
     index_t n_edges() const { return index_t(src.size()); }
 };
