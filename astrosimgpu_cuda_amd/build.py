@@ -4,6 +4,7 @@
 
 Outputs (all in-tree, git-ignored):
     astrosimgpu_cuda_amd/lib/libastrosim.so   HIP kernels + C ABI (include/astrosim.h)
+    astrosimgpu_cuda_amd/lib/libastrosim_selftest.so  test-only: device vs host math
     astrosimgpu_cuda_amd/bin/astrosim_fused   drop-in CLI linked against the library
     oracle/build/liboracle.so, astrosim_oracle  FP64 host oracle (tests / CPU baseline)
 """
@@ -22,6 +23,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "lib" / "libastrosim.so"
+SELFTEST = PKG / "lib" / "libastrosim_selftest.so"
 CLI = PKG / "bin" / "astrosim_fused"
 ORACLE = ROOT / "oracle"
 
@@ -80,10 +82,13 @@ def build_library(incremental: bool = False, jobs: int = 8) -> None:
     LIB.parent.mkdir(parents=True, exist_ok=True)
     CLI.parent.mkdir(parents=True, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, incremental), LIB_SOURCES))
+        objs = list(ex.map(lambda s: _compile(s, incremental), LIB_SOURCES + ["as_selftest.hip"]))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs)])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs[:-1])])
     os.replace(tmp, LIB)
+    tmp = SELFTEST.with_suffix(".so.tmp")
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), str(objs[-1])])
+    os.replace(tmp, SELFTEST)
     cli_obj = _compile("as_cli.cpp", incremental)
     _run([HIPCC, "-o", str(CLI), str(cli_obj), "-L", str(LIB.parent), "-lastrosim",
           "-Wl,-rpath,$ORIGIN/../lib"])

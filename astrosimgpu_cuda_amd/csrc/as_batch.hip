@@ -507,6 +507,8 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
         hh_lo = at(S.h_lo, oa);
     }
     const R Ca_tot = at(S.Ca_tot, oa), IP3_0 = at(S.IP3_0, oa), tau = at(S.tau_IP3, oa), dIP3 = at(S.delta_IP3, oa);
+    FixedDivisor tau_div{};  // FP64: the cell's tau_IP3 divisor, made once per launch
+    if constexpr (sizeof(R) == 8) tau_div = make_fixed_divisor(tau);
     std::uint32_t wout = B.first ? 0u : at(D.sic_w + std::size_t(B.wslot) * D.acap, oa4);
     bool ever = false;
     for (std::int64_t t = lo; t < hi; ++t) {
@@ -514,7 +516,8 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
         const R ip3_input = col[j * stride], noise = col[(C + j) * stride];
         R factor;
         if constexpr (sizeof(R) == 8) {
-            astro_advance(P.ac, Ca_tot, IP3_0, tau, dIP3, ip3_input, noise, P.h_step, P.substeps, ca, ip3, hh);
+            astro_advance(P.ac, Ca_tot, IP3_0, tau_div, dIP3, ip3_input, noise, P.h_step, P.h_sixth, P.substeps, ca,
+                          ip3, hh);
             factor = sic_factor(P.ac, ca);
         } else {
             astro_advance32(Q.ac, Q, Ca_tot, IP3_0, tau, dIP3, ip3_input, noise, P.substeps, ca, ca_lo, ip3, ip3_lo,
@@ -667,7 +670,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
             double I_ext = I_sic;
             if (nstd > 0.0) I_ext += noise_val;
             NeuronRegs r{V, w, g_ex, dg_ex + exc_input * p.psc_init_ex, g_in, dg_in + inh_input * p.psc_init_in, refr};
-            spiked = neuron_substeps(p, r, I_ext, V_reset, bb, P.h_step, P.substeps,
+            spiked = neuron_substeps(p, r, I_ext, V_reset, bb, P.h_step, P.h_sixth, P.substeps,
                                      exc ? P.ex_decay_e : P.ex_decay_i, exc ? P.in_decay_e : P.in_decay_i);
             V = r.V; w = r.w; g_ex = r.g_ex; dg_ex = r.dg_ex; g_in = r.g_in; dg_in = r.dg_in; refr = r.refractory;
         } else {
@@ -683,8 +686,8 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
             if (P.stp_enabled) {
                 R x = at(S.stp_x, on), u = at(S.stp_u, on), tlast = at(S.stp_t, on);
                 if constexpr (sizeof(R) == 8) {
-                    out = stp_on_spike(x, u, tlast, static_cast<double>(tl) * P.dt, P.stp_U, P.stp_tau_rec,
-                                       P.stp_tau_fac);
+                    out = stp_on_spike(x, u, tlast, static_cast<double>(tl) * P.dt, P.stp_U, P.stp_rec_div,
+                                       P.stp_fac_div);
                 } else {
                     const float t_now = static_cast<float>(static_cast<double>(tl) * P.dt);
                     const float elapsed = t_now - tlast;

@@ -72,9 +72,10 @@ AS_DM_FN double as_dm_pow2(int k) {
 #define AS_DM_SQRT2 0x1.6a09e667f3bcdp+0
 
 AS_DM_FN double as_exp(double x) {
-    if (x != x) return x;
-    if (x > 709.782712893384) return as_dm_double(0x7ff0000000000000ULL);
-    if (x < -745.2) return 0.0;
+    if (!(x >= -745.2 && x <= 709.782712893384)) { /* NaN, underflow or overflow */
+        if (x != x) return x;
+        return x > 0.0 ? as_dm_double(0x7ff0000000000000ULL) : 0.0;
+    }
     const double kd = __builtin_floor(x * AS_DM_INV_LN2 + 0.5);
     const double r = (x - kd * AS_DM_LN2_HI) - kd * AS_DM_LN2_LO;
     /* exp(r) = 1 + r + r^2 * q(r), q = sum_{n>=2} r^(n-2)/n!, Horner with
@@ -93,9 +94,18 @@ AS_DM_FN double as_exp(double x) {
     q = __builtin_fma(r, q, 0.5);                   /* 1/2!  */
     const double p = 1.0 + __builtin_fma(r * r, q, r);
     const int k = (int)kd;
+    /* p * 2^k rounded once. k lies in [-1075, 1024] and p in [0.7, 1.5], so
+     * p * 2^(k/2) is normal and exact, and the second scaling is the one
+     * rounding; the device's ldexp instruction rounds p * 2^k once as well,
+     * so both give the same bits (subnormal and overflowing results
+     * included). */
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_ldexp(p, k);
+#else
     const int k1 = k / 2;
     const int k2 = k - k1;
     return (p * as_dm_pow2(k1)) * as_dm_pow2(k2);
+#endif
 }
 
 AS_DM_FN double as_log(double x) {

@@ -38,6 +38,7 @@ RunParams resolve_params(const as_config& c) {
     P.dt = c.dt;
     P.substeps = c.substeps;
     P.h_step = c.dt / static_cast<double>(c.substeps);  // src:1024
+    P.h_sixth = P.h_step / 6.0;
     P.block = c.block;
     P.precision = c.precision;
     P.method = c.method;
@@ -49,6 +50,8 @@ RunParams resolve_params(const as_config& c) {
     P.stp_U = bio.stp_U;
     P.stp_tau_rec = bio.stp_tau_rec;
     P.stp_tau_fac = bio.stp_tau_fac;
+    P.stp_rec_div = make_fixed_divisor(P.stp_tau_rec);
+    P.stp_fac_div = P.stp_tau_fac < 1e-10 ? FixedDivisor{P.stp_tau_fac, 0.0, 0.0, 0.0} : make_fixed_divisor(P.stp_tau_fac);
     // Delays and ring depth (src:954-957); noise refresh period (src:1043).
     P.d_e = delay_to_steps(bio.d_e, c.dt);
     P.d_i = delay_to_steps(bio.d_i, c.dt);
@@ -75,8 +78,8 @@ RunParams resolve_params(const as_config& c) {
     P.noise_exc = bio.exc_noise_std;
     P.noise_inh = bio.inh_noise_std;
     P.ac = make_astro_const(bio);
-    P.ne = make_neuron_const(bio, true);
-    P.ni = make_neuron_const(bio, false);
+    P.ne = make_neuron_const(bio, true, P.h_step);
+    P.ni = make_neuron_const(bio, false, P.h_step);
     // Alpha-cascade propagators, per population (src:515-516).
     P.ex_decay_e = as_exp(-P.h_step / P.ne.tau_syn_ex);
     P.in_decay_e = as_exp(-P.h_step / P.ne.tau_syn_in);

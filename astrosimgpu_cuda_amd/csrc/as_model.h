@@ -73,11 +73,61 @@ inline int delay_to_steps(double delay_ms, double dt) {
     return steps < 1 ? 1 : steps;
 }
 
+// ----------------------------------------------------------------------------
+// Correctly rounded division by a fixed positive divisor.
+// ----------------------------------------------------------------------------
+// The compiler's FP64 a / b is eleven instructions, one of them a
+// quarter-rate reciprocal. For a divisor b > 0 that is reused across many
+// divisions, four fused operations give the same correctly rounded quotient,
+// every bit of it including the sign of a zero, from four precomputed values
+//   y = RN(1/b),  yd = RD(1/b),  yld = RN(1/b - yd) >= 0,  ny = -y:
+//   q0 = RN(a*yd + RN(a*yld))  is within half an ulp plus a relative 2^-104
+//                              of a/b, so one of the two doubles around it;
+//   rn = RN(b*q0 - a)          is exact, because q0 is within one ulp;
+//   q  = RN(q0 + rn*ny)        is RN(a/b): with y within half an ulp of 1/b
+//                              and q0 within one ulp of a/b, the exact
+//                              q0 + (a - b*q0)*y lies nearer a/b than any
+//                              rounding midpoint (Markstein's theorem, IBM
+//                              J. Res. Dev. 34(1), 1990).
+// For a = +-0 every intermediate is a zero of a's sign (yld >= 0, and the
+// residual is formed as b*q0 - a), which is the sign of a/b for b > 0. ny is
+// stored negated so that no fused result is negated: a compiler may fold
+// such a negation into the fused operation, which is exact except for the
+// sign of a zero. Valid while no step overflows or underflows (|a| between
+// 2^-960 and 2^1000, or zero, and b normal).
+struct FixedDivisor {
+    double b, yd, yld, ny;
+};
+
+AS_HD FixedDivisor make_fixed_divisor(double b) {
+    FixedDivisor d;
+    const double y = 1.0 / b;
+    double yd = y;
+    double e = __builtin_fma(-b, y, 1.0);  // 1 - b*y, exact
+    if (e < 0.0) {                         // y above 1/b: the double below it
+        yd = as_dm_double(as_dm_bits(y) - 1);
+        e = __builtin_fma(-b, yd, 1.0);
+    }
+    d.b = b;
+    d.yd = yd;
+    d.yld = e / b;
+    d.ny = -y;
+    return d;
+}
+
+// a / b, bit for bit, for the divisor d was made from.
+AS_HD double divide(double a, const FixedDivisor& d) {
+    const double q0 = __builtin_fma(a, d.yd, a * d.yld);
+    const double rn = __builtin_fma(d.b, q0, -a);
+    return __builtin_fma(rn, d.ny, q0);
+}
+
 // Li-Rinzel constants. Reference: src/astrosim_fused.cu:222-226.
 struct AstroConst {
     double Kd_IP3_1, Kd_IP3_2, Kd_act, Kd_inh, Km_SERCA;
     double k_IP3R, rate_IP3R, rate_L, rate_SERCA, ratio_ER_cyt;
     double SIC_scale, SIC_th;
+    FixedDivisor ratio_ER_cyt_div;
 };
 
 // AdEx constants of one population. Reference: src/astrosim_fused.cu:228-232.
@@ -85,24 +135,23 @@ struct NeuronConst {
     double C_m, g_L, E_L, V_th, Delta_T, a, tau_w;
     double V_peak, t_ref, E_ex, E_in, tau_syn_ex, tau_syn_in, I_e;
     double psc_init_ex, psc_init_in;  // e / tau_syn
-    // Delta_T a power of two: x / Delta_T is the exact scaling x * inv_Delta_T,
-    // bit for bit (a division by 2^k and a multiplication by 2^-k round the
-    // same real number), so the spike-current argument needs no division.
-    double inv_Delta_T;
-    int Delta_T_pow2;
+    FixedDivisor Delta_T_div, C_m_div, tau_w_div;
+    int refractory_steps;  // (int)(t_ref / h_step + 0.5), src/astrosim_fused.cu:549
 };
 
 AstroConst make_astro_const(const Biology& bio);
-NeuronConst make_neuron_const(const Biology& bio, bool excitatory);
+NeuronConst make_neuron_const(const Biology& bio, bool excitatory, double h_step);
 
 // ----------------------------------------------------------------------------
 // Per-cell FP64 arithmetic.
 // ----------------------------------------------------------------------------
 
 // Li-Rinzel right-hand side. Reference: src/astrosim_fused.cu:236-253.
-AS_HD void astro_rhs(const AstroConst& c, double Ca_tot, double IP3_0, double tau_IP3, double Ca,
-                     double IP3, double h, double noise, double& dCa, double& dIP3, double& dh) {
-    const double Ca_ER = (Ca_tot - Ca) / c.ratio_ER_cyt;
+// tau_IP3 is the cell's divisor made from its tau_IP3.
+AS_HD void astro_rhs(const AstroConst& c, double Ca_tot, double IP3_0, const FixedDivisor& tau_IP3,
+                     double Ca, double IP3, double h, double noise, double& dCa, double& dIP3,
+                     double& dh) {
+    const double Ca_ER = divide(Ca_tot - Ca, c.ratio_ER_cyt_div);
     const double m_inf = IP3 / (IP3 + c.Kd_IP3_1);
     const double n_inf = Ca / (Ca + c.Kd_act);
     const double m3 = m_inf * m_inf * m_inf;
@@ -114,15 +163,15 @@ AS_HD void astro_rhs(const AstroConst& c, double Ca_tot, double IP3_0, double ta
     const double alpha_h = c.k_IP3R * c.Kd_inh * (IP3 + c.Kd_IP3_1) / (IP3 + c.Kd_IP3_2);
     const double beta_h = c.k_IP3R * Ca;
     dCa = J_channel - J_pump + J_leak + noise;
-    dIP3 = (IP3_0 - IP3) / tau_IP3;
+    dIP3 = divide(IP3_0 - IP3, tau_IP3);
     dh = alpha_h * (1.0 - h) - beta_h * h;
 }
 
-// IP3 jump, then `substeps` classic RK4 steps, then clamps.
-// Reference: src/astrosim_fused.cu:255-278.
-AS_HD void astro_advance(const AstroConst& c, double Ca_tot, double IP3_0, double tau_IP3,
+// IP3 jump, then `substeps` classic RK4 steps, then clamps. h_sixth is
+// h_step / 6.0. Reference: src/astrosim_fused.cu:255-278.
+AS_HD void astro_advance(const AstroConst& c, double Ca_tot, double IP3_0, const FixedDivisor& tau_IP3,
                          double delta_IP3, double ip3_input, double noise, double h_step,
-                         int substeps, double& Ca, double& IP3, double& h) {
+                         double h_sixth, int substeps, double& Ca, double& IP3, double& h) {
     if (ip3_input != 0.0) IP3 += delta_IP3 * ip3_input;
     for (int s = 0; s < substeps; ++s) {
         double k1C, k1I, k1h, k2C, k2I, k2h, k3C, k3I, k3h, k4C, k4I, k4h;
@@ -133,9 +182,9 @@ AS_HD void astro_advance(const AstroConst& c, double Ca_tot, double IP3_0, doubl
                   h + 0.5 * h_step * k2h, noise, k3C, k3I, k3h);
         astro_rhs(c, Ca_tot, IP3_0, tau_IP3, Ca + h_step * k3C, IP3 + h_step * k3I,
                   h + h_step * k3h, noise, k4C, k4I, k4h);
-        Ca += (h_step / 6.0) * (k1C + 2.0 * k2C + 2.0 * k3C + k4C);
-        IP3 += (h_step / 6.0) * (k1I + 2.0 * k2I + 2.0 * k3I + k4I);
-        h += (h_step / 6.0) * (k1h + 2.0 * k2h + 2.0 * k3h + k4h);
+        Ca += h_sixth * (k1C + 2.0 * k2C + 2.0 * k3C + k4C);
+        IP3 += h_sixth * (k1I + 2.0 * k2I + 2.0 * k3I + k4I);
+        h += h_sixth * (k1h + 2.0 * k2h + 2.0 * k3h + k4h);
         Ca = Ca < 0.0 ? 0.0 : (Ca > Ca_tot ? Ca_tot : Ca);
         h = h < 0.0 ? 0.0 : (h > 1.0 ? 1.0 : h);
         IP3 = IP3 < 0.0 ? 0.0 : IP3;
@@ -153,12 +202,12 @@ AS_HD double sic_factor(const AstroConst& c, double Ca) {
 AS_HD void neuron_rhs(const NeuronConst& p, double V, double w, double g_ex, double g_in,
                       double I_ext, double& dV, double& dw) {
     const double Vc = V < p.V_peak ? V : p.V_peak;
-    const double x = p.Delta_T_pow2 ? (Vc - p.V_th) * p.inv_Delta_T : (Vc - p.V_th) / p.Delta_T;
+    const double x = divide(Vc - p.V_th, p.Delta_T_div);
     const double I_spike = p.g_L * p.Delta_T * as_exp(x);
     const double I_syn_ex = g_ex * (Vc - p.E_ex);
     const double I_syn_in = g_in * (Vc - p.E_in);
-    dV = (-p.g_L * (Vc - p.E_L) + I_spike - I_syn_ex - I_syn_in - w + p.I_e + I_ext) / p.C_m;
-    dw = (p.a * (Vc - p.E_L) - w) / p.tau_w;
+    dV = divide(-p.g_L * (Vc - p.E_L) + I_spike - I_syn_ex - I_syn_in - w + p.I_e + I_ext, p.C_m_div);
+    dw = divide(p.a * (Vc - p.E_L) - w, p.tau_w_div);
 }
 
 // Registers of one neuron across a step.
@@ -170,10 +219,11 @@ struct NeuronRegs {
 // One communication step of one neuron after its inputs are known: the
 // substep loop with the exact alpha-cascade propagator, the refractory clamp
 // (Euler on w only), RK4 on (V, w), spike detection and reset.
-// `ex_decay`/`in_decay` are as_exp(-h_step/tau_syn_*) (population constants).
+// `ex_decay`/`in_decay` are as_exp(-h_step/tau_syn_*) (population constants),
+// h_sixth is h_step / 6.0.
 // Returns true if the neuron spiked. Reference: src/astrosim_fused.cu:508-553.
 AS_HD bool neuron_substeps(const NeuronConst& p, NeuronRegs& r, double I_ext, double V_reset,
-                           double b, double h_step, int substeps, double ex_decay,
+                           double b, double h_step, double h_sixth, int substeps, double ex_decay,
                            double in_decay) {
     bool spiked = false;
     for (int sub = 0; sub < substeps; ++sub) {
@@ -197,8 +247,8 @@ AS_HD bool neuron_substeps(const NeuronConst& p, NeuronRegs& r, double I_ext, do
                        I_ext, k3V, k3w);
             neuron_rhs(p, r.V + h_step * k3V, r.w + h_step * k3w, g_ex_next, g_in_next, I_ext,
                        k4V, k4w);
-            r.V += (h_step / 6.0) * (k1V + 2.0 * k2V + 2.0 * k3V + k4V);
-            r.w += (h_step / 6.0) * (k1w + 2.0 * k2w + 2.0 * k3w + k4w);
+            r.V += h_sixth * (k1V + 2.0 * k2V + 2.0 * k3V + k4V);
+            r.w += h_sixth * (k1w + 2.0 * k2w + 2.0 * k3w + k4w);
         }
         r.g_ex = g_ex_next;
         r.dg_ex = dg_ex_next;
@@ -209,7 +259,7 @@ AS_HD bool neuron_substeps(const NeuronConst& p, NeuronRegs& r, double I_ext, do
         } else if (r.V >= p.V_peak) {
             r.V = V_reset;
             r.w += b;
-            r.refractory = static_cast<int>(p.t_ref / h_step + 0.5);
+            r.refractory = p.refractory_steps;
             spiked = true;
         }
     }
@@ -217,13 +267,14 @@ AS_HD bool neuron_substeps(const NeuronConst& p, NeuronRegs& r, double I_ext, do
 }
 
 // Per-presynaptic-neuron Tsodyks-Markram update on a spike at t_now; returns
-// the release multiplier x*u carried by every outgoing synapse.
-// Reference: src/astrosim_fused.cu:565-576.
+// the release multiplier x*u carried by every outgoing synapse. tau_rec and
+// tau_fac are divisors made from the time constants (tau_fac's only read when
+// tau_fac >= 1e-10). Reference: src/astrosim_fused.cu:565-576.
 AS_HD double stp_on_spike(double& x, double& u, double& t_last, double t_now, double U,
-                          double tau_rec, double tau_fac) {
+                          const FixedDivisor& tau_rec, const FixedDivisor& tau_fac) {
     const double elapsed = t_now - t_last;
-    const double x_decay = as_exp(-elapsed / tau_rec);
-    const double u_decay = tau_fac < 1e-10 ? 0.0 : as_exp(-elapsed / tau_fac);
+    const double x_decay = as_exp(divide(-elapsed, tau_rec));
+    const double u_decay = tau_fac.b < 1e-10 ? 0.0 : as_exp(divide(-elapsed, tau_fac));
     x = 1.0 + (x - x * u - 1.0) * x_decay;
     u = U + u * (1.0 - U) * u_decay;
     t_last = t_now;

@@ -56,8 +56,8 @@ __global__ __launch_bounds__(MAXT) void plain_step_kernel(PlainDev D, RunParams 
         const double noise =
             P.noise_astro > 0.0 ? P.noise_astro * rng_normal(seed ^ kKeyAstroNoise, stream) : 0.0;
         double ca = D.Ca[a], ip3 = D.IP3[a], hh = D.h[a];
-        astro_advance(P.ac, D.Ca_tot[a], D.IP3_0[a], D.tau_IP3[a], D.delta_IP3[a], ip3_input,
-                      noise, P.h_step, P.substeps, ca, ip3, hh);
+        astro_advance(P.ac, D.Ca_tot[a], D.IP3_0[a], make_fixed_divisor(D.tau_IP3[a]), D.delta_IP3[a],
+                      ip3_input, noise, P.h_step, P.h_sixth, P.substeps, ca, ip3, hh);
         D.Ca[a] = ca;
         D.IP3[a] = ip3;
         D.h[a] = hh;
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(MAXT) void plain_step_kernel(PlainDev D, RunParams 
         r.g_in = D.g_in[j];
         r.dg_in = D.dg_in[j] + inh_input * p.psc_init_in;
         r.refractory = D.refr[j];
-        spiked = neuron_substeps(p, r, I_ext, D.V_reset[j], D.b[j], P.h_step, P.substeps,
+        spiked = neuron_substeps(p, r, I_ext, D.V_reset[j], D.b[j], P.h_step, P.h_sixth, P.substeps,
                                  exc ? P.ex_decay_e : P.ex_decay_i, exc ? P.in_decay_e : P.in_decay_i);
         D.V[j] = r.V;
         D.w[j] = r.w;
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(MAXT) void plain_step_kernel(PlainDev D, RunParams 
             if (P.stp_enabled) {
                 double x = D.stp_x[j], u = D.stp_u[j], tl = D.stp_t[j];
                 out = stp_on_spike(x, u, tl, static_cast<double>(S.step) * P.dt, P.stp_U,
-                                   P.stp_tau_rec, P.stp_tau_fac);
+                                   P.stp_rec_div, P.stp_fac_div);
                 D.stp_x[j] = x;
                 D.stp_u[j] = u;
                 D.stp_t[j] = tl;

@@ -40,10 +40,11 @@ AstroConst make_astro_const(const Biology& bio) {
     c.ratio_ER_cyt = bio.ratio_ER_cyt;
     c.SIC_scale = bio.SIC_scale;
     c.SIC_th = bio.SIC_th;
+    c.ratio_ER_cyt_div = make_fixed_divisor(c.ratio_ER_cyt);
     return c;
 }
 
-NeuronConst make_neuron_const(const Biology& bio, bool exc) {
+NeuronConst make_neuron_const(const Biology& bio, bool exc, double h_step) {
     // Reference: src/astrosim_fused.cu:940-951. psc_init = e / tau_syn.
     NeuronConst p{};
     p.C_m = exc ? bio.e_C_m : bio.i_C_m;
@@ -62,10 +63,10 @@ NeuronConst make_neuron_const(const Biology& bio, bool exc) {
     p.I_e = 0.0;
     p.psc_init_ex = 2.7182818284590452353602874713527 / p.tau_syn_ex;
     p.psc_init_in = 2.7182818284590452353602874713527 / p.tau_syn_in;
-    int e = 0;
-    const bool pow2 = p.Delta_T > 0.0 && std::frexp(p.Delta_T, &e) == 0.5;
-    p.inv_Delta_T = 1.0 / p.Delta_T;
-    p.Delta_T_pow2 = pow2 && std::isnormal(p.inv_Delta_T) ? 1 : 0;
+    p.Delta_T_div = make_fixed_divisor(p.Delta_T);
+    p.C_m_div = make_fixed_divisor(p.C_m);
+    p.tau_w_div = make_fixed_divisor(p.tau_w);
+    p.refractory_steps = static_cast<int>(p.t_ref / h_step + 0.5);
     return p;
 }
 

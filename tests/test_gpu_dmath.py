@@ -1,0 +1,75 @@
+"""Device against host bits for the deterministic math of the FP64 path
+(as_dmath.h exp/log/cos, as_model.h fixed-divisor division): every input of
+wide random samples and of the edge ranges gives identical bits on the GPU
+and on the host. The host side is checked against the C library and IEEE
+division by test_dmath.py and test_divide.py."""
+import math
+
+import numpy as np
+import pytest
+
+import selftest_math as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(fn, x, y=None):
+    host = S.evaluate(fn, x, y)
+    dev = S.evaluate(fn, x, y, device=True)
+    bad = np.nonzero(S.bits(host) != S.bits(dev))[0]
+    assert bad.size == 0, [(float(x[i]), float(host[i]), float(dev[i])) for i in bad[:5]]
+    return dev
+
+
+def test_exp_device_matches_host():
+    rng = np.random.default_rng(21)
+    x = np.concatenate([
+        rng.uniform(-750.0, 712.0, 400_000),
+        rng.uniform(-30.0, 30.0, 400_000),         # the AdEx spike-current range
+        rng.uniform(-745.2, -708.0, 200_000),      # subnormal results
+        rng.uniform(709.0, 709.782712893384, 10_000),
+        [0.0, -0.0, 709.782712893384, 709.8, -745.2, -745.3, math.inf, -math.inf, math.nan, 1e-300, -1e-300],
+    ])
+    y = _same(S.EXP, x)
+    assert y[-11] == 1.0 and math.isinf(y[-5]) and y[-4] == 0.0 and math.isnan(y[-3])
+
+
+def test_log_device_matches_host():
+    rng = np.random.default_rng(22)
+    x = np.concatenate([
+        np.exp(rng.uniform(-740.0, 709.0, 400_000)),
+        rng.uniform(0.0, 1.0, 200_000),
+        1.0 + rng.uniform(-1e-3, 1e-3, 200_000),
+        [5e-324, 2.2250738585072014e-308, 1.0, 0.0, -1.0, math.inf, math.nan],
+    ])
+    _same(S.LOG, x)
+
+
+def test_cos_device_matches_host():
+    rng = np.random.default_rng(23)
+    x = np.concatenate([
+        6.283185307179586 * rng.uniform(0.0, 1.0, 400_000),  # the Box-Muller range
+        rng.uniform(-2.0**20, 2.0**20, 200_000),
+        [0.0, math.pi / 2, math.pi, 1.5 * math.pi, 2.0 ** 30, math.nan],
+    ])
+    _same(S.COS, x)
+
+
+def test_fixed_divisor_device_matches_ieee():
+    rng = np.random.default_rng(24)
+    parts_a, parts_b = [], []
+    for b in S.SIM_DIVISORS:
+        a = S.random_dividends(rng, 50_000)
+        parts_a.append(a)
+        parts_b.append(np.full_like(a, b))
+    a, b = S.near_midpoint_quotients(rng, 100_000)
+    parts_a.append(a)
+    parts_b.append(b)
+    tau = 1058.7589568400845 * rng.uniform(0.9, 1.05, 100_000)
+    parts_a.append(rng.uniform(-1e-2, 1e-2, tau.size))
+    parts_b.append(tau)
+    a, b = np.concatenate(parts_a), np.concatenate(parts_b)
+    dev = _same(S.DIVIDE, a, b)
+    assert np.array_equal(S.bits(dev), S.bits(a / b))
+    # and the device's own IEEE division agrees with the host's
+    assert np.array_equal(S.bits(S.evaluate(S.DIVIDE_IEEE, a, b, device=True)), S.bits(a / b))
