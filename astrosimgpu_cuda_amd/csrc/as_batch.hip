@@ -447,6 +447,18 @@ __global__ void tick_kernel(std::int64_t* clock) {
 // INTEGRATE: advance every cell over steps [t0, t0 + n) of one block.
 // ---------------------------------------------------------------------------
 
+// The Box-Muller draw of rng_normal_z evaluated in FP32, for FP32 state: the
+// same two uniforms (rounded to float; u1 clamped at 2^-126), the hardware
+// log2, and the hardware cosine, whose argument is in turns, so cos(2*pi*u2)
+// needs no range reduction. Within a few float ulps of the FP64 draw.
+__device__ __forceinline__ float normal32(std::uint64_t z) {
+    float u1 = static_cast<float>(rng_to_unit(rng_mix(z)));
+    u1 = u1 < 0x1p-126f ? 0x1p-126f : u1;
+    const float u2 = static_cast<float>(rng_to_unit(rng_mix(z + kRngCounterMul)));
+    constexpr float kMinus2Ln2 = -1.38629436111989061883f;  // -2 ln 2: ln u = ln 2 * log2 u
+    return __builtin_sqrtf(kMinus2Ln2 * __builtin_amdgcn_logf(u1)) * __builtin_amdgcn_cosf(u2);
+}
+
 // One astrocyte (global id ga, local id a) over global steps [lo, hi)
 // (src/astrosim_fused.cu:374-427). Returns whether it joins the emitter list.
 // Both cell functions advance one cell over global steps [lo, hi), at most
@@ -475,15 +487,23 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
         auto nring = [&](unsigned k) { return S.spk_v + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.ncap; };
         gather_chunk<R>(col, stride, static_cast<unsigned>(lo - B.bstart), static_cast<unsigned>(hi - B.bstart), ent,
                         cnt, nwords, nring, [&] { return D.in_rs[T_E2A][ga]; }, D.in_src[T_E2A]);
-        // random streams of the member's step tl = t - step0 and cell a
-        std::uint64_t tl_stride = std::uint64_t(lo - M.step0) * kStepStride;  // wave-uniform
+        // random streams tl*kStepStride + a of the member's step tl = t - step0
+        // (src/astrosim_fused.cu:393-405): their bases advance by a constant
+        // per step; FP32 state draws the noise in FP32 from the same uniforms
+        const std::uint64_t stream0 = std::uint64_t(lo - M.step0) * kStepStride + a;
+        std::uint64_t zp = rng_base(M.seed ^ kKeyAstroPoisson, stream0);
+        std::uint64_t zn = rng_base(M.seed ^ kKeyAstroNoise, stream0);
         for (std::int64_t t = lo; t < hi; ++t) {
             const unsigned j = static_cast<unsigned>(t - lo);
-            const std::uint64_t stream = tl_stride + a;
             int ev = 0;
-            if (P.poiss_astro) ev = rng_poisson(M.seed ^ kKeyAstroPoisson, stream, P.lam_astro, P.lim_astro);
-            const double noise =
-                P.noise_astro > 0.0 ? P.noise_astro * rng_normal(M.seed ^ kKeyAstroNoise, stream) : 0.0;
+            if (P.poiss_astro) ev = rng_poisson_z(zp, P.lam_astro, P.lim_astro);
+            R noise = R(0);
+            if (P.noise_astro > 0.0) {
+                if constexpr (sizeof(R) == 8) noise = P.noise_astro * rng_normal_z(zn);
+                else noise = static_cast<float>(P.noise_astro) * normal32(zn);
+            }
+            zp += kStepStride * kRngStreamMul;
+            zn += kStepStride * kRngStreamMul;
             R& in = col[j * stride];
             if constexpr (sizeof(R) == 8) {
                 double ip3_input = P.w_n2a * in;
@@ -494,8 +514,7 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
                 if (ev > 0) ip3_input += Q.w_poiss_astro * static_cast<float>(ev);
                 in = ip3_input;
             }
-            col[(C + j) * stride] = static_cast<R>(noise);
-            tl_stride += kStepStride;
+            col[(C + j) * stride] = noise;
         }
     }
     std::uint32_t pr = ring_row(lo, B.Rv);
@@ -572,10 +591,91 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         sort_bucket(ei, ci);
         auto nring = [&](unsigned k) { return S.spk_v + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.ncap; };
         const unsigned k0 = static_cast<unsigned>(lo - B.bstart), k1 = static_cast<unsigned>(hi - B.bstart);
-        // inhibitory source ids are local indices used as neuron ids
-        // (src/astrosim_fused.cu:459-462); the scatter has already resolved them
         gather_chunk<R>(col, stride, k0, k1, ee, ce, nwords, nring, [&] { return D.in_rs[T_EXC][gj]; },
                         D.in_src[T_EXC]);
+        // The Poisson drive of each step (stream tl*kStepStride + j), folded
+        // into its excitatory input as w_e*sum, then + w_poiss*events for
+        // steps with events (src/astrosim_fused.cu:472-479).
+        const unsigned n = static_cast<unsigned>(hi - lo);
+        const bool poiss = exc ? P.poiss_exc : P.poiss_inh;
+        const double lam = exc ? P.lam_exc : P.lam_inh, lim = exc ? P.lim_exc : P.lim_inh;
+        const std::uint64_t zstep = kStepStride * kRngStreamMul;  // stream base advance per step
+        const std::uint64_t zp0 = rng_base(M.seed ^ kKeyNeuronPoisson, std::uint64_t(tl0) * kStepStride + j);
+        auto add_events = [&](unsigned i, int ev) {
+            R& ein = col[i * stride];
+            if constexpr (sizeof(R) == 8)
+                ein = ein + (exc ? P.w_poiss_exc : P.w_poiss_inh) * static_cast<double>(ev);
+            else
+                ein = ein + (exc ? Q.w_poiss_exc : Q.w_poiss_inh) * static_cast<float>(ev);
+        };
+        for (unsigned i = 0; i < n; ++i) {
+            R& ein = col[i * stride];
+            if constexpr (sizeof(R) == 8) ein = P.w_e * ein;
+            else ein = Q.w_e_abs * ein;
+        }
+        if (poiss && lam > 30.0) {  // the normal approximation: one draw per step
+            std::uint64_t z = zp0;
+            for (unsigned i = 0; i < n; ++i, z += zstep) {
+                const int ev = rng_poisson_z(z, lam, lim);
+                if (ev > 0) add_events(i, ev);
+            }
+        } else if (poiss) {
+            // Knuth's method in two passes. First every step's first uniform
+            // u0 = m0 * 2^-53: most steps stop there with no event; the m0
+            // of the others is kept in the (not yet gathered) inhibitory
+            // region of the column. Then one loop continues those steps'
+            // products one draw per iteration, so a wave runs the maximum
+            // over its lanes of the launch's draws rather than the sum over
+            // steps of each step's maximum. Same draws, same products.
+            const std::uint64_t lim53 = static_cast<std::uint64_t>(lim * 9007199254740992.0);
+            auto stash = [&](unsigned i, std::uint64_t m0) {
+                if constexpr (sizeof(R) == 8) {
+                    col[(C + i) * stride] = static_cast<double>(m0);  // exact: m0 < 2^53
+                } else {  // the two halves' bits, in the inhibitory and SIC regions
+                    col[(C + i) * stride] = __uint_as_float(static_cast<std::uint32_t>(m0));
+                    col[(2 * C + i) * stride] = __uint_as_float(static_cast<std::uint32_t>(m0 >> 32));
+                }
+            };
+            auto unstash = [&](unsigned i) -> double {
+                if constexpr (sizeof(R) == 8) {
+                    return col[(C + i) * stride] * (1.0 / 9007199254740992.0);
+                } else {
+                    const std::uint64_t m0 = std::uint64_t(__float_as_uint(col[(C + i) * stride])) |
+                                             (std::uint64_t(__float_as_uint(col[(2 * C + i) * stride])) << 32);
+                    return static_cast<double>(m0) * (1.0 / 9007199254740992.0);
+                }
+            };
+            std::uint32_t pending = 0u;
+            std::uint64_t z = zp0;
+            for (unsigned i = 0; i < n; ++i, z += zstep) {
+                const std::uint64_t m0 = rng_mix(z) >> 11;
+                if (m0 > lim53) {
+                    pending |= 1u << i;
+                    stash(i, m0);
+                }
+            }
+            unsigned cur = 32u;  // the step being continued; 32: none
+            double product = 0.0;
+            int count = 0;
+            while (pending != 0u || cur < 32u) {
+                if (cur == 32u) {
+                    cur = static_cast<unsigned>(__builtin_ctz(pending));
+                    pending &= pending - 1u;
+                    product = unstash(cur);
+                    count = 0;
+                    z = zp0 + std::uint64_t(cur) * zstep;  // counter 0 of step cur
+                }
+                ++count;
+                z += kRngCounterMul;
+                product *= rng_to_unit(rng_mix(z));
+                if (!(product > lim)) {
+                    add_events(cur, count);
+                    cur = 32u;
+                }
+            }
+        }
+        // inhibitory source ids are local indices used as neuron ids
+        // (src/astrosim_fused.cu:459-462); the scatter has already resolved them
         gather_chunk<R>(col + C * stride, stride, k0, k1, ei, ci, nwords, nring, [&] { return D.in_rs[T_INH][gj]; },
                         D.in_src[T_INH]);
         if constexpr (kSicInLds) {
@@ -584,37 +684,28 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
             gather_chunk<R>(col + 2 * C * stride, stride, k0, k1, es(), cs, awords, sring,
                             [&] { return D.in_rs[T_SIC][gj]; }, D.in_src[T_SIC]);
         }
-        // the Poisson stream tl*stride + j of each step, folded into the inputs
-        std::uint64_t tl_stride = std::uint64_t(tl0) * kStepStride;  // wave-uniform
-        for (std::int64_t t = lo; t < hi; ++t) {
-            const unsigned i = static_cast<unsigned>(t - lo);
-            int ev = 0;
-            if (exc ? P.poiss_exc : P.poiss_inh)
-                ev = rng_poisson(M.seed ^ kKeyNeuronPoisson, tl_stride + j, exc ? P.lam_exc : P.lam_inh,
-                                 exc ? P.lim_exc : P.lim_inh);
-            R& ein = col[i * stride];
+        for (unsigned i = 0; i < n; ++i) {
             R& iin = col[(C + i) * stride];
             if constexpr (sizeof(R) == 8) {
-                double exc_input = P.w_e * ein;
-                if (ev > 0) exc_input += (exc ? P.w_poiss_exc : P.w_poiss_inh) * static_cast<double>(ev);
-                ein = exc_input;
                 iin = (-P.w_i) * iin;
             } else {
-                float exc_input = Q.w_e_abs * ein;
-                if (ev > 0) exc_input += (exc ? Q.w_poiss_exc : Q.w_poiss_inh) * static_cast<float>(ev);
-                ein = exc_input;
                 iin = Q.w_i_abs * iin;
                 R& sin_ = col[(2 * C + i) * stride];
                 sin_ = Q.w_a2n * sin_;
             }
-            tl_stride += kStepStride;
         }
         // the noise values of the launch's noise indices tl / steps_per_noise
+        // (stream idx*kNoiseStride + j, src/astrosim_fused.cu:496-500); FP32
+        // state draws them in FP32 from the same uniforms
         if (nstd > 0.0) {
             const std::uint64_t nlast = static_cast<std::uint64_t>((tl0 + (hi - lo) - 1) / P.steps_per_noise);
-            for (std::uint64_t n = nidx0; n <= nlast; ++n)
-                col[(noise_base + static_cast<unsigned>(n - nidx0)) * stride] =
-                    static_cast<R>(nstd * rng_normal(M.seed ^ kKeyNeuronNoise, n * kNoiseStride + std::uint64_t(j)));
+            std::uint64_t zn = rng_base(M.seed ^ kKeyNeuronNoise, nidx0 * kNoiseStride + std::uint64_t(j));
+            for (std::uint64_t idx = nidx0; idx <= nlast; ++idx, zn += kNoiseStride * kRngStreamMul) {
+                R v;
+                if constexpr (sizeof(R) == 8) v = nstd * rng_normal_z(zn);
+                else v = static_cast<float>(nstd) * normal32(zn);
+                col[(noise_base + static_cast<unsigned>(idx - nidx0)) * stride] = v;
+            }
         }
     }
     // FP64: SIC inputs from (source, word) pairs, values loaded a step ahead
