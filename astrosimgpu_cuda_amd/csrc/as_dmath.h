@@ -16,8 +16,9 @@
  *
  * Methods (textbook, from the mathematics):
  *   exp : Cody-Waite reduction x = k*ln2 + r, |r| <= ln2/2, with ln2 split
- *         into a 32-bit head and a tail; exp(r) by its Taylor series to
- *         degree 13 (truncation < 2^-57); 2^k applied as two exact scalings.
+ *         into a 32-bit head and a tail (the head step exact inside one
+ *         fused multiply-add); exp(r) by its Taylor series to degree 13
+ *         (truncation < 2^-57); 2^k applied with one rounding.
  *   log : x = 2^k * m with m in [sqrt(2)/2, sqrt(2)], f = m - 1,
  *         s = f/(2+f); log(m) = 2*atanh(s) = 2s + s*R(s^2) with the atanh
  *         series to s^23 (truncation < 2^-60), evaluated in the
@@ -76,8 +77,13 @@ AS_DM_FN double as_exp(double x) {
         if (x != x) return x;
         return x > 0.0 ? as_dm_double(0x7ff0000000000000ULL) : 0.0;
     }
-    const double kd = __builtin_floor(x * AS_DM_INV_LN2 + 0.5);
-    const double r = (x - kd * AS_DM_LN2_HI) - kd * AS_DM_LN2_LO;
+    /* k = round(x / ln2): adding 1.5 * 2^52 rounds to an integer, and the
+     * low 32 bits of the sum's encoding are k in two's complement */
+    const double t = __builtin_fma(x, AS_DM_INV_LN2, 0x1.8p52);
+    const double kd = t - 0x1.8p52;
+    const int k = (int)(uint32_t)as_dm_bits(t);
+    /* r = x - k*ln2, |r| <= ln2/2 (+ rounding): the head step is exact */
+    const double r = __builtin_fma(-kd, AS_DM_LN2_LO, __builtin_fma(-kd, AS_DM_LN2_HI, x));
     /* exp(r) = 1 + r + r^2 * q(r), q = sum_{n>=2} r^(n-2)/n!, Horner with
      * fused multiply-adds (one rounding per step on every IEEE target) */
     double q = 1.0 / 6227020800.0;                 /* 1/13! */
@@ -93,7 +99,6 @@ AS_DM_FN double as_exp(double x) {
     q = __builtin_fma(r, q, 1.0 / 6.0);             /* 1/3!  */
     q = __builtin_fma(r, q, 0.5);                   /* 1/2!  */
     const double p = 1.0 + __builtin_fma(r * r, q, r);
-    const int k = (int)kd;
     /* p * 2^k rounded once. k lies in [-1075, 1024] and p in [0.7, 1.5], so
      * p * 2^(k/2) is normal and exact, and the second scaling is the one
      * rounding; the device's ldexp instruction rounds p * 2^k once as well,
@@ -103,8 +108,7 @@ AS_DM_FN double as_exp(double x) {
     return __builtin_ldexp(p, k);
 #else
     const int k1 = k / 2;
-    const int k2 = k - k1;
-    return (p * as_dm_pow2(k1)) * as_dm_pow2(k2);
+    return (p * as_dm_pow2(k1)) * as_dm_pow2(k - k1);
 #endif
 }
 
