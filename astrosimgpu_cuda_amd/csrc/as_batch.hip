@@ -447,18 +447,6 @@ __global__ void tick_kernel(std::int64_t* clock) {
 // INTEGRATE: advance every cell over steps [t0, t0 + n) of one block.
 // ---------------------------------------------------------------------------
 
-// The Box-Muller draw of rng_normal_z evaluated in FP32, for FP32 state: the
-// same two uniforms (rounded to float; u1 clamped at 2^-126), the hardware
-// log2, and the hardware cosine, whose argument is in turns, so cos(2*pi*u2)
-// needs no range reduction. Within a few float ulps of the FP64 draw.
-__device__ __forceinline__ float normal32(std::uint64_t z) {
-    float u1 = static_cast<float>(rng_to_unit(rng_mix(z)));
-    u1 = u1 < 0x1p-126f ? 0x1p-126f : u1;
-    const float u2 = static_cast<float>(rng_to_unit(rng_mix(z + kRngCounterMul)));
-    constexpr float kMinus2Ln2 = -1.38629436111989061883f;  // -2 ln 2: ln u = ln 2 * log2 u
-    return __builtin_sqrtf(kMinus2Ln2 * __builtin_amdgcn_logf(u1)) * __builtin_amdgcn_cosf(u2);
-}
-
 // One astrocyte (global id ga, local id a) over global steps [lo, hi)
 // (src/astrosim_fused.cu:374-427). Returns whether it joins the emitter list.
 // Both cell functions advance one cell over global steps [lo, hi), at most
@@ -777,8 +765,8 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
             if (P.stp_enabled) {
                 R x = at(S.stp_x, on), u = at(S.stp_u, on), tlast = at(S.stp_t, on);
                 if constexpr (sizeof(R) == 8) {
-                    out = stp_on_spike(x, u, tlast, static_cast<double>(tl) * P.dt, P.stp_U, P.stp_rec_div,
-                                       P.stp_fac_div);
+                    out = stp_on_spike(x, u, tlast, static_cast<double>(tl) * P.dt, P.stp_U, P.stp_tau_rec,
+                                       P.stp_tau_fac);
                 } else {
                     const float t_now = static_cast<float>(static_cast<double>(tl) * P.dt);
                     const float elapsed = t_now - tlast;

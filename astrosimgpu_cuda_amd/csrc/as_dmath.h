@@ -77,11 +77,8 @@ AS_DM_FN double as_exp(double x) {
         if (x != x) return x;
         return x > 0.0 ? as_dm_double(0x7ff0000000000000ULL) : 0.0;
     }
-    /* k = round(x / ln2): adding 1.5 * 2^52 rounds to an integer, and the
-     * low 32 bits of the sum's encoding are k in two's complement */
-    const double t = __builtin_fma(x, AS_DM_INV_LN2, 0x1.8p52);
-    const double kd = t - 0x1.8p52;
-    const int k = (int)(uint32_t)as_dm_bits(t);
+    const double kd = __builtin_floor(x * AS_DM_INV_LN2 + 0.5); /* k = round(x / ln2) */
+    const int k = (int)kd;
     /* r = x - k*ln2, |r| <= ln2/2 (+ rounding): the head step is exact */
     const double r = __builtin_fma(-kd, AS_DM_LN2_LO, __builtin_fma(-kd, AS_DM_LN2_HI, x));
     /* exp(r) = 1 + r + r^2 * q(r), q = sum_{n>=2} r^(n-2)/n!, Horner with

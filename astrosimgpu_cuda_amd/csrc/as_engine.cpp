@@ -50,8 +50,6 @@ RunParams resolve_params(const as_config& c) {
     P.stp_U = bio.stp_U;
     P.stp_tau_rec = bio.stp_tau_rec;
     P.stp_tau_fac = bio.stp_tau_fac;
-    P.stp_rec_div = make_fixed_divisor(P.stp_tau_rec);
-    P.stp_fac_div = P.stp_tau_fac < 1e-10 ? FixedDivisor{P.stp_tau_fac, 0.0, 0.0, 0.0} : make_fixed_divisor(P.stp_tau_fac);
     // Delays and ring depth (src:954-957); noise refresh period (src:1043).
     P.d_e = delay_to_steps(bio.d_e, c.dt);
     P.d_i = delay_to_steps(bio.d_i, c.dt);

@@ -42,7 +42,6 @@ struct RunParams {
     double w_e = 5.0, w_i = -5.0, w_n2a = 0.2, w_a2n = 1.0;
     bool stp_enabled = true;
     double stp_U = 0.5, stp_tau_rec = 800.0, stp_tau_fac = 0.0;
-    FixedDivisor stp_rec_div{}, stp_fac_div{};  // tau_fac's: from tau_fac when >= 1e-10, else unused
     int d_e = 10, d_i = 10, d_a2n = 10, ring_slots = 11;
     int steps_per_noise = 10, sic_interval = 1;
     // Poisson drive: lambda = rate*dt*1e-3 and its Knuth limit exp(-lambda).

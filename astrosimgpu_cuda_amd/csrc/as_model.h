@@ -267,14 +267,15 @@ AS_HD bool neuron_substeps(const NeuronConst& p, NeuronRegs& r, double I_ext, do
 }
 
 // Per-presynaptic-neuron Tsodyks-Markram update on a spike at t_now; returns
-// the release multiplier x*u carried by every outgoing synapse. tau_rec and
-// tau_fac are divisors made from the time constants (tau_fac's only read when
-// tau_fac >= 1e-10). Reference: src/astrosim_fused.cu:565-576.
+// the release multiplier x*u carried by every outgoing synapse. (Plain
+// divisions: this runs on the rare spike path, where the fixed-divisor
+// constants would cost more scalar registers than the divisions cost time.)
+// Reference: src/astrosim_fused.cu:565-576.
 AS_HD double stp_on_spike(double& x, double& u, double& t_last, double t_now, double U,
-                          const FixedDivisor& tau_rec, const FixedDivisor& tau_fac) {
+                          double tau_rec, double tau_fac) {
     const double elapsed = t_now - t_last;
-    const double x_decay = as_exp(divide(-elapsed, tau_rec));
-    const double u_decay = tau_fac.b < 1e-10 ? 0.0 : as_exp(divide(-elapsed, tau_fac));
+    const double x_decay = as_exp(-elapsed / tau_rec);
+    const double u_decay = tau_fac < 1e-10 ? 0.0 : as_exp(-elapsed / tau_fac);
     x = 1.0 + (x - x * u - 1.0) * x_decay;
     u = U + u * (1.0 - U) * u_decay;
     t_last = t_now;

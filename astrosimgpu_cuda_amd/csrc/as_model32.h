@@ -91,6 +91,18 @@ inline Params32 make_params32(const RunParams& P) {
 
 __device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// The Box-Muller draw of rng_normal_z (src/astrosim_fused.cu:143-149)
+// evaluated in FP32 for FP32 state: the same two uniforms rounded to float
+// (u1 clamped at 2^-126), the hardware log2, and the hardware cosine, whose
+// argument is in turns, so cos(2*pi*u2) needs no range reduction.
+__device__ __forceinline__ float normal32(std::uint64_t z) {
+    float u1 = static_cast<float>(rng_to_unit(rng_mix(z)));
+    u1 = u1 < 0x1p-126f ? 0x1p-126f : u1;
+    const float u2 = static_cast<float>(rng_to_unit(rng_mix(z + kRngCounterMul)));
+    constexpr float kMinus2Ln2 = -1.38629436111989061883f;  // -2 ln 2: ln u = ln 2 * log2 u
+    return __builtin_sqrtf(kMinus2Ln2 * __builtin_amdgcn_logf(u1)) * __builtin_amdgcn_cosf(u2);
+}
+
 // hi+lo += x, error-free (TwoSum) then renormalised (Fast2Sum). Every
 // operation here must round on its own: this function is never contracted.
 __device__ __forceinline__ void comp_add(float& hi, float& lo, float x) {

@@ -125,7 +125,7 @@ __global__ __launch_bounds__(MAXT) void plain_step_kernel(PlainDev D, RunParams 
             if (P.stp_enabled) {
                 double x = D.stp_x[j], u = D.stp_u[j], tl = D.stp_t[j];
                 out = stp_on_spike(x, u, tl, static_cast<double>(S.step) * P.dt, P.stp_U,
-                                   P.stp_rec_div, P.stp_fac_div);
+                                   P.stp_tau_rec, P.stp_tau_fac);
                 D.stp_x[j] = x;
                 D.stp_u[j] = u;
                 D.stp_t[j] = tl;

@@ -99,3 +99,16 @@ def test_hostrng_pairs_share_radius():
         theta = 6.283185307179586 * u2
         assert z[2 * i] == pytest.approx(math.sqrt(r2) * math.cos(theta), rel=1e-12, abs=1e-13)
         assert z[2 * i + 1] == pytest.approx(math.sqrt(r2) * math.sin(theta), rel=1e-12, abs=1e-13)
+
+
+def test_normal_from_stream_base_matches_oracle():
+    # the kernels draw Gaussians from a precomputed stream base
+    # (seed + stream * golden gamma); the draw equals the oracle's rng_normal
+    import selftest_math as S
+    rng = np.random.default_rng(15)
+    seeds = [int(s) for s in rng.integers(0, 2**63, 400)]
+    streams = [int(s) for s in rng.integers(0, 2**63, 400)]
+    z = np.array([(s + st * 0x9E3779B97F4A7C15) % 2**64 for s, st in zip(seeds, streams)], dtype=np.uint64)
+    got = S.evaluate(S.NORMAL, z.view(np.float64))
+    want = np.array([O.rng_normal(s, st) for s, st in zip(seeds, streams)])
+    assert np.array_equal(S.bits(got), S.bits(want))
