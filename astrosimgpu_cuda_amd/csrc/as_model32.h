@@ -12,6 +12,8 @@
 // to the FP64 path. Formulas follow src/astrosim_fused.cu:236-290, 508-576.
 #pragma once
 
+#include <cmath>
+
 #include "as_model.h"
 
 namespace astrosim {
@@ -26,7 +28,7 @@ struct NeuronConst32 {
     float inv_C_m, g_L, E_L, V_th, Delta_T, a, inv_tau_w, V_peak, E_ex, E_in, I_e;
     float psc_init_ex, psc_init_in;
     int refractory_steps;                                   // int(t_ref/h + 0.5)
-    double V_th_d, inv_Delta_T_d, g_L_Delta_T_d;
+    double x_scale_d, x_shift_d;  // spike-current argument: V * x_scale_d + x_shift_d
 };
 
 struct Params32 {
@@ -60,9 +62,8 @@ inline Params32 make_params32(const RunParams& P) {
         n.psc_init_ex = float(p.psc_init_ex);
         n.psc_init_in = float(p.psc_init_in);
         n.refractory_steps = static_cast<int>(p.t_ref / P.h_step + 0.5);
-        n.V_th_d = p.V_th;
-        n.inv_Delta_T_d = 1.0 / p.Delta_T;
-        n.g_L_Delta_T_d = p.g_L * p.Delta_T;
+        n.x_scale_d = 1.0 / p.Delta_T;
+        n.x_shift_d = std::log(p.g_L * p.Delta_T) - p.V_th / p.Delta_T;
         return n;
     };
     q.ne = nc(P.ne);
@@ -115,25 +116,33 @@ __device__ __forceinline__ void comp_add(float& hi, float& lo, float x) {
     hi = h2;
 }
 
-// exp(x) in FP64 for the AdEx spike current, x = (V - V_T) / Delta_T in
-// [-700, 25]: reduction by ln2 (one rounded ln2 adds < 1.2e-16 * |k| to the
-// reduced argument) and a degree-8 Taylor polynomial with fused multiply-adds
-// (truncation < 2.2e-10 relative on |r| <= ln2/2: under 0.005 ulp of the
-// FP32 result it feeds; near threshold, |x| < 3, |k| <= 5).
-__device__ __forceinline__ double exp_near_threshold(double x) {
-    x = x < -700.0 ? -700.0 : x;  // keeps 2^k normal; exp(-700) is zero in FP32
-    const double kd = __builtin_rint(x * 0x1.71547652b82fep+0);
+// The AdEx spike current g_L * Delta_T * exp((V - V_T) / Delta_T) in FP64 for
+// FP32 state, rounded to FP32 once. The factor g_L*Delta_T enters the
+// argument as its logarithm, so x = V * x_scale + x_shift is one fused
+// operation (x_scale = 1/Delta_T, x_shift = ln(g_L*Delta_T) - V_T/Delta_T);
+// then k = round(x / ln2) by a rounding shifter (the low 32 bits of the sum's
+// encoding), r = x - k*ln2 with |r| <= ln2/2, a degree-8 Taylor polynomial
+// (truncation < 2.2e-10 relative: under 0.005 ulp of the FP32 result) and
+// 2^k built in the exponent field. x is held at -700 or above, which keeps
+// 2^k normal (the current is zero in FP32 long before).
+__device__ __forceinline__ float spike_current32(const NeuronConst32& p, float Vc) {
+    double x = __builtin_fma(static_cast<double>(Vc), p.x_scale_d, p.x_shift_d);
+    x = __builtin_fmax(x, -700.0);
+    const double t = __builtin_fma(x, 0x1.71547652b82fep+0, 0x1.8p52);
+    const double kd = t - 0x1.8p52;
+    const int k = static_cast<int>(static_cast<std::uint32_t>(__double_as_longlong(t)));
     const double r = __builtin_fma(-kd, 0x1.62e42fefa39efp-1, x);
-    double p = 1.0 / 40320.0;
-    p = __builtin_fma(r, p, 1.0 / 5040.0);
-    p = __builtin_fma(r, p, 1.0 / 720.0);
-    p = __builtin_fma(r, p, 1.0 / 120.0);
-    p = __builtin_fma(r, p, 1.0 / 24.0);
-    p = __builtin_fma(r, p, 1.0 / 6.0);
-    p = __builtin_fma(r, p, 0.5);
-    p = __builtin_fma(r, p, 1.0);
-    p = __builtin_fma(r, p, 1.0);
-    return p * as_dm_pow2(static_cast<int>(kd));
+    double q = 1.0 / 40320.0;
+    q = __builtin_fma(r, q, 1.0 / 5040.0);
+    q = __builtin_fma(r, q, 1.0 / 720.0);
+    q = __builtin_fma(r, q, 1.0 / 120.0);
+    q = __builtin_fma(r, q, 1.0 / 24.0);
+    q = __builtin_fma(r, q, 1.0 / 6.0);
+    q = __builtin_fma(r, q, 0.5);
+    q = __builtin_fma(r, q, 1.0);
+    q = __builtin_fma(r, q, 1.0);
+    const double two_k = __longlong_as_double(static_cast<long long>(k + 1023) << 52);
+    return static_cast<float>(q * two_k);
 }
 
 // Li-Rinzel right-hand side in FP32 (src/astrosim_fused.cu:236-253).
@@ -199,8 +208,7 @@ __device__ __forceinline__ void neuron_rhs32(const NeuronConst32& p, float V, fl
                                              float I_ext, float& dV, float& dw) {
 #pragma clang fp contract(fast)
     const float Vc = V < p.V_peak ? V : p.V_peak;
-    const float I_spike = static_cast<float>(
-        p.g_L_Delta_T_d * exp_near_threshold((static_cast<double>(Vc) - p.V_th_d) * p.inv_Delta_T_d));
+    const float I_spike = spike_current32(p, Vc);
     dV = (-p.g_L * (Vc - p.E_L) + I_spike - g_ex * (Vc - p.E_ex) - g_in * (Vc - p.E_in) - w + p.I_e + I_ext) *
          p.inv_C_m;
     dw = (p.a * (Vc - p.E_L) - w) * p.inv_tau_w;
