@@ -447,6 +447,22 @@ __global__ void tick_kernel(std::int64_t* clock) {
 // INTEGRATE: advance every cell over steps [t0, t0 + n) of one block.
 // ---------------------------------------------------------------------------
 
+// A uniform value copied into vector registers. The FP64 neuron step loop
+// uses more uniform constants than the scalar registers a wave can address,
+// and the compiler then parks some in lanes of vector registers, paying a
+// v_readlane at every use on every step. Values copied here are read from
+// vector registers instead (the kernel has vector registers to spare). The
+// asm is pure, so the copy is made once, before the loop.
+__device__ __forceinline__ double in_vgpr(double x) {
+    double y;
+    asm("v_mov_b64 %0, %1" : "=v"(y) : "s"(x));
+    return y;
+}
+
+__device__ __forceinline__ FixedDivisor in_vgpr(const FixedDivisor& d) {
+    return FixedDivisor{in_vgpr(d.b), in_vgpr(d.yd), in_vgpr(d.yld), in_vgpr(d.ny)};
+}
+
 // One astrocyte (global id ga, local id a) over global steps [lo, hi)
 // (src/astrosim_fused.cu:374-427). Returns whether it joins the emitter list.
 // Both cell functions advance one cell over global steps [lo, hi), at most
@@ -722,6 +738,13 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     }
     unsigned nslot = 0;
     int nrem = static_cast<int>(tl0 % P.steps_per_noise);
+    // the population's constants; FP64 holds its three divisors in vector registers
+    NeuronConst pc = exc ? P.ne : P.ni;
+    if constexpr (sizeof(R) == 8) {
+        pc.Delta_T_div = in_vgpr(pc.Delta_T_div);
+        pc.C_m_div = in_vgpr(pc.C_m_div);
+        pc.tau_w_div = in_vgpr(pc.tau_w_div);
+    }
     for (std::int64_t t = lo; t < hi; ++t) {
         const unsigned k = static_cast<unsigned>(t - B.bstart), i = static_cast<unsigned>(t - lo);
         const std::int64_t tl = t - M.step0;  // the member's own step
@@ -745,7 +768,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
             noise_val = col[(noise_base + nslot) * stride];
         bool spiked;
         if constexpr (sizeof(R) == 8) {
-            const NeuronConst& p = exc ? P.ne : P.ni;
+            const NeuronConst& p = pc;
             double I_ext = I_sic;
             if (nstd > 0.0) I_ext += noise_val;
             NeuronRegs r{V, w, g_ex, dg_ex + exc_input * p.psc_init_ex, g_in, dg_in + inh_input * p.psc_init_in, refr};
