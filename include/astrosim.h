@@ -35,7 +35,7 @@ typedef enum as_status {
 
 typedef enum as_precision {
     AS_FP64 = 0,  /* IEEE double throughout; bit-identical to the FP64 host oracle */
-    AS_FP32C = 1  /* FP32 state and RHS, compensated hi+lo Ca/IP3/h, FP64 near-threshold AdEx exp */
+    AS_FP32C = 1  /* FP32 state and RHS, compensated hi+lo Ca/IP3/h, FP64 AdEx exponential */
 } as_precision;
 
 typedef enum as_method {
