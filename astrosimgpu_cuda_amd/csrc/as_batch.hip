@@ -389,6 +389,8 @@ __device__ __forceinline__ R input_sum(const InputStream<R, NR>& s, unsigned k, 
 // SCATTER: emitters of block b - d/L append to their targets' buckets.
 // One wave per emitter, lanes over its out-edges.
 // ---------------------------------------------------------------------------
+constexpr unsigned kScatterUnroll = 4;
+
 template <class R>
 __global__ __launch_bounds__(256) void scatter_kernel(BatchDev D, BlkLaunch B, const R* __restrict__ spk_v) {
     resolve_launch(B, D.clock);
@@ -424,17 +426,41 @@ __global__ __launch_bounds__(256) void scatter_kernel(BatchDev D, BlkLaunch B, c
         // A neuron walks its exc and n2a out-edges and the inhibitory rows
         // whose stored LOCAL id equals its own local id (src/astrosim_fused.cu:
         // 459-462); an astrocyte walks its a2n out-edges.
-        for (int pass = 0; pass < (is_neuron ? 3 : 1); ++pass) {
-            const int T = is_neuron ? (pass == 0 ? T_EXC : (pass == 1 ? T_INH : T_E2A)) : T_SIC;
-            const uint2 r = D.out_rs[T][src];
-            for (index_t e = r.x + lane_id(); e < r.y; e += 64) {
-                const uint2 te = D.out_ent[T][e];  // (target, in-CSR slot)
-                const unsigned slot = atomicAdd(&D.bcnt[T][te.x], 1u);
-                if (slot < kBucketCap)
-                    D.bent[T][std::size_t(te.x) * kBucketCap + slot] = make_uint4(te.y, tag, vlo, vhi);
-                else if (slot == kBucketCap)
-                    atomicAdd(D.overflow, 1u);
+        // Each lane appends kScatterUnroll edges per iteration: their loads,
+        // then their returning atomics, then their stores, so that several
+        // independent atomics are in flight per lane (an append is a chain of
+        // dependent memory operations; one at a time leaves the scatter
+        // latency-bound). Bucket order does not matter: buckets are sorted by
+        // in-CSR slot when they are read. Each connection type is walked with
+        // its own uniform pointers (constant indices into the type arrays).
+        auto walk = [&](const uint2* __restrict__ rs, const uint2* __restrict__ ent, unsigned* __restrict__ cnt,
+                        uint4* __restrict__ bkt) {
+            const uint2 r = rs[src];
+            for (index_t e0 = r.x + lane_id(); e0 < r.y; e0 += 64u * kScatterUnroll) {
+                uint2 te[kScatterUnroll];  // (target, in-CSR slot)
+                unsigned slot[kScatterUnroll];
+#pragma unroll
+                for (unsigned u = 0; u < kScatterUnroll; ++u)
+                    if (e0 + 64u * u < r.y) te[u] = ent[e0 + 64u * u];
+#pragma unroll
+                for (unsigned u = 0; u < kScatterUnroll; ++u)
+                    if (e0 + 64u * u < r.y) slot[u] = atomicAdd(&cnt[te[u].x], 1u);
+#pragma unroll
+                for (unsigned u = 0; u < kScatterUnroll; ++u) {
+                    if (e0 + 64u * u >= r.y) continue;
+                    if (slot[u] < kBucketCap)
+                        bkt[std::size_t(te[u].x) * kBucketCap + slot[u]] = make_uint4(te[u].y, tag, vlo, vhi);
+                    else if (slot[u] == kBucketCap)
+                        atomicAdd(D.overflow, 1u);
+                }
             }
+        };
+        if (is_neuron) {
+            walk(D.out_rs[T_EXC], D.out_ent[T_EXC], D.bcnt[T_EXC], D.bent[T_EXC]);
+            walk(D.out_rs[T_INH], D.out_ent[T_INH], D.bcnt[T_INH], D.bent[T_INH]);
+            walk(D.out_rs[T_E2A], D.out_ent[T_E2A], D.bcnt[T_E2A], D.bent[T_E2A]);
+        } else {
+            walk(D.out_rs[T_SIC], D.out_ent[T_SIC], D.bcnt[T_SIC], D.bent[T_SIC]);
         }
     }
 }
