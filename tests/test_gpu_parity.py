@@ -68,6 +68,15 @@ def test_blocked_coarse_dt_bit_exact():
     run_pair(2000, 4, 300, dt=0.3, chunk=3, method="blocked")
 
 
+@pytest.mark.parametrize("method", METHODS)
+def test_strong_poisson_drive_bit_exact(method):
+    # dt 1 ms: Poisson means 2.7 (excitatory), 2.5 (inhibitory) and 3e-3
+    # (astrocytes) per step, so most steps draw several uniforms and the
+    # blocked method's merged Knuth continuation runs long per-lane chains;
+    # the delay is one step (one-step blocks; the raster ring holds 2 steps)
+    run_pair(2000, 11, 200, dt=1.0, chunk=2, method=method)
+
+
 def test_blocked_unaligned_chunks_bit_exact():
     # launches that start and end inside blocks (partial blocks)
     run_pair(4000, 9, 301, chunk=7, method="blocked")
