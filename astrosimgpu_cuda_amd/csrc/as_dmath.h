@@ -72,11 +72,8 @@ AS_DM_FN double as_dm_pow2(int k) {
 #define AS_DM_TWO_OVER_PI 0x1.45f306dc9c883p-1
 #define AS_DM_SQRT2 0x1.6a09e667f3bcdp+0
 
-AS_DM_FN double as_exp(double x) {
-    if (!(x >= -745.2 && x <= 709.782712893384)) { /* NaN, underflow or overflow */
-        if (x != x) return x;
-        return x > 0.0 ? as_dm_double(0x7ff0000000000000ULL) : 0.0;
-    }
+/* exp on [-745.2, 709.78]: the reduction, the polynomial and the scaling. */
+AS_DM_FN double as_dm_exp_core(double x) {
     const double kd = __builtin_floor(x * AS_DM_INV_LN2 + 0.5); /* k = round(x / ln2) */
     const int k = (int)kd;
     /* r = x - k*ln2, |r| <= ln2/2 (+ rounding): the head step is exact */
@@ -107,6 +104,21 @@ AS_DM_FN double as_exp(double x) {
     const int k1 = k / 2;
     return (p * as_dm_pow2(k1)) * as_dm_pow2(k - k1);
 #endif
+}
+
+AS_DM_FN double as_exp(double x) {
+    if (!(x >= -745.2 && x <= 709.782712893384)) { /* NaN, underflow or overflow */
+        if (x != x) return x;
+        return x > 0.0 ? as_dm_double(0x7ff0000000000000ULL) : 0.0;
+    }
+    return as_dm_exp_core(x);
+}
+
+/* as_exp(x), bit for bit, for x that is not NaN and not above 709.78: the
+ * range test becomes one clamp, because below -745.2 both this and as_exp
+ * give +0 (exp(-745.2) rounds to zero). */
+AS_DM_FN double as_exp_upto709(double x) {
+    return as_dm_exp_core(__builtin_fmax(x, -745.2));
 }
 
 AS_DM_FN double as_log(double x) {

@@ -202,8 +202,11 @@ AS_HD double sic_factor(const AstroConst& c, double Ca) {
 AS_HD void neuron_rhs(const NeuronConst& p, double V, double w, double g_ex, double g_in,
                       double I_ext, double& dV, double& dw) {
     const double Vc = V < p.V_peak ? V : p.V_peak;
+    // x is never NaN (the clamp above maps a NaN V to V_peak) and at most
+    // (V_peak - V_th) / Delta_T, which make_neuron_const holds to <= 709, so
+    // the clamped exp returns as_exp's bits.
     const double x = divide(Vc - p.V_th, p.Delta_T_div);
-    const double I_spike = p.g_L * p.Delta_T * as_exp(x);
+    const double I_spike = p.g_L * p.Delta_T * as_exp_upto709(x);
     const double I_syn_ex = g_ex * (Vc - p.E_ex);
     const double I_syn_in = g_in * (Vc - p.E_in);
     dV = divide(-p.g_L * (Vc - p.E_L) + I_spike - I_syn_ex - I_syn_in - w + p.I_e + I_ext, p.C_m_div);

@@ -63,6 +63,10 @@ NeuronConst make_neuron_const(const Biology& bio, bool exc, double h_step) {
     p.I_e = 0.0;
     p.psc_init_ex = 2.7182818284590452353602874713527 / p.tau_syn_ex;
     p.psc_init_in = 2.7182818284590452353602874713527 / p.tau_syn_in;
+    // the spike-current exponent (V - V_th) / Delta_T is at most this (V is
+    // clamped at V_peak), which the FP64 step's exp requires to stay <= 709
+    if (!(p.Delta_T > 0.0) || !((p.V_peak - p.V_th) / p.Delta_T <= 709.0))
+        throw std::invalid_argument("AdEx constants out of range: (V_peak - V_th) / Delta_T must be in (-inf, 709]");
     p.Delta_T_div = make_fixed_divisor(p.Delta_T);
     p.C_m_div = make_fixed_divisor(p.C_m);
     p.tau_w_div = make_fixed_divisor(p.tau_w);
