@@ -999,7 +999,7 @@ Geometry geometry(const RunParams& rp) {
 // First-fit allocator of [begin, end) ranges in one id space.
 class RangeAllocator {
 public:
-    explicit RangeAllocator(std::uint64_t cap) : cap_(cap) {
+    explicit RangeAllocator(std::uint64_t cap) {
         if (cap > 0) free_.push_back({0, cap});
     }
     bool can(std::uint64_t n) const {
@@ -1038,7 +1038,6 @@ public:
     }
 
 private:
-    std::uint64_t cap_;
     std::vector<std::pair<std::uint64_t, std::uint64_t>> free_;
 };
 
