@@ -401,7 +401,8 @@ __global__ __launch_bounds__(256) void scatter_kernel(BatchDev D, BlkLaunch B, c
     const std::uint32_t nN = D.lcount[B.rslot * 2 + 0];
     const std::uint32_t nA = D.lcount[B.rslot * 2 + 1];
     const unsigned waves = gridDim.x * (blockDim.x / 64);
-    const unsigned wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    // wave-uniform (readfirstlane tells the compiler so): one emitter per wave
+    const unsigned wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
     for (std::uint32_t i = wave; i < nN + nA; i += waves) {
         const bool is_neuron = i < nN;
         const std::uint32_t src = is_neuron ? D.nlist[std::size_t(B.rslot) * D.ncap + i]
@@ -426,41 +427,52 @@ __global__ __launch_bounds__(256) void scatter_kernel(BatchDev D, BlkLaunch B, c
         // A neuron walks its exc and n2a out-edges and the inhibitory rows
         // whose stored LOCAL id equals its own local id (src/astrosim_fused.cu:
         // 459-462); an astrocyte walks its a2n out-edges.
-        // Each lane appends kScatterUnroll edges per iteration: their loads,
-        // then their returning atomics, then their stores, so that several
-        // independent atomics are in flight per lane (an append is a chain of
-        // dependent memory operations; one at a time leaves the scatter
-        // latency-bound). Bucket order does not matter: buckets are sorted by
-        // in-CSR slot when they are read. Each connection type is walked with
-        // its own uniform pointers (constant indices into the type arrays).
-        auto walk = [&](const uint2* __restrict__ rs, const uint2* __restrict__ ent, unsigned* __restrict__ cnt,
-                        uint4* __restrict__ bkt) {
-            const uint2 r = rs[src];
-            for (index_t e0 = r.x + lane_id(); e0 < r.y; e0 += 64u * kScatterUnroll) {
-                uint2 te[kScatterUnroll];  // (target, in-CSR slot)
-                unsigned slot[kScatterUnroll];
-#pragma unroll
-                for (unsigned u = 0; u < kScatterUnroll; ++u)
-                    if (e0 + 64u * u < r.y) te[u] = ent[e0 + 64u * u];
-#pragma unroll
-                for (unsigned u = 0; u < kScatterUnroll; ++u)
-                    if (e0 + 64u * u < r.y) slot[u] = atomicAdd(&cnt[te[u].x], 1u);
-#pragma unroll
-                for (unsigned u = 0; u < kScatterUnroll; ++u) {
-                    if (e0 + 64u * u >= r.y) continue;
-                    if (slot[u] < kBucketCap)
-                        bkt[std::size_t(te[u].x) * kBucketCap + slot[u]] = make_uint4(te[u].y, tag, vlo, vhi);
-                    else if (slot[u] == kBucketCap)
-                        atomicAdd(D.overflow, 1u);
-                }
-            }
-        };
+        // The emitter's out-rows (a neuron's exc, inh and n2a rows; an
+        // astrocyte's a2n row) are walked as one sequence of edge slots, each
+        // lane appending kScatterUnroll slots per iteration: their loads, then
+        // their returning atomics, then their stores. An append is a chain of
+        // dependent memory operations; this keeps several chains in flight per
+        // lane and walks all rows in one round instead of one round per row.
+        // Bucket order does not matter: buckets are sorted by in-CSR slot when
+        // they are read.
+        const uint2* ent0 = is_neuron ? D.out_ent[T_EXC] : D.out_ent[T_SIC];
+        unsigned* cnt0 = is_neuron ? D.bcnt[T_EXC] : D.bcnt[T_SIC];
+        uint4* bkt0 = is_neuron ? D.bent[T_EXC] : D.bent[T_SIC];
+        const uint2 r0 = (is_neuron ? D.out_rs[T_EXC] : D.out_rs[T_SIC])[src];
+        uint2 r1 = make_uint2(0u, 0u), r2 = make_uint2(0u, 0u);
         if (is_neuron) {
-            walk(D.out_rs[T_EXC], D.out_ent[T_EXC], D.bcnt[T_EXC], D.bent[T_EXC]);
-            walk(D.out_rs[T_INH], D.out_ent[T_INH], D.bcnt[T_INH], D.bent[T_INH]);
-            walk(D.out_rs[T_E2A], D.out_ent[T_E2A], D.bcnt[T_E2A], D.bent[T_E2A]);
-        } else {
-            walk(D.out_rs[T_SIC], D.out_ent[T_SIC], D.bcnt[T_SIC], D.bent[T_SIC]);
+            r1 = D.out_rs[T_INH][src];
+            r2 = D.out_rs[T_E2A][src];
+        }
+        const unsigned n0 = r0.y - r0.x, n01 = n0 + (r1.y - r1.x), n = n01 + (r2.y - r2.x);
+        for (unsigned s0 = lane_id(); s0 < n; s0 += 64u * kScatterUnroll) {
+            uint2 te[kScatterUnroll];  // (target, in-CSR slot)
+            unsigned slot[kScatterUnroll];
+            int ty[kScatterUnroll];
+#pragma unroll
+            for (unsigned u = 0; u < kScatterUnroll; ++u) {
+                const unsigned sl = s0 + 64u * u;
+                if (sl >= n) continue;
+                ty[u] = sl < n0 ? 0 : (sl < n01 ? 1 : 2);
+                const std::size_t e = ty[u] == 0 ? r0.x + sl : (ty[u] == 1 ? r1.x + (sl - n0) : r2.x + (sl - n01));
+                const uint2* ent = ty[u] == 0 ? ent0 : (ty[u] == 1 ? D.out_ent[T_INH] : D.out_ent[T_E2A]);
+                te[u] = ent[e];
+            }
+#pragma unroll
+            for (unsigned u = 0; u < kScatterUnroll; ++u) {
+                if (s0 + 64u * u >= n) continue;
+                unsigned* cnt = ty[u] == 0 ? cnt0 : (ty[u] == 1 ? D.bcnt[T_INH] : D.bcnt[T_E2A]);
+                slot[u] = atomicAdd(&cnt[te[u].x], 1u);
+            }
+#pragma unroll
+            for (unsigned u = 0; u < kScatterUnroll; ++u) {
+                if (s0 + 64u * u >= n) continue;
+                uint4* bkt = ty[u] == 0 ? bkt0 : (ty[u] == 1 ? D.bent[T_INH] : D.bent[T_E2A]);
+                if (slot[u] < kBucketCap)
+                    bkt[std::size_t(te[u].x) * kBucketCap + slot[u]] = make_uint4(te[u].y, tag, vlo, vhi);
+                else if (slot[u] == kBucketCap)
+                    atomicAdd(D.overflow, 1u);
+            }
         }
     }
 }
