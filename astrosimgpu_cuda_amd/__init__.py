@@ -63,7 +63,7 @@ class _NetInfo(C.Structure):
                 ("n_astro", C.c_uint32), ("n_neurons", C.c_uint32), ("p_primary", C.c_double),
                 ("n_exc_edges", C.c_uint64), ("n_inh_edges", C.c_uint64),
                 ("n_e2a_edges", C.c_uint64), ("n_sic_edges", C.c_uint64),
-                ("build_seconds", C.c_double)]
+                ("build_seconds", C.c_double), ("built_on_gpu", C.c_int32)]
 
 
 class _BatchCap(C.Structure):
@@ -80,7 +80,7 @@ class _BatchInfo(C.Structure):
 class _ServerConfig(C.Structure):
     _fields_ = [("dt", C.c_double), ("block", C.c_int32), ("hbm_fraction", C.c_double),
                 ("fp64_share", C.c_double), ("build_threads", C.c_int32), ("blocks_per_round", C.c_int32),
-                ("max_members", C.c_int32)]
+                ("max_members", C.c_int32), ("setup_where", C.c_int32)]
 
 
 class _Request(C.Structure):
@@ -108,6 +108,7 @@ _lib = None
 SIGNATURES = [
     ("as_config_default", None, [C.POINTER(_Config)]),
     ("as_network_build", C.c_int, [C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("as_network_build_on", C.c_int, [C.c_uint32, C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]),
     ("as_network_info_get", C.c_int, [C.c_void_p, C.POINTER(_NetInfo)]),
     ("as_network_csr", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     ("as_network_params", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
@@ -214,10 +215,16 @@ class Network:
     CSR_NAMES = {"exc": 0, "inh": 1, "sic": 2, "e2a": 3}
     PARAM_NAMES = {"Ca_tot": 0, "IP3_0": 1, "tau_IP3": 2, "delta_IP3": 3, "V_reset": 4, "b": 5}
 
-    def __init__(self, n_total: int, seed: int = 1):
+    BUILD_WHERE = {"host": 0, "gpu": 1, "gpu-only": 2}
+
+    def __init__(self, n_total: int, seed: int = 1, setup: str = "host"):
+        """``setup``: "host" walks the setup stream sequentially; "gpu" generates
+        the same connectivity with the parallel generator on the current GPU
+        (the host walk when it declines a network); "gpu-only" raises instead."""
         lib = load()
         h = C.c_void_p()
-        _check(lib.as_network_build(C.c_uint32(n_total), C.c_uint64(seed), C.byref(h)))
+        _check(lib.as_network_build_on(C.c_uint32(n_total), C.c_uint64(seed),
+                                       C.c_int32(self.BUILD_WHERE[setup]), C.byref(h)))
         self._h = h
         info = _NetInfo()
         _check(lib.as_network_info_get(h, C.byref(info)))
@@ -228,6 +235,7 @@ class Network:
         self.n_edges = {"exc": info.n_exc_edges, "inh": info.n_inh_edges,
                         "e2a": info.n_e2a_edges, "sic": info.n_sic_edges}
         self.build_seconds = info.build_seconds
+        self.built_on_gpu = bool(info.built_on_gpu)
 
     @property
     def handle(self):

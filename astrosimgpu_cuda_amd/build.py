@@ -46,8 +46,10 @@ DEVICE = ["-x", "hip", f"--offload-arch={ARCH}",
 # (correctly rounded either way; the instruction avoids a libm call).
 HOST = ["-D__HIP_PLATFORM_AMD__", "-mfma", "-I", str(ROCM / "include")]
 
-LIB_SOURCES = ["as_setup.cpp", "as_engine.cpp", "as_capi.cpp", "as_plain.hip", "as_batch.hip", "as_server.cpp"]
-HEADERS = ["as_dmath.h", "as_rng.h", "as_model.h", "as_model32.h", "as_setup.h", "as_engine.h", "as_batch.h"]
+LIB_SOURCES = ["as_setup.cpp", "as_engine.cpp", "as_capi.cpp", "as_plain.hip", "as_batch.hip", "as_server.cpp",
+               "as_gpubuild.hip"]
+HEADERS = ["as_dmath.h", "as_rng.h", "as_model.h", "as_model32.h", "as_setup.h", "as_engine.h", "as_batch.h",
+           "as_gpubuild.h", "as_gpubuild_run.h"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:
@@ -87,7 +89,8 @@ def build_library(incremental: bool = False, jobs: int = 8) -> None:
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs[:-1])])
     os.replace(tmp, LIB)
     tmp = SELFTEST.with_suffix(".so.tmp")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), str(objs[-1])])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), str(objs[-1]),
+          "-L", str(LIB.parent), "-lastrosim", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, SELFTEST)
     cli_obj = _compile("as_cli.cpp", incremental)
     _run([HIPCC, "-o", str(CLI), str(cli_obj), "-L", str(LIB.parent), "-lastrosim",

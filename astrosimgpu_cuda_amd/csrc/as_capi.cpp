@@ -13,6 +13,7 @@
 
 struct as_network {
     astrosim::Network net;
+    bool built_on_gpu = false;
 };
 
 struct as_sim {
@@ -93,6 +94,24 @@ as_status as_network_build(uint32_t n_total, uint64_t seed, as_network** out) {
     });
 }
 
+as_status as_network_build_on(uint32_t n_total, uint64_t seed, int32_t where, as_network** out) {
+    return guarded([&] {
+        require(out != nullptr, "out is NULL");
+        require(where >= AS_BUILD_HOST && where <= AS_BUILD_GPU_ONLY, "where must be 0..2");
+        *out = nullptr;
+        std::unique_ptr<as_network> n(new as_network);
+        std::string why;
+        if (where != AS_BUILD_HOST && astrosim::build_network_gpu(n_total, seed, n->net, &why)) {
+            n->built_on_gpu = true;
+        } else {
+            if (where == AS_BUILD_GPU_ONLY)
+                throw astrosim::Error(AS_ERR_INVALID, "GPU network build declined: " + why);
+            astrosim::build_network(n_total, seed, n->net);
+        }
+        *out = n.release();
+    });
+}
+
 as_status as_network_info_get(const as_network* net, as_network_info* info) {
     return guarded([&] {
         require(net && info, "NULL argument");
@@ -108,6 +127,7 @@ as_status as_network_info_get(const as_network* net, as_network_info* info) {
         info->n_e2a_edges = net->net.inc_e2a.n_edges();
         info->n_sic_edges = net->net.inc_sic.n_edges();
         info->build_seconds = net->net.build_seconds;
+        info->built_on_gpu = net->built_on_gpu ? 1 : 0;
     });
 }
 

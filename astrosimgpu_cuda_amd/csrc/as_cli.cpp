@@ -34,6 +34,7 @@ int main(int argc, char** argv) {
     bool no_stp = false;
     std::string dump_edges;
     int precision = AS_FP64, method = AS_METHOD_PLAIN;
+    int setup_where = -1;  // default: GPU generator, host walk for --dump-edges
 
     for (int i = 1; i < argc; ++i) {
         auto arg = [&](const char* name) { return std::strcmp(argv[i], name) == 0; };
@@ -59,20 +60,27 @@ int main(int argc, char** argv) {
             if (v == "plain") method = AS_METHOD_PLAIN;
             else if (v == "blocked") method = AS_METHOD_BLOCKED;
             else { std::fprintf(stderr, "unknown --method %s (plain|blocked)\n", v.c_str()); return 1; }
+        } else if (arg("--setup")) {
+            const std::string v = next();
+            if (v == "host") setup_where = AS_BUILD_HOST;
+            else if (v == "gpu") setup_where = AS_BUILD_GPU;
+            else { std::fprintf(stderr, "unknown --setup %s (host|gpu)\n", v.c_str()); return 1; }
         } else if (arg("--help") || arg("-h")) {
             std::printf(
                 "astrosim_fused --n N_total [--seed S] [--dt 0.1] [--substeps 1] "
                 "[--pre-ms 1000] [--sim-ms 5000] [--block 128]\n"
                 "         [--w-n2a W] [--w-a2n W] [--no-stp] [--dump-edges PATH]\n"
-                "         [--precision fp64|fp32c] [--method plain|blocked]\n"
+                "         [--precision fp64|fp32c] [--method plain|blocked] [--setup host|gpu]\n"
                 "N_total is split by the fixed-in-degree scaling law: N_E=0.8*N, N_I=0.2*N,\n"
                 "N_A=N/5, p_primary=80/N_E. --w-n2a/--w-a2n override the neuron->astrocyte /\n"
                 "astrocyte->neuron coupling weight (default 0.2 / 1.0); 0 disconnects one\n"
                 "direction. --no-stp disables Tsodyks-Markram STP (multiplier 1.0).\n"
                 "--dump-edges PATH writes the connectivity as CSV (type,source,target) and\n"
-                "exits before touching the GPU. --block is the workgroup size (multiple of 64).\n"
+                "exits (on the host unless --setup gpu is given). --block is the workgroup size (multiple of 64).\n"
                 "--precision fp32c runs FP32 state with compensated Ca/IP3/h; --method blocked\n"
-                "advances one minimum-delay block of steps per launch (MI355X gfx950).\n");
+                "advances one minimum-delay block of steps per launch (MI355X gfx950).\n"
+                "--setup gpu (the default except with --dump-edges) generates the connectivity\n"
+                "on the GPU, identical to --setup host's sequential walk.\n");
             return 0;
         } else {
             std::fprintf(stderr, "unknown argument: %s (try --help)\n", argv[i]);
@@ -80,9 +88,10 @@ int main(int argc, char** argv) {
         }
     }
 
+    if (setup_where < 0) setup_where = dump_edges.empty() ? AS_BUILD_GPU : AS_BUILD_HOST;
     as_network* net = nullptr;
     const auto t_conn0 = std::chrono::steady_clock::now();
-    if (as_network_build(n_total, seed, &net) != AS_OK) {
+    if (as_network_build_on(n_total, seed, setup_where, &net) != AS_OK) {
         std::fprintf(stderr, "network: %s\n", as_last_error());
         return 1;
     }

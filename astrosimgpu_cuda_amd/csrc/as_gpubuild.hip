@@ -1,0 +1,150 @@
+// as_gpubuild.hip -- the parallel connectivity generator (as_gpubuild.h,
+// as_gpubuild_run.h) on the current GPU: the decode, chain, path, repair and
+// row walks as kernels of one lane per item; hipCUB for the exclusive scans
+// and the stable radix sorts that invert the generation-ordered edges into
+// target-major rows (src/astrosim_fused.cu:307-314). Device memory comes from
+// the stream-ordered allocator on a stream of the build's own, so builds on
+// several host threads neither share state nor wait for other streams'
+// work (a plain hipFree would wait for the whole device).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <chrono>
+#include <stdexcept>
+#include <string>
+#include <thread>
+
+#include "as_gpubuild_run.h"
+#include "as_setup.h"
+
+namespace astrosim {
+namespace {
+
+using namespace gb;
+
+template <class F>
+__global__ __launch_bounds__(256) void for_each_kernel(F f, u64 n) {
+    const u64 i = u64(blockIdx.x) * 256u + threadIdx.x;
+    if (i < n) f(i);
+}
+
+struct HipFailure {
+    std::string what;
+};
+
+#define GB_CHECK(expr)                                                                            \
+    do {                                                                                          \
+        const hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                                     \
+            throw HipFailure{std::string(#expr) + ": " + hipGetErrorString(e_) + " (" __FILE__ ":" + \
+                             std::to_string(__LINE__) + ")"};                                     \
+    } while (0)
+
+class HipBackend {
+public:
+    HipBackend() { GB_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking)); }
+    ~HipBackend() {
+        (void)hipStreamSynchronize(s_);
+        (void)hipStreamDestroy(s_);
+    }
+    template <class T>
+    T* alloc(u64 n) {
+        void* p = nullptr;
+        GB_CHECK(hipMallocAsync(&p, n * sizeof(T), s_));
+        return static_cast<T*>(p);
+    }
+    void release(void* p) { (void)hipFreeAsync(p, s_); }
+    void h2d(void* d, const void* h, u64 bytes) {
+        if (bytes == 0) return;
+        GB_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s_));
+        GB_CHECK(hipStreamSynchronize(s_));
+    }
+    void d2h(void* h, const void* d, u64 bytes) {
+        if (bytes == 0) return;
+        GB_CHECK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s_));
+        GB_CHECK(hipStreamSynchronize(s_));
+    }
+    void zero(void* d, u64 bytes) {
+        if (bytes) GB_CHECK(hipMemsetAsync(d, 0, bytes, s_));
+    }
+    template <class F>
+    void run(u64 n, const F& f) {
+        if (n == 0) return;
+        const u64 blocks = (n + 255) / 256;
+        if (blocks >= (u64(1) << 31)) throw HipFailure{"launch too large"};
+        hipLaunchKernelGGL(for_each_kernel<F>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s_, f, n);
+        GB_CHECK(hipGetLastError());
+    }
+    void scan_u64(u64* p, u64 n) { scan(p, n); }
+    void scan_u32(u32* p, u64 n) { scan(p, n); }
+    void sort_pairs(const u32* kin, const u32* vin, u32* kout, u32* vout, u64 n, int bits) {
+        size_t bytes = 0;
+        GB_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, static_cast<int>(n), 0,
+                                                    bits, s_));
+        void* tmp = alloc<char>(bytes + 16);
+        GB_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, kin, kout, vin, vout, static_cast<int>(n), 0, bits,
+                                                    s_));
+        release(tmp);
+    }
+
+private:
+    template <class T>
+    void scan(T* p, u64 n) {
+        if (n >= (u64(1) << 31)) throw HipFailure{"scan too large"};
+        size_t bytes = 0;
+        GB_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, p, p, static_cast<int>(n), s_));
+        void* tmp = alloc<char>(bytes + 16);
+        GB_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, p, p, static_cast<int>(n), s_));
+        release(tmp);
+    }
+    hipStream_t s_ = nullptr;
+};
+
+}  // namespace
+
+bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::string* why) {
+    std::string reason;
+    auto decline = [&](const std::string& r) {
+        if (why) *why = r;
+        return false;
+    };
+    if (n_total == 0 || n_total > kMaxNTotal) throw std::invalid_argument("N_total out of range");
+    const ScalingConfig cfg = make_scaling(n_total);
+    // the method needs geometric skips (p < 1) and the rejection-sampled pools
+    if (cfg.n_exc <= 80 || cfg.n_astro <= 4 * ScalingConfig::kPoolSize) return decline("network too small");
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) return decline("no GPU");
+
+    const auto t0 = std::chrono::steady_clock::now();
+    Network out;
+    out.cfg = cfg;
+    out.seed = seed;
+    // the per-cell parameters come from their own stream: drawn meanwhile
+    std::thread params([&] { draw_cell_params(cfg, seed, out); });
+    bool ok = false;
+    gb::BuildStats st;
+    try {
+        HostRng rng(seed, kSetupStream);
+        rng.skip_normals(4ull * cfg.n_astro + 2ull * cfg.n_neurons());
+        const std::vector<index_t> pool = make_pools(cfg, rng);
+        const auto t_pools = std::chrono::steady_clock::now();
+        out.phase_seconds[0] = std::chrono::duration<double>(t_pools - t0).count();
+        HipBackend bk;
+        ok = gb::build_connectivity(bk, cfg, seed, rng.counter(), pool, ScalingConfig::kPoolSize, out, st, reason);
+    } catch (const HipFailure& f) {
+        params.join();
+        throw std::runtime_error("GPU network build: " + f.what);
+    } catch (...) {
+        params.join();
+        throw;
+    }
+    params.join();
+    if (!ok) return decline(reason);
+    const auto t1 = std::chrono::steady_clock::now();
+    out.build_seconds = std::chrono::duration<double>(t1 - t0).count();
+    out.phase_seconds[1] = st.seconds;
+    net = std::move(out);
+    return true;
+}
+
+}  // namespace astrosim

@@ -6,14 +6,25 @@
 // bit for bit over whole input ranges (subnormal and overflowing results,
 // signed zeros, quotients next to rounding midpoints). It also evaluates the
 // Gaussian draws: the FP64 one on both sides, and the FP32 one FP32 state
-// uses (device only), for a bound on their difference.
+// uses (device only), for a bound on their difference. And it runs the
+// parallel connectivity generator of as_gpubuild_run.h over a host backend
+// (loops for kernels, std::stable_sort for the radix sort), so that its
+// method is checked against the sequential builder without a GPU.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
 
 #include "as_engine.h"
+#include "as_gpubuild_run.h"
 #include "as_model.h"
 #include "as_model32.h"
+#include "as_setup.h"
 
 namespace {
 
@@ -72,4 +83,119 @@ extern "C" int as_selftest_eval_device(int fn, const double* x, const double* y,
     (void)hipFree(dy);
     (void)hipFree(dout);
     return e == hipSuccess ? 0 : static_cast<int>(e);
+}
+
+namespace {
+
+using namespace astrosim::gb;
+
+// Host stand-in for the GPU backend: every "kernel" is a loop over its items
+// (split over a few threads; the bodies use atomics where items share data).
+class HostBackend {
+public:
+    template <class T>
+    T* alloc(u64 n) {
+        return static_cast<T*>(::operator new(n * sizeof(T)));
+    }
+    void release(void* p) { ::operator delete(p); }
+    void h2d(void* d, const void* h, u64 bytes) { if (bytes) std::memcpy(d, h, bytes); }
+    void d2h(void* h, const void* d, u64 bytes) { if (bytes) std::memcpy(h, d, bytes); }
+    void zero(void* d, u64 bytes) { if (bytes) std::memset(d, 0, bytes); }
+    template <class F>
+    void run(u64 n, const F& f) {
+        const u64 T = n < 4096 ? 1 : 8;
+        std::vector<std::thread> th;
+        for (u64 t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                for (u64 i = n * t / T; i < n * (t + 1) / T; ++i) f(i);
+            });
+        for (auto& x : th) x.join();
+    }
+    void scan_u64(u64* p, u64 n) { scan(p, n); }
+    void scan_u32(u32* p, u64 n) { scan(p, n); }
+    void sort_pairs(const u32* kin, const u32* vin, u32* kout, u32* vout, u64 n, int) {
+        std::vector<u64> idx(n);
+        std::iota(idx.begin(), idx.end(), u64(0));
+        std::stable_sort(idx.begin(), idx.end(), [&](u64 a, u64 b) { return kin[a] < kin[b]; });
+        for (u64 i = 0; i < n; ++i) {
+            kout[i] = kin[idx[i]];
+            vout[i] = vin[idx[i]];
+        }
+    }
+
+private:
+    template <class T>
+    void scan(T* p, u64 n) {
+        T run = 0;
+        for (u64 i = 0; i < n; ++i) {
+            const T v = p[i];
+            p[i] = run;
+            run += v;
+        }
+    }
+};
+
+bool same_csr(const astrosim::HostCSR& a, const astrosim::HostCSR& b) {
+    return a.row_start == b.row_start && a.src.size() == b.src.size() &&
+           std::equal(a.src.begin(), a.src.end(), b.src.begin());
+}
+
+}  // namespace
+
+// Builds the network (n_total, seed) with the sequential host builder and
+// again with the parallel generator -- on the host backend (where = 0) or on
+// the current GPU through build_network_gpu (where = 1) -- and compares the
+// four CSRs (and, for where = 1, the per-cell parameters). Returns 0 when
+// identical, 1 when different, 2 when the parallel generator declined, -1
+// on an exception. stats (16 doubles, may be NULL): counters, chains,
+// ambiguous skips, host rows of passes 0-2, records of passes 0 and 2,
+// repairs of passes 0 and 2, largest index shift, generator seconds, host
+// builder seconds, edges compared, mask of what differs (1 exc, 2 inh, 4 sic,
+// 8 e2a, 16 parameters).
+extern "C" int as_selftest_build_compare(uint32_t n_total, uint64_t seed, int where, double* stats) {
+    using namespace astrosim;
+    try {
+        Network ref;
+        const auto t0 = std::chrono::steady_clock::now();
+        build_network(n_total, seed, ref);
+        const double t_host = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        Network got;
+        BuildStats st;
+        std::string why;
+        bool ok;
+        if (where == 0) {
+            const ScalingConfig cfg = make_scaling(n_total);
+            got.cfg = cfg;
+            got.seed = seed;
+            HostRng rng(seed, kSetupStream);
+            rng.skip_normals(4ull * cfg.n_astro + 2ull * cfg.n_neurons());
+            const std::vector<index_t> pool = make_pools(cfg, rng);
+            const u32 take = static_cast<u32>(std::min<index_t>(ScalingConfig::kPoolSize, cfg.n_astro));
+            HostBackend bk;
+            ok = build_connectivity(bk, cfg, seed, rng.counter(), pool, take, got, st, why);
+        } else {
+            ok = build_network_gpu(n_total, seed, got, &why);
+            st.seconds = got.build_seconds;
+        }
+        if (stats) {
+            const double v[16] = {double(st.counters), double(st.chains), double(st.ambiguous),
+                                  double(st.host_rows[0]), double(st.host_rows[1]), double(st.host_rows[2]),
+                                  double(st.records[0]), double(st.records[2]), double(st.repairs[0]),
+                                  double(st.repairs[2]), double(st.max_shift), st.seconds, t_host,
+                                  double(ref.inc_exc.n_edges() + ref.inc_inh.n_edges() + ref.inc_sic.n_edges() +
+                                         ref.inc_e2a.n_edges()),
+                                  0.0, 0.0};
+            std::memcpy(stats, v, sizeof v);
+        }
+        if (!ok) return 2;
+        int diff = (same_csr(ref.inc_exc, got.inc_exc) ? 0 : 1) | (same_csr(ref.inc_inh, got.inc_inh) ? 0 : 2) |
+                   (same_csr(ref.inc_sic, got.inc_sic) ? 0 : 4) | (same_csr(ref.inc_e2a, got.inc_e2a) ? 0 : 8);
+        if (where == 1 && !(ref.Ca_tot == got.Ca_tot && ref.IP3_0 == got.IP3_0 && ref.tau_IP3 == got.tau_IP3 &&
+                            ref.delta_IP3 == got.delta_IP3 && ref.V_reset == got.V_reset && ref.b == got.b))
+            diff |= 16;
+        if (stats) stats[14] = diff;
+        return diff == 0 ? 0 : 1;
+    } catch (...) {
+        return -1;
+    }
 }

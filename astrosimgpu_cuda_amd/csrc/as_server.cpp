@@ -91,6 +91,7 @@ public:
             AS_HIP_CHECK(hipStreamCreateWithFlags(&stream_[i], hipStreamNonBlocking));
             batch_[i] = make_batch(rp_[i], cap_[i], stream_[i]);
         }
+        setup_where_ = c.setup_where;
         const int nb = std::max(1, c.build_threads);
         for (int i = 0; i < nb; ++i) builders_.emplace_back([this] { build_loop(); });
         driver_ = std::thread([this] { drive_loop(); });
@@ -194,7 +195,10 @@ private:
             auto net = std::make_unique<Network>();
             bool ok = true;
             try {
-                build_network(q->in.n_total, q->in.seed, *net);
+                if (setup_where_ == AS_BUILD_HOST || !build_network_gpu(q->in.n_total, q->in.seed, *net, nullptr)) {
+                    if (setup_where_ == AS_BUILD_GPU_ONLY) throw std::runtime_error("GPU build declined");
+                    build_network(q->in.n_total, q->in.seed, *net);
+                }
             } catch (...) {
                 ok = false;
             }
@@ -345,6 +349,7 @@ private:
     std::size_t finished_ = 0;
     bool stop_ = false;
     std::string error_;
+    int setup_where_ = AS_BUILD_GPU;
     std::vector<std::thread> builders_;
     std::thread driver_;
     std::uint64_t peak_active_ = 0, peak_cells_ = 0;
@@ -385,6 +390,7 @@ void as_server_config_default(as_server_config* c) {
     c->build_threads = 8;
     c->blocks_per_round = 4;
     c->max_members = 512;
+    c->setup_where = AS_BUILD_GPU;
 }
 
 as_status as_server_create(const as_server_config* cfg, as_server** out) {

@@ -1,6 +1,8 @@
 // as_setup.cpp -- host network construction (see as_setup.h).
 #include "as_setup.h"
 
+#include "as_gpubuild.h"
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -202,17 +204,13 @@ private:
     }
 
     // g = floor(log1p(-u) / log1p(-p)) with glibc's log1p, as the reference.
-    // Fast path: u lies on the 2^-53 grid, so 1 - u is exact and log1p(-u) =
-    // log(1 - u) exactly; as_log evaluates it within a few ulps and glibc's
-    // log1p within one, so the two quotients differ by less than 2^-46 of
-    // their size. When the fast quotient is farther than that from an
-    // integer, both floors agree; otherwise glibc decides.
+    // The fast evaluation (gb::skip_fast) decides unless the quotient lies
+    // too close to an integer; then glibc's log1p decides.
     static std::uint32_t skip_of(double u, double log1m_p) {
-        const double q = as_log(1.0 - u) / log1m_p;
-        const double margin = std::fabs(q) * 0x1p-46 + 0x1p-1000;
-        double g = std::floor(q - margin);
-        if (g != std::floor(q + margin)) g = std::floor(std::log1p(-u) / log1m_p);
-        return g >= 4294967295.0 ? 0xFFFFFFFFu : static_cast<std::uint32_t>(g);
+        std::uint32_t g;
+        if (gb::skip_fast(u, log1m_p, g)) return g;
+        const double q = std::floor(std::log1p(-u) / log1m_p);
+        return q >= 4294967295.0 ? 0xFFFFFFFFu : static_cast<std::uint32_t>(q);
     }
 
     // Every counter is decoded all three ways the walk may read it; the
@@ -374,6 +372,59 @@ struct SourceMajor {
 
 }  // namespace
 
+std::vector<index_t> make_pools(const ScalingConfig& cfg, HostRng& rng) {
+    // Astrocyte pools, `take` distinct astrocytes per postsynaptic neuron
+    // (src/astrosim_fused.cu:651-671).
+    const index_t n_astro = cfg.n_astro, n_neurons = cfg.n_neurons();
+    const int take = std::min<int>(ScalingConfig::kPoolSize, static_cast<int>(n_astro));
+    std::vector<index_t> pool(std::size_t(n_neurons) * take);
+    std::vector<index_t> candidates;
+    for (index_t post = 0; post < n_neurons; ++post) {
+        index_t* pl = &pool[std::size_t(post) * take];
+        if (static_cast<std::int64_t>(take) * 4 >= static_cast<std::int64_t>(n_astro)) {
+            candidates.resize(n_astro);
+            for (index_t i = 0; i < n_astro; ++i) candidates[i] = i;
+            for (int k = 0; k < take; ++k) {
+                const auto pick = static_cast<std::size_t>(rng.uniform() * (candidates.size() - k)) + k;
+                std::swap(candidates[k], candidates[std::min(pick, candidates.size() - 1)]);
+                pl[k] = candidates[k];
+            }
+        } else {
+            int have = 0;
+            while (have < take) {
+                const auto cand = static_cast<index_t>(rng.uniform() * static_cast<double>(n_astro)) % n_astro;
+                bool dup = false;
+                for (int q = 0; q < have; ++q) dup = dup || (pl[q] == cand);
+                if (!dup) pl[have++] = cand;
+            }
+        }
+    }
+    return pool;
+}
+
+void draw_cell_params(const ScalingConfig& cfg, std::uint64_t seed, Network& net) {
+    // A fresh stream from counter 0 (src/astrosim_fused.cu:917-933).
+    const Biology bio;
+    const index_t n_astro = cfg.n_astro, n_neurons = cfg.n_neurons(), n_exc = cfg.n_exc;
+    HostRng prng(seed, kSetupStream);
+    net.Ca_tot.resize(n_astro); net.IP3_0.resize(n_astro);
+    net.tau_IP3.resize(n_astro); net.delta_IP3.resize(n_astro);
+    for (index_t i = 0; i < n_astro; ++i) {
+        net.Ca_tot[i] = draw_scaled(bio.Ca_tot, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, prng);
+        net.IP3_0[i] = draw_scaled(bio.IP3_0, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, prng);
+        net.tau_IP3[i] = draw_scaled(bio.tau_IP3, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, prng);
+        net.delta_IP3[i] = draw_scaled(bio.delta_IP3, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, prng);
+    }
+    net.V_reset.resize(n_neurons); net.b.resize(n_neurons);
+    for (index_t i = 0; i < n_neurons; ++i) {
+        const bool exc = i < n_exc;
+        net.V_reset[i] = draw_scaled(exc ? bio.e_V_reset : bio.i_V_reset, bio.randomize_neuron,
+                                     bio.rn_lower, bio.rn_upper, bio.rn_var, prng);
+        net.b[i] = draw_scaled(exc ? bio.e_b : bio.i_b, bio.randomize_neuron, bio.rn_lower,
+                               bio.rn_upper, bio.rn_var, prng);
+    }
+}
+
 void build_network(index_t n_total, std::uint64_t seed, Network& net) {
     if (n_total == 0) throw std::invalid_argument("N_total must be at least 1");
     if (n_total > kMaxNTotal) throw std::invalid_argument("N_total exceeds the 32-bit edge-id limit");
@@ -392,34 +443,8 @@ void build_network(index_t n_total, std::uint64_t seed, Network& net) {
     // (src/astrosim_fused.cu:644-645).
     rng.skip_normals(4ull * n_astro + 2ull * n_neurons);
 
-    // Astrocyte pools, `take` distinct astrocytes per postsynaptic neuron
-    // (src/astrosim_fused.cu:651-671).
     const int take = std::min<int>(ScalingConfig::kPoolSize, static_cast<int>(n_astro));
-    std::vector<index_t> pool(std::size_t(n_neurons) * take);
-    {
-        std::vector<index_t> candidates;
-        for (index_t post = 0; post < n_neurons; ++post) {
-            index_t* pl = &pool[std::size_t(post) * take];
-            if (static_cast<std::int64_t>(take) * 4 >= static_cast<std::int64_t>(n_astro)) {
-                candidates.resize(n_astro);
-                for (index_t i = 0; i < n_astro; ++i) candidates[i] = i;
-                for (int k = 0; k < take; ++k) {
-                    const auto pick = static_cast<std::size_t>(rng.uniform() * (candidates.size() - k)) + k;
-                    std::swap(candidates[k], candidates[std::min(pick, candidates.size() - 1)]);
-                    pl[k] = candidates[k];
-                }
-            } else {
-                int have = 0;
-                while (have < take) {
-                    const auto cand =
-                        static_cast<index_t>(rng.uniform() * static_cast<double>(n_astro)) % n_astro;
-                    bool dup = false;
-                    for (int q = 0; q < have; ++q) dup = dup || (pl[q] == cand);
-                    if (!dup) pl[have++] = cand;
-                }
-            }
-        }
-    }
+    std::vector<index_t> pool = make_pools(cfg, rng);
     const auto t_pools = std::chrono::steady_clock::now();
 
     // Edge generation. Excitatory rows: pre over [0, n_exc), posts over one
@@ -541,25 +566,8 @@ void build_network(index_t n_total, std::uint64_t seed, Network& net) {
     net.phase_seconds[1] = std::chrono::duration<double>(t_gen - t_pools).count();
     net.phase_seconds[2] = std::chrono::duration<double>(t1 - t_gen).count();
 
-    // ---- per-cell parameters: a fresh stream from counter 0 (src/astrosim_fused.cu:917-933) ----
-    const Biology bio;
-    HostRng prng(seed, kSetupStream);
-    net.Ca_tot.resize(n_astro); net.IP3_0.resize(n_astro);
-    net.tau_IP3.resize(n_astro); net.delta_IP3.resize(n_astro);
-    for (index_t i = 0; i < n_astro; ++i) {
-        net.Ca_tot[i] = draw_scaled(bio.Ca_tot, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, prng);
-        net.IP3_0[i] = draw_scaled(bio.IP3_0, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, prng);
-        net.tau_IP3[i] = draw_scaled(bio.tau_IP3, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, prng);
-        net.delta_IP3[i] = draw_scaled(bio.delta_IP3, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, prng);
-    }
-    net.V_reset.resize(n_neurons); net.b.resize(n_neurons);
-    for (index_t i = 0; i < n_neurons; ++i) {
-        const bool exc = i < n_exc;
-        net.V_reset[i] = draw_scaled(exc ? bio.e_V_reset : bio.i_V_reset, bio.randomize_neuron,
-                                     bio.rn_lower, bio.rn_upper, bio.rn_var, prng);
-        net.b[i] = draw_scaled(exc ? bio.e_b : bio.i_b, bio.randomize_neuron, bio.rn_lower,
-                               bio.rn_upper, bio.rn_var, prng);
-    }
+    // ---- per-cell parameters ----
+    draw_cell_params(cfg, seed, net);
     net.phase_seconds[3] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
 }
 

@@ -89,6 +89,21 @@ struct Network {
 // sizes the 32-bit edge ids cannot hold.
 void build_network(index_t n_total, std::uint64_t seed, Network& net);
 
+// The same network with the connectivity generated on the current GPU
+// (as_gpubuild.hip); identical bit for bit. Returns false, with the reason
+// in *why and `net` untouched, for networks that method declines (too small
+// for its draw statistics, or a rare draw pattern it does not resolve); the
+// caller then uses build_network.
+bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::string* why);
+
+// Astrocyte pools drawn from `rng` ([neuron][min(5, n_astro)] astrocyte ids).
+// Reference: src/astrosim_fused.cu:651-671.
+std::vector<index_t> make_pools(const ScalingConfig& cfg, HostRng& rng);
+
+// Per-cell parameters (Ca_tot, IP3_0, tau_IP3, delta_IP3, V_reset, b) from a
+// fresh setup stream. Reference: src/astrosim_fused.cu:917-933.
+void draw_cell_params(const ScalingConfig& cfg, std::uint64_t seed, Network& net);
+
 // Writes the edge list as CSV (type,source,target), astrocytes prefixed 'A',
 // inhibitory sources shifted to global ids. Reference: src/astrosim_fused.cu:873-902.
 bool dump_edges_csv(const Network& net, const std::string& path);

@@ -30,10 +30,11 @@ def ensemble_sizes(n: int = 256, lo: int = 500, hi: int = 200_000, seed: int = 3
     return [int(round(math.exp(a + splitmix_uniform(seed, 0, i) * (b - a)))) for i in range(n)]
 
 
-def build_networks(sizes: list[int], seed0: int = 1000, workers: int = 8) -> list[Network]:
-    """Builds network i with seed seed0 + i (host; the C calls run in parallel)."""
+def build_networks(sizes: list[int], seed0: int = 1000, workers: int = 8, setup: str = "host") -> list[Network]:
+    """Builds network i with seed seed0 + i (the C calls run in parallel);
+    ``setup`` as for Network ("gpu": the parallel generator on the GPU)."""
     with ThreadPoolExecutor(max_workers=workers) as ex:
-        return list(ex.map(lambda a: Network(a[1], seed0 + a[0]), enumerate(sizes)))
+        return list(ex.map(lambda a: Network(a[1], seed0 + a[0], setup=setup), enumerate(sizes)))
 
 
 def run_batched(nets: list[Network], steps: int, precision: str = "fp32c", graph_blocks: int = 50,

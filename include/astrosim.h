@@ -67,7 +67,15 @@ typedef struct as_network_info {
     double p_primary;
     uint64_t n_exc_edges, n_inh_edges, n_e2a_edges, n_sic_edges;
     double build_seconds; /* connectivity generation + inversion, as printed at src:869-871 */
+    int32_t built_on_gpu; /* 1 when as_network_build_on generated the connectivity on the GPU */
 } as_network_info;
+
+/* Where as_network_build_on generates the connectivity. */
+typedef enum as_build_where {
+    AS_BUILD_HOST = 0,     /* sequential host walk (as_network_build) */
+    AS_BUILD_GPU = 1,      /* parallel generator on the current GPU; the host walk when it declines */
+    AS_BUILD_GPU_ONLY = 2  /* parallel generator only: AS_ERR_INVALID when it declines */
+} as_build_where;
 
 /* ---- network construction (host only, no GPU needed) ---- */
 
@@ -75,6 +83,13 @@ typedef struct as_network_info {
  * Replaces src/astrosim_fused.cu:846 (ScalingConfig::make), :867
  * (build_connectivity) and :916-933 (population draws). */
 as_status as_network_build(uint32_t n_total, uint64_t seed, as_network** out);
+/* The same network, bit for bit, with the connectivity generated where
+ * `where` (as_build_where) says. The GPU generator walks the reference's one
+ * sequential setup stream in parallel (speculative chains stitched where
+ * they meet, autapse repairs, an exact walk of every row that proves the
+ * result) and inverts the edges with stable radix sorts; it declines
+ * networks with N_total below a few thousand. Same spans as as_network_build. */
+as_status as_network_build_on(uint32_t n_total, uint64_t seed, int32_t where, as_network** out);
 as_status as_network_info_get(const as_network* net, as_network_info* info);
 /* CSR by target. which: 0 exc, 1 inh, 2 sic (astro->neuron), 3 e2a (neuron->astro).
  * Either pointer may be NULL; row_start needs n_targets+1 entries.
@@ -204,6 +219,7 @@ typedef struct as_server_config {
     int32_t build_threads;    /* host network builders */
     int32_t blocks_per_round; /* blocks advanced between admission checks */
     int32_t max_members;      /* member slots per batch */
+    int32_t setup_where;      /* as_build_where of the request networks (default AS_BUILD_GPU) */
 } as_server_config;
 typedef struct as_request {
     uint32_t n_total;   /* --n */

@@ -31,6 +31,8 @@ def load():
             fn = getattr(lib, name)
             fn.restype = C.c_int
             fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+        lib.as_selftest_build_compare.restype = C.c_int
+        lib.as_selftest_build_compare.argtypes = [C.c_uint32, C.c_uint64, C.c_int, C.POINTER(C.c_double)]
         _lib = lib
     return _lib
 
@@ -89,3 +91,16 @@ def random_dividends(rng: np.random.Generator, n: int) -> np.ndarray:
     a = np.ldexp(1.0 + rng.random(n), rng.integers(-40, 40, n)) * np.where(rng.random(n) < 0.5, -1.0, 1.0)
     a[:4] = [0.0, -0.0, 0.0, -0.0]
     return a
+
+
+BUILD_STATS = ["counters", "chains", "ambiguous", "host_rows0", "host_rows1", "host_rows2", "records0",
+               "records2", "repairs0", "repairs2", "max_shift", "gen_seconds", "host_seconds", "edges", "diff"]
+
+
+def build_compare(n_total: int, seed: int, gpu: bool) -> tuple[int, dict]:
+    """The parallel connectivity generator (host emulation, or the GPU build)
+    against the sequential host builder: (0 identical / 1 different / 2
+    declined / -1 error, statistics)."""
+    st = (C.c_double * 16)()
+    rc = load().as_selftest_build_compare(n_total, seed, 1 if gpu else 0, st)
+    return rc, dict(zip(BUILD_STATS, list(st)))
