@@ -220,7 +220,11 @@ class Network:
     def __init__(self, n_total: int, seed: int = 1, setup: str = "host"):
         """``setup``: "host" walks the setup stream sequentially; "gpu" generates
         the same connectivity with the parallel generator on the current GPU
-        (the host walk when it declines a network); "gpu-only" raises instead."""
+        from N_total = 100000 on (the host walk below that, and where the
+        generator declines); "gpu-only" uses the generator for every size it
+        accepts and raises otherwise. A GPU-built network's connectivity stays
+        in device memory (simulations copy it device to device); csr() copies
+        it to the host on first use."""
         lib = load()
         h = C.c_void_p()
         _check(lib.as_network_build_on(C.c_uint32(n_total), C.c_uint64(seed),
@@ -517,9 +521,13 @@ REQUEST_STATUS = {0: "building", 1: "queued", 2: "running", 3: "done", 4: "rejec
 
 class Server:
     """Continuous-batching simulation server (native, include/astrosim.h):
-    requests are built on host threads, admitted into resident FP32c/FP64
-    batches as their footprint fits, advanced with both batches overlapping
-    on the device, and retired when done."""
+    requests are built by worker threads (networks from N_total = 100000 on
+    with the GPU connectivity generator, kept in device memory; smaller ones
+    by the host walk; ``setup_where`` as in include/astrosim.h), admitted into
+    resident FP32c/FP64 batches as their footprint fits, advanced with both
+    batches overlapping on the device, and retired when done. Give the process
+    more hardware queues than build threads + 4 (GPU_MAX_HW_QUEUES, read when
+    HIP initialises): streams that share a queue wait for each other's work."""
 
     def __init__(self, **kw):
         lib = load()

@@ -1202,9 +1202,8 @@ public:
     bool fits(const Network& net) const override {
         if (free_slot() < 0) return false;
         if (!nalloc_.can(net.cfg.n_neurons()) || !aalloc_.can(net.cfg.n_astro)) return false;
-        const HostCSR* c[4] = {&net.inc_exc, &net.inc_inh, &net.inc_sic, &net.inc_e2a};
         for (int i = 0; i < 4; ++i)
-            if (!ealloc_[i].can(c[i]->n_edges())) return false;
+            if (!ealloc_[i].can(net.edges(i))) return false;
         return true;
     }
 
@@ -1226,9 +1225,8 @@ public:
         m.step0 = step_;
         m.step_end = step_ + n_steps;
         m.slot = static_cast<std::uint32_t>(slot);
-        const HostCSR* c[4] = {&net.inc_exc, &net.inc_inh, &net.inc_sic, &net.inc_e2a};
         for (int i = 0; i < 4; ++i) {
-            h.ecount[i] = c[i]->n_edges();
+            h.ecount[i] = net.edges(i);
             h.ebase[i] = ealloc_[i].take(h.ecount[i]);
         }
         upload_member(net, h);
@@ -1612,19 +1610,23 @@ private:
         const index_t tbase[4] = {m.nb, m.nb, m.nb, m.ab};
         const index_t nsrc[4] = {m.ne, net.cfg.n_inh, m.na, m.ne};
         const index_t nspace[4] = {m.nn, m.nn, m.na, m.nn};
+        // a network generated on this GPU is copied device to device
+        const bool dev = net.on_device();
         for (int T = 0; T < 4; ++T) {
-            const index_t nt = c[T]->n_targets();
-            const std::size_t ne = c[T]->n_edges();
+            const index_t nt = dev ? net.dev[T].n_targets : c[T]->n_targets();
+            const std::size_t ne = net.edges(T);
             const index_t eb = static_cast<index_t>(h.ebase[T]);
             index_t* src = const_cast<index_t*>(D_.in_src[T]) + eb;
             if (ne) {
-                AS_HIP_CHECK(hipMemcpyAsync(src, c[T]->src.data(), ne * sizeof(index_t), hipMemcpyHostToDevice, stream_));
+                AS_HIP_CHECK(hipMemcpyAsync(src, dev ? net.dev[T].src : c[T]->src.data(), ne * sizeof(index_t),
+                                            dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
                 if (sbase[T] != 0)
                     hipLaunchKernelGGL(rebase_kernel, dim3(unsigned((ne + 255) / 256)), dim3(256), 0, stream_, src, ne,
                                        sbase[T]);
             }
-            AS_HIP_CHECK(hipMemcpyAsync(scratch0_, c[T]->row_start.data(), (std::size_t(nt) + 1) * sizeof(index_t),
-                                        hipMemcpyHostToDevice, stream_));
+            AS_HIP_CHECK(hipMemcpyAsync(scratch0_, dev ? net.dev[T].row_start : c[T]->row_start.data(),
+                                        (std::size_t(nt) + 1) * sizeof(index_t),
+                                        dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
             uint2* in_rs = const_cast<uint2*>(D_.in_rs[T]) + tbase[T];
             if (nt) hipLaunchKernelGGL(rows_kernel, dim3((nt + 255) / 256), dim3(256), 0, stream_, scratch0_, nt, eb, in_rs);
             // out-CSR over the member's source space
@@ -1748,10 +1750,7 @@ BatchCapacity capacity_for(const Network* const* nets, int n) {
         const Network& net = *nets[i];
         c.neurons += net.cfg.n_neurons();
         c.astros += net.cfg.n_astro;
-        c.edges[0] += net.inc_exc.n_edges();
-        c.edges[1] += net.inc_inh.n_edges();
-        c.edges[2] += net.inc_sic.n_edges();
-        c.edges[3] += net.inc_e2a.n_edges();
+        for (int k = 0; k < 4; ++k) c.edges[k] += net.edges(k);
     }
     return c;
 }

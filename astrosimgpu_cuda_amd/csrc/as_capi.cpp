@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -14,6 +15,12 @@
 struct as_network {
     astrosim::Network net;
     bool built_on_gpu = false;
+    std::mutex mu;  // host CSRs of a device-resident network are filled on first use
+    const astrosim::Network& host() {
+        std::lock_guard<std::mutex> g(mu);
+        astrosim::materialize_host_csrs(net);
+        return net;
+    }
 };
 
 struct as_sim {
@@ -101,7 +108,8 @@ as_status as_network_build_on(uint32_t n_total, uint64_t seed, int32_t where, as
         *out = nullptr;
         std::unique_ptr<as_network> n(new as_network);
         std::string why;
-        if (where != AS_BUILD_HOST && astrosim::build_network_gpu(n_total, seed, n->net, &why)) {
+        const bool try_gpu = where == AS_BUILD_GPU_ONLY || (where == AS_BUILD_GPU && n_total >= astrosim::kGpuBuildAutoMin);
+        if (try_gpu && astrosim::build_network_gpu(n_total, seed, n->net, &why, nullptr, /*keep_on_device=*/true)) {
             n->built_on_gpu = true;
         } else {
             if (where == AS_BUILD_GPU_ONLY)
@@ -122,10 +130,10 @@ as_status as_network_info_get(const as_network* net, as_network_info* info) {
         info->n_astro = c.n_astro;
         info->n_neurons = c.n_neurons();
         info->p_primary = c.p_primary;
-        info->n_exc_edges = net->net.inc_exc.n_edges();
-        info->n_inh_edges = net->net.inc_inh.n_edges();
-        info->n_e2a_edges = net->net.inc_e2a.n_edges();
-        info->n_sic_edges = net->net.inc_sic.n_edges();
+        info->n_exc_edges = net->net.edges(0);
+        info->n_inh_edges = net->net.edges(1);
+        info->n_e2a_edges = net->net.edges(3);
+        info->n_sic_edges = net->net.edges(2);
         info->build_seconds = net->net.build_seconds;
         info->built_on_gpu = net->built_on_gpu ? 1 : 0;
     });
@@ -135,8 +143,8 @@ as_status as_network_csr(const as_network* net, int which, uint32_t* row_start, 
     return guarded([&] {
         require(net != nullptr, "net is NULL");
         require(which >= 0 && which < 4, "which must be 0..3");
-        const astrosim::HostCSR* c[4] = {&net->net.inc_exc, &net->net.inc_inh, &net->net.inc_sic,
-                                         &net->net.inc_e2a};
+        const astrosim::Network& hn = const_cast<as_network*>(net)->host();
+        const astrosim::HostCSR* c[4] = {&hn.inc_exc, &hn.inc_inh, &hn.inc_sic, &hn.inc_e2a};
         const auto& csr = *c[which];
         if (row_start) std::memcpy(row_start, csr.row_start.data(), csr.row_start.size() * sizeof(uint32_t));
         if (src && !csr.src.empty()) std::memcpy(src, csr.src.data(), csr.src.size() * sizeof(uint32_t));
@@ -156,7 +164,7 @@ as_status as_network_params(const as_network* net, int which, double* out) {
 as_status as_network_dump_edges(const as_network* net, const char* path) {
     return guarded([&] {
         require(net && path, "NULL argument");
-        if (!astrosim::dump_edges_csv(net->net, path))
+        if (!astrosim::dump_edges_csv(const_cast<as_network*>(net)->host(), path))
             throw astrosim::Error(AS_ERR_IO, std::string("cannot open ") + path + " for --dump-edges");
     });
 }

@@ -55,6 +55,7 @@ constexpr u32 kGMask = (1u << 28) - 1;
 constexpr u32 kCoin = 1u << 28;
 constexpr int kSlotShift = 29;
 constexpr u64 kNoSelf = ~u64(0);
+constexpr u32 kNone = 0xFFFFFFFFu;  // no own target position
 
 AS_HD u32 atomic_add_u32(u32* p, u32 v) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -102,13 +103,13 @@ AS_HD u32 encode_entry(double r, u32 g, u32 take) {
 // targets. Passes 0 and 1 are the excitatory rows over the excitatory and
 // the inhibitory targets (with third-factor draws), pass 2 the inhibitory
 // rows over all neurons (src/astrosim_fused.cu:688-723). A row's own neuron
-// sits at target position self_base + row index (kNoSelf: never).
+// sits at target position self_base + row index (kNone: never).
 struct PassDesc {
     u32 pc;
     u32 tf;
     u32 n_rows;
     u32 post_offset;
-    u64 self_base;
+    u32 self_base;  // kNone: no row meets its own neuron
 };
 
 // Walk state: next counter, position in the row (0 exactly at a row start
@@ -125,33 +126,24 @@ enum : int { kEvEdge = 1, kEvTf = 2, kEvRowEnd = 4 };
 // least the remaining targets ends the row (src/astrosim_fused.cu:621-623,
 // 694); so does a synapse on the last target, once its coin (and pick) are
 // drawn, because the row loop's condition then fails (:692); a synapse onto
-// the row's own neuron (self_pos) is dropped without a coin draw (:697).
-AS_HD int walker_step(Walker& w, u32 e, const PassDesc& P, u64 self_pos) {
+// the row's own target position `self` (kNone: none) is dropped without a
+// coin draw (:697). Written with selects only: lanes of a wave in different
+// phases then run one instruction stream.
+AS_HD int walker_step(Walker& w, u32 e, const PassDesc& P, u32 self) {
     ++w.c;
-    int ev = 0;
-    if (w.phase == 0) {
-        const u32 g = e & kGMask;
-        if (g >= P.pc - w.pos) {
-            w.pos = 0;
-            return kEvRowEnd;
-        }
-        w.last = w.pos + g;
-        w.pos = w.last + 1;
-        if (w.last != self_pos) {
-            ev = kEvEdge;
-            w.phase = P.tf;
-        }
-    } else if (w.phase == 1) {
-        w.phase = (e & kCoin) ? 2u : 0u;
-    } else {
-        w.phase = 0;
-        ev = kEvTf;
-    }
-    if (w.phase == 0 && w.pos == P.pc) {
-        w.pos = 0;
-        ev |= kEvRowEnd;
-    }
-    return ev;
+    const u32 g = e & kGMask;
+    const bool s = w.phase == 0, t = w.phase == 1;
+    const bool end_s = s && g >= P.pc - w.pos;
+    const u32 last = w.pos + g;
+    const bool hit = s && !end_s;
+    const bool edge = hit && last != self;
+    const u32 phase = edge ? P.tf : ((t && (e & kCoin) != 0) ? 2u : 0u);
+    const u32 pos = hit ? last + 1 : w.pos;
+    const bool end = end_s || (phase == 0 && pos == P.pc);
+    w.pos = end ? 0u : pos;
+    w.last = hit ? last : w.last;
+    w.phase = phase;
+    return (edge ? kEvEdge : 0) | ((!s && !t) ? kEvTf : 0) | (end ? kEvRowEnd : 0);
 }
 
 // Decoded draws for counters [base, base + len).
@@ -165,6 +157,101 @@ AS_HD bool table_get(const Table& t, u64 c, u32& e) {
     if (c < t.base || k >= t.len) return false;
     e = t.e[k];
     return true;
+}
+
+// Sixteen table entries: a walk reads the table one chunk at a time, the
+// next chunk requested while the current one is consumed, so that a lane
+// walking alone pays one memory latency per sixteen draws, not per draw.
+struct alignas(16) Quad {
+    u32 v[4];
+};
+constexpr u64 kChunk = 16;
+
+AS_HD void load_chunk(const Table& t, u64 chunk, Quad (&q)[4]) {
+    const Quad* p = reinterpret_cast<const Quad*>(t.e) + chunk * 4;
+    q[0] = p[0];
+    q[1] = p[1];
+    q[2] = p[2];
+    q[3] = p[3];
+}
+
+// Calls step(e) with the entries of counters c, c + 1, ... while it returns
+// true. Returns false when the table ends first. The table's storage holds
+// whole chunks (its length rounded up to kChunk, plus one chunk).
+template <class Step>
+AS_HD bool drive(const Table& t, u64 c, Step&& step) {
+    if (c < t.base) return false;
+    u64 k = c - t.base;
+    if (k >= t.len) return false;
+    u64 chunk = k / kChunk;
+    u32 first = static_cast<u32>(k % kChunk);
+    Quad cur[4];
+    load_chunk(t, chunk, cur);
+    for (;;) {
+        // here chunk * kChunk < len, so chunk + 1 lies inside the storage
+        Quad nxt[4];
+        load_chunk(t, chunk + 1, nxt);
+        const u64 k0 = chunk * kChunk;
+#pragma unroll
+        for (u32 j = 0; j < kChunk; ++j) {
+            if (j >= first) {
+                if (k0 + j >= t.len) return false;
+                if (!step(cur[j / 4].v[j % 4])) return true;
+            }
+        }
+        ++chunk;
+        first = 0;
+        if (chunk * kChunk >= t.len) return false;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cur[j] = nxt[j];
+    }
+}
+
+// One bit per counter from `base`, `bits` of them (whole 32-bit words, the
+// unused tail zero); base is congruent to the table's base modulo 32, so a
+// table chunk's sixteen bits lie in one word.
+struct Bitmap {
+    const u32* w;
+    u64 base, bits;
+};
+
+AS_HD u32 chunk_bits(const Table& t, const Bitmap& b, u64 chunk) {
+    const u64 c = t.base + chunk * kChunk;
+    if (c < b.base || c - b.base >= b.bits) return 0u;
+    const u64 rel = c - b.base;
+    return (b.w[rel >> 5] >> (rel & 31)) & 0xFFFFu;
+}
+
+// As drive, passing each counter's bit of `b` along with its entry.
+template <class Step>
+AS_HD bool drive_bits(const Table& t, const Bitmap& b, u64 c, Step&& step) {
+    if (c < t.base) return false;
+    u64 k = c - t.base;
+    if (k >= t.len) return false;
+    u64 chunk = k / kChunk;
+    u32 first = static_cast<u32>(k % kChunk);
+    Quad cur[4];
+    load_chunk(t, chunk, cur);
+    u32 bcur = chunk_bits(t, b, chunk);
+    for (;;) {
+        Quad nxt[4];
+        load_chunk(t, chunk + 1, nxt);
+        const u32 bnxt = chunk_bits(t, b, chunk + 1);
+        const u64 k0 = chunk * kChunk;
+#pragma unroll
+        for (u32 j = 0; j < kChunk; ++j) {
+            if (j >= first) {
+                if (k0 + j >= t.len) return false;
+                if (!step(cur[j / 4].v[j % 4], (bcur >> j) & 1u)) return true;
+            }
+        }
+        ++chunk;
+        first = 0;
+        if (chunk * kChunk >= t.len) return false;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cur[j] = nxt[j];
+        bcur = bnxt;
+    }
 }
 
 AS_HD u64 lower_bound_u32(const u32* a, u64 n, u32 v) {
@@ -221,6 +308,7 @@ enum : u32 { kChainOk = 0, kChainTable = 1, kChainCapA = 2, kChainRegion = 3, kC
 struct ChainArgs {
     Table T;
     ChainPass pass[3];
+    u32* abits[3];  // per pass: the chains' recorded row starts, from pass[q].r0
     u64 W;
     u32 capA;
     u32* A;       // [chain][capA]: row starts in [a_j, a_{j+1}) as offsets from a_j
@@ -246,14 +334,23 @@ AS_HD void chain_a(const ChainArgs& a, u64 g) {
     u32 n = 0, st = kChainOk;
     A[n++] = 0;
     Walker w{a0, 0, 0, 0, 0};
-    while (w.c < a1) {
-        u32 e;
-        if (!table_get(a.T, w.c, e)) { st = kChainTable; break; }
-        if ((walker_step(w, e, cp.P, kNoSelf) & kEvRowEnd) && w.c < a1) {
-            if (n == a.capA) { st = kChainCapA; break; }
+    // the chain owns the bitmap words of [a0, a1) (W is a multiple of 32)
+    u32* bm = a.abits[chain_pass_of(a, g)];
+    bm[(a0 - cp.r0) >> 5] |= 1u << ((a0 - cp.r0) & 31);
+    const bool stopped = drive(a.T, w.c, [&](u32 e) {
+        if (w.c >= a1) return false;
+        if ((walker_step(w, e, cp.P, kNone) & kEvRowEnd) && w.c < a1) {
+            if (n == a.capA) {
+                st = kChainCapA;
+                return false;
+            }
             A[n++] = static_cast<u32>(w.c - a0);
+            const u64 rel = w.c - cp.r0;
+            bm[rel >> 5] |= 1u << (rel & 31);
         }
-    }
+        return true;
+    });
+    if (!stopped) st = kChainTable;
     a.nA[g] = n;
     a.state[g] = w;
     a.status_a[g] = st;
@@ -267,29 +364,35 @@ AS_HD void chain_b(const ChainArgs& a, u64 g) {
     if (a.status_a[g] != kChainOk) { a.status_b[g] = a.status_a[g]; return; }
     const u64 rend = cp.r0 + u64(cp.n_chains) * a.W;
     Walker w = a.state[g];
-    u32 n = 0;
-    for (;;) {
+    u32 n = 0, st = kChainTable;
+    const Bitmap bm{a.abits[chain_pass_of(a, g)], cp.r0, u64(cp.n_chains) * a.W};
+    drive_bits(a.T, bm, w.c, [&](u32 e, u32 bit) {
         if (w.pos == 0 && w.phase == 0) {
-            if (w.c >= rend) { a.status_b[g] = kChainRegion; return; }
-            const u64 k = (w.c - cp.r0) / a.W;
-            const u64 gk = cp.chain0 + k;
-            const u32 off = static_cast<u32>(w.c - (cp.r0 + k * a.W));
-            const u32* Ak = a.A + gk * a.capA;
-            const u64 r = lower_bound_u32(Ak, a.nA[gk], off);
-            if (r < a.nA[gk] && Ak[r] == off) {
-                a.couple_x[g] = w.c;
-                a.couple_k[g] = static_cast<u32>(k);
-                a.couple_r[g] = static_cast<u32>(r);
-                a.nB[g] = n;
-                a.status_b[g] = kChainOk;
-                return;
+            if (bit) {
+                st = kChainOk;
+                return false;
+            }
+            if (w.c >= rend) {
+                st = kChainRegion;
+                return false;
             }
             ++n;
         }
-        u32 e;
-        if (!table_get(a.T, w.c, e)) { a.status_b[g] = kChainTable; return; }
-        walker_step(w, e, cp.P, kNoSelf);
+        walker_step(w, e, cp.P, kNone);
+        return true;
+    });
+    if (st == kChainOk) {
+        // the chain that recorded this row start, and its rank there
+        const u64 k = (w.c - cp.r0) / a.W;
+        const u64 gk = cp.chain0 + k;
+        const u32* Ak = a.A + gk * a.capA;
+        const u64 r = lower_bound_u32(Ak, a.nA[gk], static_cast<u32>(w.c - (cp.r0 + k * a.W)));
+        a.couple_x[g] = w.c;
+        a.couple_k[g] = static_cast<u32>(k);
+        a.couple_r[g] = static_cast<u32>(r);
     }
+    a.nB[g] = st == kChainOk ? n : 0;
+    a.status_b[g] = st;
 }
 
 // One stretch of the stitched path: chain k's recorded starts from rank r on,
@@ -321,16 +424,15 @@ AS_HD void path_write(const PathArgs& a, u64 idx) {
     if (pe.nb == 0) return;
     Walker w = a.C.state[pe.g];
     u32 n = 0;
-    while (n < pe.nb) {
+    const bool stopped = drive(a.T, w.c, [&](u32 e) {
         if (w.pos == 0 && w.phase == 0) {
             a.L[o++] = w.c;
-            ++n;
-            if (n == pe.nb) break;
+            if (++n == pe.nb) return false;
         }
-        u32 e;
-        if (!table_get(a.T, w.c, e)) { atomic_or_u32(a.fail, 1u); return; }
-        walker_step(w, e, cp.P, kNoSelf);
-    }
+        walker_step(w, e, cp.P, kNone);
+        return true;
+    });
+    if (!stopped) atomic_or_u32(a.fail, 1u);
 }
 
 // ---- 3. autapse repair ----
@@ -352,10 +454,8 @@ AS_HD void verify_row(const VerifyArgs& a, u64 idx) {
     const u64 i = a.m + idx;
     const i64 target = static_cast<i64>(a.P.self_base + i);
     Walker w{a.L[i], 0, 0, 0, 0};
-    for (;;) {
-        u32 e;
-        if (!table_get(a.T, w.c, e)) { atomic_or_u32(a.fail, 1u); return; }
-        const int ev = walker_step(w, e, a.P, kNoSelf);
+    const bool stopped = drive(a.T, w.c, [&](u32 e) {
+        const int ev = walker_step(w, e, a.P, kNone);
         if (ev & kEvEdge) {
             const i64 off = static_cast<i64>(w.last) - target;
             if (off >= -a.K && off <= a.K) {
@@ -368,7 +468,11 @@ AS_HD void verify_row(const VerifyArgs& a, u64 idx) {
                 }
             }
         }
-        if (ev & kEvRowEnd) break;
+        return (ev & kEvRowEnd) == 0;
+    });
+    if (!stopped) {
+        atomic_or_u32(a.fail, 1u);
+        return;
     }
     if (w.c != a.L[i + 1]) atomic_or_u32(a.fail, 4u);
 }
@@ -378,6 +482,7 @@ struct RepairArgs {
     PassDesc P;
     const u64* L;
     u64 nL;
+    Bitmap lbits;  // the list's row starts
     const u64* rec_i;
     const int* rec_off;
     u32 cap_rows;
@@ -400,38 +505,59 @@ AS_HD void repair_walk(const RepairArgs& a, u32 rec, u64* out) {
     const i64 t = static_cast<i64>(i) + a.rec_off[rec];
     if (out == nullptr) a.out_j[rec] = kNoSelf;
     if (t < 0) return;
-    u64 x = a.L[i];
-    u32 rows = 0;
-    for (;;) {
-        const u64 self = a.P.self_base + static_cast<u64>(t) + rows;
-        Walker w{x, 0, 0, 0, 0};
-        for (;;) {
-            u32 e;
-            if (!table_get(a.T, w.c, e)) { atomic_or_u32(a.fail, 8u); return; }
-            if (walker_step(w, e, a.P, self) & kEvRowEnd) break;
-        }
-        ++rows;
-        x = w.c;
-        if (static_cast<u64>(t) + rows >= a.P.n_rows) {
-            if (out != nullptr) {
-                out[rows - 1] = x;
-            } else {
-                a.out_j[rec] = kPassEnd;
-                a.out_m[rec] = rows;
+    const u64 x0 = a.L[i];
+    u32 rows = 0, res = 0;  // res: 1 rejoined, 2 pass end, 3 too long
+    u32 self = a.P.self_base + static_cast<u32>(t);
+    Walker w{x0, 0, 0, 0, 0};
+    const bool stopped = drive_bits(a.T, a.lbits, w.c, [&](u32 e, u32 bit) {
+        if (w.pos == 0 && w.phase == 0 && rows > 0) {
+            if (static_cast<u64>(t) + rows >= a.P.n_rows) {
+                if (out != nullptr) out[rows - 1] = w.c;
+                res = 2;
+                return false;
             }
-            return;
-        }
-        const u64 j = lower_bound_u64(a.L, i + 1, a.nL, x);
-        if (j < a.nL && a.L[j] == x) {
-            if (out == nullptr) {
-                a.out_j[rec] = j;
-                a.out_m[rec] = rows;
+            if (bit) {
+                res = 1;
+                return false;
             }
-            return;
+            if (rows >= a.cap_rows) {
+                res = 3;
+                return false;
+            }
+            if (out != nullptr) out[rows - 1] = w.c;
+            self = a.P.self_base + static_cast<u32>(t) + rows;
         }
-        if (rows >= a.cap_rows) return;
-        if (out != nullptr) out[rows - 1] = x;
+        if (walker_step(w, e, a.P, self) & kEvRowEnd) ++rows;
+        return true;
+    });
+    if (!stopped) {
+        atomic_or_u32(a.fail, 8u);
+        return;
     }
+    if (out != nullptr) return;
+    if (res == 2) {
+        a.out_j[rec] = kPassEnd;
+        a.out_m[rec] = rows;
+    } else if (res == 1) {
+        const u64 j = lower_bound_u64(a.L, i + 1, a.nL, w.c);
+        if (j < a.nL && a.L[j] == w.c) {
+            a.out_j[rec] = j;
+            a.out_m[rec] = rows;
+        }
+    }
+}
+
+// Bits of the list entries [lo, hi) that lie in the bitmap's range.
+struct MarkArgs {
+    const u64* L;
+    u64 lo;
+    u32* w;
+    u64 base, bits;
+};
+AS_HD void mark_list(const MarkArgs& a, u64 idx) {
+    const u64 x = a.L[a.lo + idx];
+    if (x < a.base || x - a.base >= a.bits) return;
+    atomic_or_u32(a.w + ((x - a.base) >> 5), 1u << ((x - a.base) & 31));
 }
 
 AS_HD void repair_row(const RepairArgs& a, u64 rec) { repair_walk(a, static_cast<u32>(rec), nullptr); }
@@ -486,12 +612,10 @@ struct EmitArgs {
 
 template <bool kEmit>
 AS_HD void exact_row(const EmitArgs& a, u64 i) {
-    const u64 self = a.P.self_base == kNoSelf ? kNoSelf : a.P.self_base + i;
+    const u32 self = a.P.self_base == kNone ? kNone : a.P.self_base + static_cast<u32>(i);
     Walker w{a.F[i], 0, 0, 0, 0};
     u64 ne = 0, nt = 0;
-    for (;;) {
-        u32 e;
-        if (!table_get(a.T, w.c, e)) { atomic_or_u32(a.fail, 16u); return; }
+    const bool stopped = drive(a.T, w.c, [&](u32 e) {
         const int ev = walker_step(w, e, a.P, self);
         if (ev & kEvEdge) {
             if (kEmit) {
@@ -510,7 +634,11 @@ AS_HD void exact_row(const EmitArgs& a, u64 i) {
             }
             ++nt;
         }
-        if (ev & kEvRowEnd) break;
+        return (ev & kEvRowEnd) == 0;
+    });
+    if (!stopped) {
+        atomic_or_u32(a.fail, 16u);
+        return;
     }
     if (!kEmit) {
         if (w.c != a.F[i + 1]) atomic_or_u32(a.fail, 32u);

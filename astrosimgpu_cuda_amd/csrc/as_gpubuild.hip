@@ -102,7 +102,8 @@ private:
 
 }  // namespace
 
-bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::string* why) {
+bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::string* why,
+                       gb::BuildStats* stats, bool keep_on_device) {
     std::string reason;
     auto decline = [&](const std::string& r) {
         if (why) *why = r;
@@ -129,8 +130,21 @@ bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::s
         const std::vector<index_t> pool = make_pools(cfg, rng);
         const auto t_pools = std::chrono::steady_clock::now();
         out.phase_seconds[0] = std::chrono::duration<double>(t_pools - t0).count();
-        HipBackend bk;
-        ok = gb::build_connectivity(bk, cfg, seed, rng.counter(), pool, ScalingConfig::kPoolSize, out, st, reason);
+        st.pools_seconds = out.phase_seconds[0];
+        std::vector<void*> kept;
+        {
+            HipBackend bk;  // its destructor waits for the build's stream
+            ok = gb::build_connectivity(bk, cfg, seed, rng.counter(), pool, ScalingConfig::kPoolSize, out, st,
+                                        reason, keep_on_device ? &kept : nullptr);
+        }
+        if (ok && keep_on_device) {
+            auto* held = new std::vector<void*>(std::move(kept));
+            out.dev_owner = std::shared_ptr<void>(held, [](void* p) {
+                auto* v = static_cast<std::vector<void*>*>(p);
+                for (void* q : *v) (void)hipFreeAsync(q, nullptr);
+                delete v;
+            });
+        }
     } catch (const HipFailure& f) {
         params.join();
         throw std::runtime_error("GPU network build: " + f.what);
@@ -138,13 +152,32 @@ bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::s
         params.join();
         throw;
     }
+    const auto t_join = std::chrono::steady_clock::now();
     params.join();
+    st.params_wait_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_join).count();
+    if (stats) *stats = st;
     if (!ok) return decline(reason);
     const auto t1 = std::chrono::steady_clock::now();
     out.build_seconds = std::chrono::duration<double>(t1 - t0).count();
     out.phase_seconds[1] = st.seconds;
     net = std::move(out);
     return true;
+}
+
+void materialize_host_csrs(Network& net) {
+    if (net.host_csrs()) return;
+    HostCSR* c[4] = {&net.inc_exc, &net.inc_inh, &net.inc_sic, &net.inc_e2a};
+    for (int k = 0; k < 4; ++k) {
+        const DeviceCSR& d = net.dev[k];
+        c[k]->row_start.resize(std::size_t(d.n_targets) + 1);
+        c[k]->src.resize(d.n_edges);
+        hipError_t e = hipMemcpy(c[k]->row_start.data(), d.row_start, c[k]->row_start.size() * sizeof(index_t),
+                                 hipMemcpyDeviceToHost);
+        if (e == hipSuccess && d.n_edges)
+            e = hipMemcpy(c[k]->src.data(), d.src, d.n_edges * sizeof(index_t), hipMemcpyDeviceToHost);
+        if (e != hipSuccess)
+            throw std::runtime_error(std::string("copying a GPU-built network to the host: ") + hipGetErrorString(e));
+    }
 }
 
 }  // namespace astrosim

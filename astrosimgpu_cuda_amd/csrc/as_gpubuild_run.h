@@ -39,6 +39,12 @@ struct BuildStats {
     u32 repairs[3] = {0, 0, 0};    // autapses that moved the walk off the chains
     int max_shift = 0;
     double seconds = 0.0;
+    // seconds since the start at: decode, chains, passes 0-2 stitched,
+    // counts, edges emitted, the four inversions
+    double mark[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double pools_seconds = 0.0, params_wait_seconds = 0.0;
+    u64 max_b_draws = 0;      // longest chain walk past the next anchor before it met another chain
+    u32 max_repair_rows = 0;  // longest repair walk, in rows
 };
 
 // The draw at counter c decoded on the host, with the C library deciding
@@ -66,6 +72,7 @@ struct CountF { EmitArgs a; AS_HD void operator()(u64 i) const { exact_row<false
 struct EmitF { EmitArgs a; AS_HD void operator()(u64 i) const { exact_row<true>(a, i); } };
 struct AstroF { AstroArgs a; AS_HD void operator()(u64 i) const { tf_astro(a, i); } };
 struct HistF { HistArgs a; AS_HD void operator()(u64 i) const { histogram(a, i); } };
+struct MarkF { MarkArgs a; AS_HD void operator()(u64 i) const { mark_list(a, i); } };
 struct WidenF {
     const u32* in;
     u64* out;
@@ -94,6 +101,11 @@ struct Arena {
             held.erase(it);
         }
     }
+    // Hands an allocation over to the caller (no longer freed here).
+    void keep(void* p) {
+        auto it = std::find(held.begin(), held.end(), p);
+        if (it != held.end()) held.erase(it);
+    }
 };
 
 struct Declined {
@@ -114,10 +126,17 @@ constexpr u32 kAmbCap = 1u << 16;
 // pools `pool` ([neuron][take]). Returns false with a reason when the
 // network is outside what the method handles (the caller then builds it on
 // the host); the result, when true, equals build_network's bit for bit.
+// With `kept`, the CSRs stay in the backend's memory (net.dev; the host CSRs
+// are left empty) and their allocations are appended to *kept for the
+// caller to free.
 template <class BK>
 bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, const std::vector<index_t>& pool,
-                        u32 take, Network& net, BuildStats& st, std::string& why) {
+                        u32 take, Network& net, BuildStats& st, std::string& why,
+                        std::vector<void*>* kept = nullptr) {
     const auto t0 = std::chrono::steady_clock::now();
+    auto mark = [&](int k) {
+        st.mark[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
     Arena<BK> ar(bk);
     try {
         const u32 n_exc = cfg.n_exc, n_inh = cfg.n_inh, n_neu = cfg.n_exc + cfg.n_inh, n_astro = cfg.n_astro;
@@ -125,7 +144,7 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
         if (!(p > 0.0 && p < 1.0)) throw Declined{"p_primary must be in (0, 1)"};
         const double log1m_p = std::log1p(-p);
         const PassDesc P[3] = {{n_exc, 1u, n_exc, 0u, 0u},
-                               {n_inh, 1u, n_exc, n_exc, kNoSelf},
+                               {n_inh, 1u, n_exc, n_exc, kNone},
                                {n_neu, 0u, n_inh, 0u, n_exc}};
 
         // ---- regions: expected draws per pass with generous margins ----
@@ -143,6 +162,7 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
             const u64 est = u64(start_est);
             r0[q] = est > c0 + lo_m ? est - lo_m : c0;
             if (q == 0) r0[q] = c0;
+            r0[q] = c0 + (r0[q] - c0) / 32 * 32;  // bitmaps share the table's alignment
             var_start += S2[q];
             start_est += D[q];
             rend[q] = u64(start_est) + margin(var_start);
@@ -164,7 +184,8 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
         st.chains = G;
 
         // ---- 1. decode every counter ----
-        u32* E = ar.template get<u32>(len);
+        // whole chunks plus one: a walk's next-chunk request stays inside
+        u32* E = ar.template get<u32>((len + kChunk - 1) / kChunk * kChunk + kChunk);
         u32* flags = ar.template get<u32>(8);  // [0] ambiguous count, [1] fail bits, [2] records
         bk.zero(flags, 8 * sizeof(u32));
         u64* amb = ar.template get<u64>(kAmbCap);
@@ -182,6 +203,7 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
             }
         }
         const Table T{E, base, len};
+        mark(0);
 
         // ---- 2. chains ----
         C.T = T;
@@ -196,6 +218,11 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
         C.couple_x = ar.template get<u64>(G);
         C.couple_k = ar.template get<u32>(G);
         C.couple_r = ar.template get<u32>(G);
+        for (int q = 0; q < 3; ++q) {
+            const u64 words = u64(C.pass[q].n_chains) * kW / 32;
+            C.abits[q] = ar.template get<u32>(words);
+            bk.zero(C.abits[q], words * sizeof(u32));
+        }
         bk.run(G, ChainAF{C});
         bk.run(G, ChainBF{C});
         std::vector<u32> nA(G), sa(G), sb(G), nB(G), ck(G), cr(G);
@@ -205,6 +232,17 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
         bk.d2h(nB.data(), C.nB, G * sizeof(u32));
         bk.d2h(ck.data(), C.couple_k, G * sizeof(u32));
         bk.d2h(cr.data(), C.couple_r, G * sizeof(u32));
+        {
+            std::vector<u64> cx(G);
+            bk.d2h(cx.data(), C.couple_x, G * sizeof(u64));
+            for (int q = 0; q < 3; ++q)
+                for (u64 j = 0; j + 1 < C.pass[q].n_chains; ++j) {
+                    const u64 g = C.pass[q].chain0 + j;
+                    if (sb[g] == kChainOk)
+                        st.max_b_draws = std::max<u64>(st.max_b_draws, cx[g] - (C.pass[q].r0 + (j + 1) * kW));
+                }
+        }
+        mark(1);
 
         // ---- per pass: stitch, repair, final row starts ----
         u64* F[3] = {nullptr, nullptr, nullptr};
@@ -213,7 +251,7 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
         for (int q = 0; q < 3; ++q) {
             const PassDesc& Pq = P[q];
             const ChainPass& cp = C.pass[q];
-            const bool has_self = Pq.self_base != kNoSelf;
+            const bool has_self = Pq.self_base != kNone;
             const u64 need = u64(Pq.n_rows) + 1 + (has_self ? u64(kK) + 2 : 0);
 
             // exact host walk from the pass's first row start until it meets a chain
@@ -247,7 +285,7 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
                     }
                 }
                 if (host_rows.size() > (1u << 20)) throw Declined{"pass start did not meet the chains"};
-                const u64 self = has_self ? Pq.self_base + host_rows.size() : kNoSelf;
+                const u32 self = has_self ? Pq.self_base + static_cast<u32>(host_rows.size()) : kNone;
                 host_rows.push_back(x);
                 Walker w{x, 0, 0, 0, 0};
                 while (!(walker_step(w, host_entry(seed, w.c, log1m_p, take), Pq, self) & kEvRowEnd)) {
@@ -261,6 +299,7 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
                 bk.h2d(Fq, host_rows.data(), m * sizeof(u64));
                 F[q] = Fq;
                 s = host_rows.back();
+                mark(2 + q);
                 continue;
             }
 
@@ -310,8 +349,13 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
                 std::vector<u64> hi(nrec), hj(nrec);
                 std::vector<int> ho(nrec);
                 std::vector<u32> hm(nrec);
-                RepairArgs ra{T, Pq, L, need, rec_i, rec_off, kRepairRows, nullptr, nullptr,
-                              nullptr, nullptr, nullptr, flags + 1};
+                // the list's row starts as a bitmap over the pass's region
+                const u64 lwords = u64(cp.n_chains) * kW / 32;
+                u32* lw = ar.template get<u32>(lwords);
+                bk.zero(lw, lwords * sizeof(u32));
+                bk.run(need - m, MarkF{MarkArgs{L, m, lw, cp.r0, lwords * 32}});
+                RepairArgs ra{T, Pq, L, need, Bitmap{lw, cp.r0, lwords * 32}, rec_i, rec_off, kRepairRows, nullptr,
+                              nullptr, nullptr, nullptr, nullptr, flags + 1};
                 if (nrec > 0) {
                     ra.out_j = ar.template get<u64>(nrec);
                     ra.out_m = ar.template get<u32>(nrec);
@@ -323,6 +367,8 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
                     bk.d2h(hj.data(), ra.out_j, nrec * sizeof(u64));
                     bk.d2h(hm.data(), ra.out_m, nrec * sizeof(u32));
                 }
+                for (u32 k = 0; k < nrec; ++k)
+                    if (hj[k] != kNoSelf) st.max_repair_rows = std::max(st.max_repair_rows, hm[k]);
                 std::vector<u32> order(nrec);
                 for (u32 k = 0; k < nrec; ++k) order[k] = k;
                 std::sort(order.begin(), order.end(), [&](u32 a, u32 b) {
@@ -384,6 +430,7 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
                 F[q] = Fq;
             }
             bk.d2h(&s, F[q] + Pq.n_rows, sizeof(u64));
+            mark(2 + q);
         }
         bk.d2h(hflags, flags, sizeof hflags);
         if (hflags[1] != 0) throw Declined{"path walk left the table"};
@@ -416,6 +463,7 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
         }
         bk.d2h(hflags, flags, sizeof hflags);
         if (hflags[1] != 0) throw Declined{"exact walk disagrees with the row starts"};
+        mark(5);
         const u64 e_exc = n_e[0] + n_e[1], e_inh = n_e[2], n_tf = n_t[0] + n_t[1];
         if (e_exc >= (u64(1) << 31) || e_inh >= (u64(1) << 31)) throw Declined{"edge count beyond the sort range"};
         u32* exc_src = ar.template get<u32>(e_exc);
@@ -441,9 +489,10 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
         bk.d2h(hflags, flags, sizeof hflags);
         if (hflags[1] & 64u) throw Declined{"an astrocyte reaches the out-degree cap"};
         if (hflags[1] != 0) throw Declined{"exact walk left the table"};
+        mark(6);
 
         // ---- inversion: stable sort by target, row starts from the target counts ----
-        auto invert = [&](const u32* key, const u32* val, u64 n, u32 n_targets, HostCSR& out) {
+        auto invert = [&](const u32* key, const u32* val, u64 n, u32 n_targets, HostCSR& out, int which) {
             int bits = 1;
             while (bits < 32 && (u64(1) << bits) < u64(n_targets)) ++bits;
             u32* ko = ar.template get<u32>(n);
@@ -453,20 +502,37 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
             bk.zero(cnt, (u64(n_targets) + 1) * sizeof(u32));
             bk.run(n, HistF{HistArgs{key, cnt}});
             bk.scan_u32(cnt, u64(n_targets) + 1);
+            ar.drop(ko);
+            if (kept != nullptr) {
+                net.dev[which] = DeviceCSR{cnt, vo, n_targets, n};
+                ar.keep(cnt);
+                ar.keep(vo);
+                kept->push_back(cnt);
+                kept->push_back(vo);
+                return;
+            }
             out.row_start.resize(u64(n_targets) + 1);
             out.src.resize(n);
             bk.d2h(out.row_start.data(), cnt, (u64(n_targets) + 1) * sizeof(u32));
             if (n > 0) bk.d2h(out.src.data(), vo, n * sizeof(u32));
-            ar.drop(ko);
             ar.drop(vo);
             ar.drop(cnt);
         };
-        invert(exc_dst, exc_src, e_exc, n_neu, net.inc_exc);
-        invert(inh_dst, inh_src, e_inh, n_neu, net.inc_inh);
-        invert(tf_post, astro, n_tf, n_neu, net.inc_sic);
-        invert(astro, tf_pre, n_tf, n_astro, net.inc_e2a);
+        invert(exc_dst, exc_src, e_exc, n_neu, net.inc_exc, 0);
+        mark(7);
+        invert(inh_dst, inh_src, e_inh, n_neu, net.inc_inh, 1);
+        mark(8);
+        invert(tf_post, astro, n_tf, n_neu, net.inc_sic, 2);
+        mark(9);
+        invert(astro, tf_pre, n_tf, n_astro, net.inc_e2a, 3);
+        mark(10);
     } catch (const Declined& d) {
         why = d.why;
+        if (kept != nullptr) {
+            for (void* p : *kept) bk.release(p);
+            kept->clear();
+            for (auto& d2 : net.dev) d2 = DeviceCSR{};
+        }
         return false;
     }
     st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

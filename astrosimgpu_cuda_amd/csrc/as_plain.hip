@@ -160,8 +160,9 @@ PlainLayout plain_layout(const Network& net, const RunParams& rp) {
     PlainLayout o{};
     const HostCSR* c[4] = {&net.inc_exc, &net.inc_inh, &net.inc_sic, &net.inc_e2a};
     for (int i = 0; i < 4; ++i) {
-        o.csr_rs[i] = L.add(c[i]->row_start.size() * sizeof(index_t));
-        o.csr_src[i] = L.add(std::max<std::size_t>(1, c[i]->src.size()) * sizeof(index_t));
+        const std::size_t rows = net.on_device() ? std::size_t(net.dev[i].n_targets) + 1 : c[i]->row_start.size();
+        o.csr_rs[i] = L.add(rows * sizeof(index_t));
+        o.csr_src[i] = L.add(std::max<std::size_t>(1, net.edges(i)) * sizeof(index_t));
     }
     const std::size_t na = net.cfg.n_astro, nn = net.cfg.n_neurons();
     for (int i = 0; i < 7; ++i) o.astro[i] = L.add(na * sizeof(double));
@@ -191,12 +192,15 @@ public:
         char* b = static_cast<char*>(base_);
         auto dptr = [&](std::size_t off) { return static_cast<void*>(b + off); };
         const HostCSR* c[4] = {&net.inc_exc, &net.inc_inh, &net.inc_sic, &net.inc_e2a};
+        const bool dev = net.on_device();  // a network generated on this GPU: device to device
         for (int i = 0; i < 4; ++i) {
-            AS_HIP_CHECK(hipMemcpy(dptr(lay_.csr_rs[i]), c[i]->row_start.data(),
-                                   c[i]->row_start.size() * sizeof(index_t), hipMemcpyHostToDevice));
-            if (!c[i]->src.empty())
-                AS_HIP_CHECK(hipMemcpy(dptr(lay_.csr_src[i]), c[i]->src.data(),
-                                       c[i]->src.size() * sizeof(index_t), hipMemcpyHostToDevice));
+            const std::size_t rows = dev ? std::size_t(net.dev[i].n_targets) + 1 : c[i]->row_start.size();
+            AS_HIP_CHECK(hipMemcpy(dptr(lay_.csr_rs[i]), dev ? net.dev[i].row_start : c[i]->row_start.data(),
+                                   rows * sizeof(index_t), dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+            if (net.edges(i))
+                AS_HIP_CHECK(hipMemcpy(dptr(lay_.csr_src[i]), dev ? net.dev[i].src : c[i]->src.data(),
+                                       net.edges(i) * sizeof(index_t),
+                                       dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
         }
         D_.exc_rs = (const index_t*)dptr(lay_.csr_rs[0]);
         D_.exc_src = (const index_t*)dptr(lay_.csr_src[0]);

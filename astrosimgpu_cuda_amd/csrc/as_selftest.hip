@@ -147,11 +147,13 @@ bool same_csr(const astrosim::HostCSR& a, const astrosim::HostCSR& b) {
 // the current GPU through build_network_gpu (where = 1) -- and compares the
 // four CSRs (and, for where = 1, the per-cell parameters). Returns 0 when
 // identical, 1 when different, 2 when the parallel generator declined, -1
-// on an exception. stats (16 doubles, may be NULL): counters, chains,
+// on an exception. stats (32 doubles, may be NULL): counters, chains,
 // ambiguous skips, host rows of passes 0-2, records of passes 0 and 2,
 // repairs of passes 0 and 2, largest index shift, generator seconds, host
 // builder seconds, edges compared, mask of what differs (1 exc, 2 inh, 4 sic,
-// 8 e2a, 16 parameters).
+// 8 e2a, 16 parameters), then the generator's phase marks (decode, chains,
+// passes 0-2, counts, emission, four inversions; seconds from its start),
+// the pools' seconds and the wait for the parameter draws.
 extern "C" int as_selftest_build_compare(uint32_t n_total, uint64_t seed, int where, double* stats) {
     using namespace astrosim;
     try {
@@ -174,10 +176,15 @@ extern "C" int as_selftest_build_compare(uint32_t n_total, uint64_t seed, int wh
             HostBackend bk;
             ok = build_connectivity(bk, cfg, seed, rng.counter(), pool, take, got, st, why);
         } else {
-            ok = build_network_gpu(n_total, seed, got, &why);
+            ok = build_network_gpu(n_total, seed, got, &why, &st);
             st.seconds = got.build_seconds;
         }
         if (stats) {
+            for (int k = 0; k < 11; ++k) stats[16 + k] = st.mark[k];
+            stats[27] = st.pools_seconds;
+            stats[28] = st.params_wait_seconds;
+            stats[29] = double(st.max_b_draws);
+            stats[30] = double(st.max_repair_rows);
             const double v[16] = {double(st.counters), double(st.chains), double(st.ambiguous),
                                   double(st.host_rows[0]), double(st.host_rows[1]), double(st.host_rows[2]),
                                   double(st.records[0]), double(st.records[2]), double(st.repairs[0]),

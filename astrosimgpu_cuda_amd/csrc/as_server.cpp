@@ -195,7 +195,10 @@ private:
             auto net = std::make_unique<Network>();
             bool ok = true;
             try {
-                if (setup_where_ == AS_BUILD_HOST || !build_network_gpu(q->in.n_total, q->in.seed, *net, nullptr)) {
+                // a GPU-built network stays in device memory until its admission copies it
+                const bool try_gpu = setup_where_ == AS_BUILD_GPU_ONLY ||
+                                     (setup_where_ == AS_BUILD_GPU && q->in.n_total >= kGpuBuildAutoMin);
+                if (!try_gpu || !build_network_gpu(q->in.n_total, q->in.seed, *net, nullptr, nullptr, true)) {
                     if (setup_where_ == AS_BUILD_GPU_ONLY) throw std::runtime_error("GPU build declined");
                     build_network(q->in.n_total, q->in.seed, *net);
                 }

@@ -71,6 +71,14 @@ struct HostCSR {
     index_t n_edges() const { return index_t(src.size()); }
 };
 
+// A CSR in the memory of the GPU that generated it.
+struct DeviceCSR {
+    const index_t* row_start = nullptr;  // n_targets + 1
+    const index_t* src = nullptr;        // n_edges
+    index_t n_targets = 0;
+    std::uint64_t n_edges = 0;
+};
+
 // Everything a simulation needs that depends only on (N_total, seed).
 struct Network {
     ScalingConfig cfg;
@@ -83,6 +91,20 @@ struct Network {
     std::vector<double> V_reset, b;                          // per neuron
     double build_seconds = 0.0;  // connectivity generation + inversion
     double phase_seconds[4] = {0, 0, 0, 0};  // skip+pools, edge generation, inversion, parameters
+    // Connectivity kept on the GPU by build_network_gpu (keep_on_device):
+    // the four CSRs (exc, inh, sic, e2a) in device memory, freed with the
+    // last copy of dev_owner; the host CSRs above are then empty. Only a
+    // batch on the same GPU consumes such a network.
+    DeviceCSR dev[4];
+    std::shared_ptr<void> dev_owner;
+    bool on_device() const { return dev_owner != nullptr; }
+    bool host_csrs() const { return !on_device() || !inc_exc.row_start.empty(); }
+    // Edges of CSR `which` (0 exc, 1 inh, 2 sic, 3 e2a), wherever it is held.
+    std::uint64_t edges(int which) const {
+        if (on_device()) return dev[which].n_edges;
+        const HostCSR* c[4] = {&inc_exc, &inc_inh, &inc_sic, &inc_e2a};
+        return c[which]->n_edges();
+    }
 };
 
 // Builds the network for (n_total, seed). Throws std::invalid_argument for
@@ -94,7 +116,18 @@ void build_network(index_t n_total, std::uint64_t seed, Network& net);
 // in *why and `net` untouched, for networks that method declines (too small
 // for its draw statistics, or a rare draw pattern it does not resolve); the
 // caller then uses build_network.
-bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::string* why);
+// Smallest N_total for which the automatic choice (AS_BUILD_GPU) takes the
+// GPU generator: its latency is about constant, the host walk's grows with
+// the network, and they cross near here on one MI355X and its host.
+constexpr index_t kGpuBuildAutoMin = 100000;
+
+namespace gb { struct BuildStats; }
+bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::string* why,
+                       gb::BuildStats* stats = nullptr, bool keep_on_device = false);
+
+// Fills the host CSRs of a network whose connectivity build_network_gpu
+// kept on the device (a copy; the device CSRs stay). No-op otherwise.
+void materialize_host_csrs(Network& net);
 
 // Astrocyte pools drawn from `rng` ([neuron][min(5, n_astro)] astrocyte ids).
 // Reference: src/astrosim_fused.cu:651-671.

@@ -73,7 +73,8 @@ typedef struct as_network_info {
 /* Where as_network_build_on generates the connectivity. */
 typedef enum as_build_where {
     AS_BUILD_HOST = 0,     /* sequential host walk (as_network_build) */
-    AS_BUILD_GPU = 1,      /* parallel generator on the current GPU; the host walk when it declines */
+    AS_BUILD_GPU = 1,      /* parallel generator on the current GPU from N_total = 100000 on (below,
+                              and where it declines, the host walk, which is then as fast) */
     AS_BUILD_GPU_ONLY = 2  /* parallel generator only: AS_ERR_INVALID when it declines */
 } as_build_where;
 

@@ -101,6 +101,10 @@ def build_compare(n_total: int, seed: int, gpu: bool) -> tuple[int, dict]:
     """The parallel connectivity generator (host emulation, or the GPU build)
     against the sequential host builder: (0 identical / 1 different / 2
     declined / -1 error, statistics)."""
-    st = (C.c_double * 16)()
+    st = (C.c_double * 32)()
     rc = load().as_selftest_build_compare(n_total, seed, 1 if gpu else 0, st)
-    return rc, dict(zip(BUILD_STATS, list(st)))
+    d = dict(zip(BUILD_STATS, list(st)))
+    d["marks"] = list(st)[16:27]
+    d["pools_seconds"], d["params_wait_seconds"] = st[27], st[28]
+    d["max_b_draws"], d["max_repair_rows"] = st[29], st[30]
+    return rc, d

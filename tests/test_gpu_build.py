@@ -35,3 +35,26 @@ def test_gpu_only_declines_small_networks():
     with pytest.raises(Exception):
         A.Network(100, 1, setup="gpu-only")
     assert not A.Network(100, 1, setup="gpu").built_on_gpu
+    # the automatic choice keeps mid-size networks on the host, large ones on the GPU
+    assert not A.Network(20000, 1, setup="gpu").built_on_gpu
+    assert A.Network(100000, 1, setup="gpu").built_on_gpu
+
+
+@pytest.mark.parametrize("method,precision", [("plain", "fp64"), ("blocked", "fp64"), ("blocked", "fp32c")])
+def test_device_resident_network_simulates_like_host_built(method, precision):
+    # the GPU-built network stays in device memory and the engine copies it
+    # device to device; the simulation must equal one of the host-built network
+    print(f"device-resident network, {method} {precision}", flush=True)
+    host = A.Network(24000, 17, setup="host")
+    gpu = A.Network(24000, 17, setup="gpu-only")
+    a = A.Simulation(host, method=method, precision=precision)
+    b = A.Simulation(gpu, method=method, precision=precision)
+    a.step(400)
+    b.step(400)
+    assert a.spike_count() == b.spike_count() > 0
+    for f in ("V", "Ca"):
+        assert np.array_equal(a.state(f), b.state(f)), f
+    # the host copy of the connectivity, made on first request, is the same too
+    for which in ("exc", "sic"):
+        for x, y in zip(host.csr(which), gpu.csr(which)):
+            assert np.array_equal(x, y)
