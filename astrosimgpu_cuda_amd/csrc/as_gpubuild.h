@@ -146,6 +146,14 @@ AS_HD int walker_step(Walker& w, u32 e, const PassDesc& P, u32 self) {
     return (edge ? kEvEdge : 0) | ((!s && !t) ? kEvTf : 0) | (end ? kEvRowEnd : 0);
 }
 
+// w = go ? nw : w, field by field (a whole-struct select is kept in memory).
+AS_HD void commit(Walker& w, const Walker& nw, bool go) {
+    w.c = go ? nw.c : w.c;
+    w.pos = go ? nw.pos : w.pos;
+    w.phase = go ? nw.phase : w.phase;
+    w.last = go ? nw.last : w.last;
+}
+
 // Decoded draws for counters [base, base + len).
 struct Table {
     const u32* e;
@@ -177,28 +185,41 @@ AS_HD void load_chunk(const Table& t, u64 chunk, Quad (&q)[4]) {
 
 // Calls step(e) with the entries of counters c, c + 1, ... while it returns
 // true. Returns false when the table ends first. The table's storage holds
-// whole chunks (its length rounded up to kChunk, plus one chunk).
+// whole chunks (its length rounded up to kChunk, plus one chunk). The walk
+// leaves the unrolled chunk body only at its end (a `live` flag guards each
+// step), which keeps the compiled loop free of per-step exit paths.
 template <class Step>
 AS_HD bool drive(const Table& t, u64 c, Step&& step) {
     if (c < t.base) return false;
-    u64 k = c - t.base;
+    const u64 k = c - t.base;
     if (k >= t.len) return false;
     u64 chunk = k / kChunk;
     u32 first = static_cast<u32>(k % kChunk);
     Quad cur[4];
     load_chunk(t, chunk, cur);
+    bool live = true;
     for (;;) {
         // here chunk * kChunk < len, so chunk + 1 lies inside the storage
         Quad nxt[4];
         load_chunk(t, chunk + 1, nxt);
         const u64 k0 = chunk * kChunk;
-#pragma unroll
-        for (u32 j = 0; j < kChunk; ++j) {
-            if (j >= first) {
-                if (k0 + j >= t.len) return false;
-                if (!step(cur[j / 4].v[j % 4])) return true;
-            }
-        }
+        bool ran_out = false;
+        // written out sixteen times (constant indices keep cur in registers)
+#define AS_GB_STEP(j)                                      \
+    if (live && (j) >= first) {                            \
+        if (k0 + (j) < t.len) {                            \
+            live = step(cur[(j) / 4].v[(j) % 4]);          \
+        } else {                                           \
+            live = false;                                  \
+            ran_out = true;                                \
+        }                                                  \
+    }
+        AS_GB_STEP(0) AS_GB_STEP(1) AS_GB_STEP(2) AS_GB_STEP(3)
+        AS_GB_STEP(4) AS_GB_STEP(5) AS_GB_STEP(6) AS_GB_STEP(7)
+        AS_GB_STEP(8) AS_GB_STEP(9) AS_GB_STEP(10) AS_GB_STEP(11)
+        AS_GB_STEP(12) AS_GB_STEP(13) AS_GB_STEP(14) AS_GB_STEP(15)
+#undef AS_GB_STEP
+        if (!live) return !ran_out;
         ++chunk;
         first = 0;
         if (chunk * kChunk >= t.len) return false;
@@ -226,25 +247,35 @@ AS_HD u32 chunk_bits(const Table& t, const Bitmap& b, u64 chunk) {
 template <class Step>
 AS_HD bool drive_bits(const Table& t, const Bitmap& b, u64 c, Step&& step) {
     if (c < t.base) return false;
-    u64 k = c - t.base;
+    const u64 k = c - t.base;
     if (k >= t.len) return false;
     u64 chunk = k / kChunk;
     u32 first = static_cast<u32>(k % kChunk);
     Quad cur[4];
     load_chunk(t, chunk, cur);
     u32 bcur = chunk_bits(t, b, chunk);
+    bool live = true;
     for (;;) {
         Quad nxt[4];
         load_chunk(t, chunk + 1, nxt);
         const u32 bnxt = chunk_bits(t, b, chunk + 1);
         const u64 k0 = chunk * kChunk;
-#pragma unroll
-        for (u32 j = 0; j < kChunk; ++j) {
-            if (j >= first) {
-                if (k0 + j >= t.len) return false;
-                if (!step(cur[j / 4].v[j % 4], (bcur >> j) & 1u)) return true;
-            }
-        }
+        bool ran_out = false;
+#define AS_GB_STEP(j)                                                  \
+    if (live && (j) >= first) {                                        \
+        if (k0 + (j) < t.len) {                                        \
+            live = step(cur[(j) / 4].v[(j) % 4], (bcur >> (j)) & 1u);  \
+        } else {                                                       \
+            live = false;                                              \
+            ran_out = true;                                            \
+        }                                                              \
+    }
+        AS_GB_STEP(0) AS_GB_STEP(1) AS_GB_STEP(2) AS_GB_STEP(3)
+        AS_GB_STEP(4) AS_GB_STEP(5) AS_GB_STEP(6) AS_GB_STEP(7)
+        AS_GB_STEP(8) AS_GB_STEP(9) AS_GB_STEP(10) AS_GB_STEP(11)
+        AS_GB_STEP(12) AS_GB_STEP(13) AS_GB_STEP(14) AS_GB_STEP(15)
+#undef AS_GB_STEP
+        if (!live) return !ran_out;
         ++chunk;
         first = 0;
         if (chunk * kChunk >= t.len) return false;
@@ -303,7 +334,7 @@ struct ChainPass {
     u32 chain0;    // global index of chain 0
 };
 
-enum : u32 { kChainOk = 0, kChainTable = 1, kChainCapA = 2, kChainRegion = 3, kChainLast = 4 };
+enum : u32 { kChainOk = 0, kChainTable = 1, kChainCapA = 2, kChainRegion = 3, kChainLast = 4, kChainCapB = 5 };
 
 struct ChainArgs {
     Table T;
@@ -316,7 +347,10 @@ struct ChainArgs {
     Walker* state;  // chain_a's final state (first draw at or past a_{j+1})
     u32* status_a;
     u32* status_b;
-    u32* nB;        // row starts chain_b passed before coupling
+    u32 capB;
+    u32* B;         // [chain][capB]: the first row starts chain_b passed before coupling, offsets from a_{j+1}
+    Walker* over;   // where a chain with more than capB of them stood at its (capB+1)-th
+    u32* nB;        // all row starts chain_b passed before coupling
     u64* couple_x;  // the coupling row start
     u32* couple_k;  // the chain (index within the pass) that recorded it
     u32* couple_r;  // its rank in that chain's list
@@ -365,21 +399,24 @@ AS_HD void chain_b(const ChainArgs& a, u64 g) {
     const u64 rend = cp.r0 + u64(cp.n_chains) * a.W;
     Walker w = a.state[g];
     u32 n = 0, st = kChainTable;
+    const u64 a1 = cp.r0 + (j + 1) * a.W;
+    u32* B = a.B + g * a.capB;
     const Bitmap bm{a.abits[chain_pass_of(a, g)], cp.r0, u64(cp.n_chains) * a.W};
     drive_bits(a.T, bm, w.c, [&](u32 e, u32 bit) {
-        if (w.pos == 0 && w.phase == 0) {
-            if (bit) {
-                st = kChainOk;
-                return false;
-            }
-            if (w.c >= rend) {
-                st = kChainRegion;
-                return false;
-            }
+        const bool rs = w.pos == 0 && w.phase == 0;
+        const bool done = rs && bit;
+        const bool region = rs && !bit && w.c >= rend;
+        st = done ? kChainOk : (region ? kChainRegion : st);
+        const bool go = !(done || region);
+        if (rs && go) {
+            if (n < a.capB) B[n] = static_cast<u32>(w.c - a1);
+            if (n == a.capB) a.over[g] = w;
             ++n;
         }
-        walker_step(w, e, cp.P, kNone);
-        return true;
+        Walker nw = w;
+        walker_step(nw, e, cp.P, kNone);
+        commit(w, nw, go);
+        return go;
     });
     if (st == kChainOk) {
         // the chain that recorded this row start, and its rank there
@@ -419,11 +456,15 @@ AS_HD void path_write(const PathArgs& a, u64 idx) {
     const u64 j = pe.g - cp.chain0;
     const u64 a0 = cp.r0 + j * a.C.W;
     const u32* A = a.C.A + u64(pe.g) * a.C.capA;
+    const u32* B = a.C.B + u64(pe.g) * a.C.capB;
     u64 o = pe.out;
     for (u32 i = 0; i < pe.na; ++i) a.L[o++] = a0 + A[pe.rank + i];
-    if (pe.nb == 0) return;
-    Walker w = a.C.state[pe.g];
-    u32 n = 0;
+    const u32 stored = pe.nb < a.C.capB ? pe.nb : a.C.capB;
+    for (u32 i = 0; i < stored; ++i) a.L[o++] = a0 + a.C.W + B[i];
+    if (pe.nb <= a.C.capB) return;
+    // the rest again from where the chain stood at its (capB+1)-th start
+    Walker w = a.C.over[pe.g];
+    u32 n = stored;
     const bool stopped = drive(a.T, w.c, [&](u32 e) {
         if (w.pos == 0 && w.phase == 0) {
             a.L[o++] = w.c;
@@ -486,9 +527,10 @@ struct RepairArgs {
     const u64* rec_i;
     const int* rec_off;
     u32 cap_rows;
-    u64* out_j;   // list index where the exact walk rejoins (kNoSelf: none; kPassEnd)
-    u32* out_m;   // rows walked up to there
-    // write mode: for record sel[idx], the intermediate starts go to F[pos[idx] ...]
+    u64* out_j;    // list index where the exact walk rejoins (kNoSelf: none; kPassEnd)
+    u32* out_m;    // rows walked up to there
+    u64* rows;     // [record][cap_rows]: the row starts it passed (and, at the pass end, the end)
+    // copy mode: for record sel[idx], its starts go to F[pos[idx] ...]
     const u32* sel;
     const u64* pos;
     u64* F;
@@ -499,42 +541,37 @@ constexpr u64 kPassEnd = ~u64(0) - 1;
 
 // Exact walk from list row i whose true index is i + off, row after row,
 // until a row start is again in the list, or through the pass's last row
-// (out_j = kPassEnd; in write mode the pass's end is written too).
-AS_HD void repair_walk(const RepairArgs& a, u32 rec, u64* out) {
+// (out_j = kPassEnd; the pass's end is then stored as the last start).
+AS_HD void repair_row(const RepairArgs& a, u64 rec) {
     const u64 i = a.rec_i[rec];
     const i64 t = static_cast<i64>(i) + a.rec_off[rec];
-    if (out == nullptr) a.out_j[rec] = kNoSelf;
+    a.out_j[rec] = kNoSelf;
     if (t < 0) return;
-    const u64 x0 = a.L[i];
+    u64* out = a.rows + rec * a.cap_rows;
     u32 rows = 0, res = 0;  // res: 1 rejoined, 2 pass end, 3 too long
     u32 self = a.P.self_base + static_cast<u32>(t);
-    Walker w{x0, 0, 0, 0, 0};
+    Walker w{a.L[i], 0, 0, 0, 0};
     const bool stopped = drive_bits(a.T, a.lbits, w.c, [&](u32 e, u32 bit) {
-        if (w.pos == 0 && w.phase == 0 && rows > 0) {
-            if (static_cast<u64>(t) + rows >= a.P.n_rows) {
-                if (out != nullptr) out[rows - 1] = w.c;
-                res = 2;
-                return false;
-            }
-            if (bit) {
-                res = 1;
-                return false;
-            }
-            if (rows >= a.cap_rows) {
-                res = 3;
-                return false;
-            }
-            if (out != nullptr) out[rows - 1] = w.c;
-            self = a.P.self_base + static_cast<u32>(t) + rows;
-        }
-        if (walker_step(w, e, a.P, self) & kEvRowEnd) ++rows;
-        return true;
+        // at a row start after the first row: the pass's end, a start of the
+        // list, or (room permitting: out[rows - 1] < cap_rows) one to keep
+        const bool rs = w.pos == 0 && w.phase == 0 && rows > 0;
+        const bool end = rs && static_cast<u64>(t) + rows >= a.P.n_rows;
+        const bool join = rs && !end && bit;
+        const bool full = rs && !join && rows >= a.cap_rows;
+        res = full ? 3u : (end ? 2u : (join ? 1u : res));
+        if (rs && !join && !full) out[rows - 1] = w.c;
+        const bool go = !(end || join || full);
+        self = rs ? a.P.self_base + static_cast<u32>(t) + rows : self;
+        Walker nw = w;
+        const int ev = walker_step(nw, e, a.P, self);
+        rows += (go && (ev & kEvRowEnd)) ? 1u : 0u;
+        commit(w, nw, go);
+        return go;
     });
     if (!stopped) {
         atomic_or_u32(a.fail, 8u);
         return;
     }
-    if (out != nullptr) return;
     if (res == 2) {
         a.out_j[rec] = kPassEnd;
         a.out_m[rec] = rows;
@@ -545,6 +582,15 @@ AS_HD void repair_walk(const RepairArgs& a, u32 rec, u64* out) {
             a.out_m[rec] = rows;
         }
     }
+}
+
+// The starts a chosen repair passed: rows - 1 of them when it rejoined the
+// list, `rows` (the last is the pass's end) when it reached the pass's end.
+AS_HD void repair_copy(const RepairArgs& a, u64 idx) {
+    const u32 rec = a.sel[idx];
+    const u32 n = a.out_j[rec] == kPassEnd ? a.out_m[rec] : a.out_m[rec] - 1;
+    const u64* src = a.rows + u64(rec) * a.cap_rows;
+    for (u32 k = 0; k < n; ++k) a.F[a.pos[idx] + k] = src[k];
 }
 
 // Bits of the list entries [lo, hi) that lie in the bitmap's range.
@@ -560,8 +606,6 @@ AS_HD void mark_list(const MarkArgs& a, u64 idx) {
     atomic_or_u32(a.w + ((x - a.base) >> 5), 1u << ((x - a.base) & 31));
 }
 
-AS_HD void repair_row(const RepairArgs& a, u64 rec) { repair_walk(a, static_cast<u32>(rec), nullptr); }
-AS_HD void repair_write(const RepairArgs& a, u64 idx) { repair_walk(a, a.sel[idx], a.F + a.pos[idx]); }
 
 // F[o + k] = L[s + k] for the splice table's copy segments.
 struct SpliceSeg {

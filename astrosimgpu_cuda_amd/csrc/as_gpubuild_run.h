@@ -66,7 +66,7 @@ struct ChainBF { ChainArgs a; AS_HD void operator()(u64 i) const { chain_b(a, i)
 struct PathF { PathArgs a; AS_HD void operator()(u64 i) const { path_write(a, i); } };
 struct VerifyF { VerifyArgs a; AS_HD void operator()(u64 i) const { verify_row(a, i); } };
 struct RepairF { RepairArgs a; AS_HD void operator()(u64 i) const { repair_row(a, i); } };
-struct RepairWriteF { RepairArgs a; AS_HD void operator()(u64 i) const { repair_write(a, i); } };
+struct RepairCopyF { RepairArgs a; AS_HD void operator()(u64 i) const { repair_copy(a, i); } };
 struct CopyF { CopyArgs a; AS_HD void operator()(u64 i) const { splice_copy(a, i); } };
 struct CountF { EmitArgs a; AS_HD void operator()(u64 i) const { exact_row<false>(a, i); } };
 struct EmitF { EmitArgs a; AS_HD void operator()(u64 i) const { exact_row<true>(a, i); } };
@@ -118,7 +118,13 @@ struct Declined {
 constexpr u64 kW = 4096;
 constexpr int kK = 12;
 constexpr u32 kRecordCap = 1u << 18;
-constexpr u32 kRepairRows = 1u << 14;
+constexpr u32 kRepairRows = 2048;
+// Row starts stored per chain past its region (more are walked again when
+// the path needs them); a build may lower it to exercise that walk.
+#ifndef AS_GB_CAP_B
+#define AS_GB_CAP_B 2048
+#endif
+constexpr u32 kCapB = AS_GB_CAP_B;
 constexpr u32 kAmbCap = 1u << 16;
 
 // Builds the four target-major CSRs of `net` (cfg and seed already set) from
@@ -215,6 +221,9 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
         C.status_a = ar.template get<u32>(G);
         C.status_b = ar.template get<u32>(G);
         C.nB = ar.template get<u32>(G);
+        C.capB = kCapB;
+        C.B = ar.template get<u32>(G * kCapB);
+        C.over = ar.template get<Walker>(G);
         C.couple_x = ar.template get<u64>(G);
         C.couple_k = ar.template get<u32>(G);
         C.couple_r = ar.template get<u32>(G);
@@ -308,12 +317,16 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
             u64 total = m;
             u64 g = g_in, r = r_in;
             while (total < need) {
-                if (sa[g] != kChainOk) throw Declined{"chain walk failed on the path"};
+                if (sa[g] != kChainOk)
+                    throw Declined{"chain walk failed on the path (status " + std::to_string(sa[g]) + ")"};
                 PathEntry pe{static_cast<u32>(g), static_cast<u32>(r), 0u, 0u, total};
                 pe.na = static_cast<u32>(std::min<u64>(nA[g] - r, need - total));
                 total += pe.na;
                 if (total < need) {
-                    if (sb[g] != kChainOk) throw Declined{"chains did not couple on the path"};
+                    if (sb[g] != kChainOk)
+                        throw Declined{"chains did not couple on the path (status " + std::to_string(sb[g]) +
+                                       ", pass " + std::to_string(q) + ", chain " + std::to_string(g - cp.chain0) +
+                                       " of " + std::to_string(cp.n_chains) + ")"};
                     pe.nb = static_cast<u32>(std::min<u64>(nB[g], need - total));
                     total += pe.nb;
                 }
@@ -355,10 +368,11 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
                 bk.zero(lw, lwords * sizeof(u32));
                 bk.run(need - m, MarkF{MarkArgs{L, m, lw, cp.r0, lwords * 32}});
                 RepairArgs ra{T, Pq, L, need, Bitmap{lw, cp.r0, lwords * 32}, rec_i, rec_off, kRepairRows, nullptr,
-                              nullptr, nullptr, nullptr, nullptr, flags + 1};
+                              nullptr, nullptr, nullptr, nullptr, nullptr, flags + 1};
                 if (nrec > 0) {
                     ra.out_j = ar.template get<u64>(nrec);
                     ra.out_m = ar.template get<u32>(nrec);
+                    ra.rows = ar.template get<u64>(u64(nrec) * kRepairRows);
                     bk.run(nrec, RepairF{ra});
                     bk.d2h(hflags, flags, sizeof hflags);
                     if (hflags[1] != 0) throw Declined{"repair walk left the table"};
@@ -425,7 +439,7 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
                     ra.sel = dsel;
                     ra.pos = dpos;
                     ra.F = Fq;
-                    bk.run(sel.size(), RepairWriteF{ra});
+                    bk.run(sel.size(), RepairCopyF{ra});
                 }
                 F[q] = Fq;
             }
