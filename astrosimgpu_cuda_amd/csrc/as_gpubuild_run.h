@@ -120,11 +120,8 @@ constexpr int kK = 12;
 constexpr u32 kRecordCap = 1u << 18;
 constexpr u32 kRepairRows = 2048;
 // Row starts stored per chain past its region (more are walked again when
-// the path needs them); a build may lower it to exercise that walk.
-#ifndef AS_GB_CAP_B
-#define AS_GB_CAP_B 2048
-#endif
-constexpr u32 kCapB = AS_GB_CAP_B;
+// the path needs them; a test lowers the count to exercise that walk).
+constexpr u32 kCapB = 2048;
 constexpr u32 kAmbCap = 1u << 16;
 
 // Builds the four target-major CSRs of `net` (cfg and seed already set) from
@@ -138,7 +135,7 @@ constexpr u32 kAmbCap = 1u << 16;
 template <class BK>
 bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, const std::vector<index_t>& pool,
                         u32 take, Network& net, BuildStats& st, std::string& why,
-                        std::vector<void*>* kept = nullptr) {
+                        std::vector<void*>* kept = nullptr, u32 cap_b = kCapB) {
     const auto t0 = std::chrono::steady_clock::now();
     auto mark = [&](int k) {
         st.mark[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -221,8 +218,8 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
         C.status_a = ar.template get<u32>(G);
         C.status_b = ar.template get<u32>(G);
         C.nB = ar.template get<u32>(G);
-        C.capB = kCapB;
-        C.B = ar.template get<u32>(G * kCapB);
+        C.capB = cap_b;
+        C.B = ar.template get<u32>(G * cap_b);
         C.over = ar.template get<Walker>(G);
         C.couple_x = ar.template get<u64>(G);
         C.couple_k = ar.template get<u32>(G);

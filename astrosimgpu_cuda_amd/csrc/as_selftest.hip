@@ -153,8 +153,11 @@ bool same_csr(const astrosim::HostCSR& a, const astrosim::HostCSR& b) {
 // builder seconds, edges compared, mask of what differs (1 exc, 2 inh, 4 sic,
 // 8 e2a, 16 parameters), then the generator's phase marks (decode, chains,
 // passes 0-2, counts, emission, four inversions; seconds from its start),
-// the pools' seconds and the wait for the parameter draws.
-extern "C" int as_selftest_build_compare(uint32_t n_total, uint64_t seed, int where, double* stats) {
+// the pools' seconds and the wait for the parameter draws. cap_b (0: the
+// default) is the number of row starts a chain stores past its region on
+// the host backend; a small one makes most chains walk the rest again.
+extern "C" int as_selftest_build_compare(uint32_t n_total, uint64_t seed, int where, double* stats,
+                                         uint32_t cap_b) {
     using namespace astrosim;
     try {
         Network ref;
@@ -174,7 +177,8 @@ extern "C" int as_selftest_build_compare(uint32_t n_total, uint64_t seed, int wh
             const std::vector<index_t> pool = make_pools(cfg, rng);
             const u32 take = static_cast<u32>(std::min<index_t>(ScalingConfig::kPoolSize, cfg.n_astro));
             HostBackend bk;
-            ok = build_connectivity(bk, cfg, seed, rng.counter(), pool, take, got, st, why);
+            ok = build_connectivity(bk, cfg, seed, rng.counter(), pool, take, got, st, why, nullptr,
+                                    cap_b == 0 ? kCapB : cap_b);
         } else {
             ok = build_network_gpu(n_total, seed, got, &why, &st);
             st.seconds = got.build_seconds;

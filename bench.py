@@ -182,14 +182,14 @@ def ensemble_leg(pre_ms: float = 100.0, sim_ms: float = 500.0, n_sample: int = 8
     }
 
 
-def serving_leg(duration_s: float) -> dict:
+def serving_leg(duration_s: float, blocks_per_round: int = 4) -> dict:
     """Serving load (40 req/s Poisson arrivals, N_total log-uniform in
     [1e3, 1e6], sim time uniform in [200, 2000] ms, 20% FP64, own seeds)
     replayed in real time for `duration_s` seconds of arrivals against the
     continuous-batching server; latency is submit -> result."""
     from astrosimgpu_cuda_amd import Server, serving
     load = serving.poisson_load(duration_s=duration_s)
-    with Server(hbm_fraction=0.5, fp64_share=0.35, build_threads=12, blocks_per_round=4) as srv:
+    with Server(hbm_fraction=0.5, fp64_share=0.35, build_threads=12, blocks_per_round=blocks_per_round) as srv:
         out = serving.replay(srv, load, timeout_s=900.0)
     st = out["stats"]
     log(f"serving: {st['completed']}/{len(load)} done, p50 {st['p50_latency_s']:.2f} s, "
@@ -201,6 +201,7 @@ def serving_leg(duration_s: float) -> dict:
     q = lambda v, p: float(sorted(v)[min(len(v) - 1, int(p * len(v)))]) if v else None
     return {
         "arrival_rate_hz": 40.0, "arrival_window_s": duration_s, "requests": len(load),
+        "blocks_per_round": blocks_per_round,
         "completed": st["completed"], "rejected": st["rejected"], "all_done": st["all_done"],
         "p50_latency_s": st["p50_latency_s"], "p99_latency_s": st["p99_latency_s"],
         "mean_latency_s": st["mean_latency_s"], "sims_per_s": st["sims_per_s"],
@@ -262,6 +263,8 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=10.0,
                     help="arrival window of the serving load (0 skips it; the full load is 120 s)")
+    ap.add_argument("--serving-blocks", type=int, default=4,
+                    help="1 ms blocks the server advances between admission checks")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the known-size copy that HBM counter readings are scaled by")
     args = ap.parse_args()
@@ -318,7 +321,7 @@ def main() -> int:
     ens = None
     if not args.no_ensemble:
         ens = ensemble_leg()
-    srv = serving_leg(args.serving_seconds) if args.serving_seconds > 0 else None
+    srv = serving_leg(args.serving_seconds, args.serving_blocks) if args.serving_seconds > 0 else None
     line = {
         "metric": "steps/s at N=10M FP32 vs FP64 (1 GPU); ensemble sims/s + p50/p99 latency",
         "value": head["steps_per_s"],

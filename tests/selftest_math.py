@@ -32,7 +32,8 @@ def load():
             fn.restype = C.c_int
             fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
         lib.as_selftest_build_compare.restype = C.c_int
-        lib.as_selftest_build_compare.argtypes = [C.c_uint32, C.c_uint64, C.c_int, C.POINTER(C.c_double)]
+        lib.as_selftest_build_compare.argtypes = [C.c_uint32, C.c_uint64, C.c_int, C.POINTER(C.c_double),
+                                                  C.c_uint32]
         _lib = lib
     return _lib
 
@@ -97,12 +98,13 @@ BUILD_STATS = ["counters", "chains", "ambiguous", "host_rows0", "host_rows1", "h
                "records2", "repairs0", "repairs2", "max_shift", "gen_seconds", "host_seconds", "edges", "diff"]
 
 
-def build_compare(n_total: int, seed: int, gpu: bool) -> tuple[int, dict]:
+def build_compare(n_total: int, seed: int, gpu: bool, cap_b: int = 0) -> tuple[int, dict]:
     """The parallel connectivity generator (host emulation, or the GPU build)
     against the sequential host builder: (0 identical / 1 different / 2
-    declined / -1 error, statistics)."""
+    declined / -1 error, statistics). cap_b: row starts a chain stores past
+    its region in the emulation (0: the default)."""
     st = (C.c_double * 32)()
-    rc = load().as_selftest_build_compare(n_total, seed, 1 if gpu else 0, st)
+    rc = load().as_selftest_build_compare(n_total, seed, 1 if gpu else 0, st, cap_b)
     d = dict(zip(BUILD_STATS, list(st)))
     d["marks"] = list(st)[16:27]
     d["pools_seconds"], d["params_wait_seconds"] = st[27], st[28]

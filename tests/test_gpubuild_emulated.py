@@ -23,3 +23,11 @@ def test_small_networks_are_declined():
     # p_primary = 1 (no geometric skips) or too few astrocytes for rejection pools
     rc, _ = S.build_compare(100, 1, gpu=False)
     assert rc == 2
+
+
+@pytest.mark.parametrize("n_total,seed", [(4096, 6), (33333, 7)])
+def test_chains_walking_again_past_their_stored_starts(n_total, seed):
+    # with 8 stored starts per chain almost every chain on the path is
+    # stitched partly from a second walk from where it stood at its ninth
+    rc, st = S.build_compare(n_total, seed, gpu=False, cap_b=8)
+    assert rc == 0, (rc, st)
