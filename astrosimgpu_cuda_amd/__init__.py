@@ -99,7 +99,7 @@ class _ServerStats(C.Structure):
                 ("p50_latency_s", C.c_double), ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double),
                 ("sims_per_s", C.c_double), ("device_bytes", C.c_uint64), ("peak_active", C.c_uint64),
                 ("peak_resident_cells", C.c_uint64), ("capacity_cells", C.c_uint64), ("gpu_busy_s", C.c_double),
-                ("cell_steps", C.c_double)]
+                ("cell_steps", C.c_double), ("admit_s", C.c_double), ("retire_s", C.c_double)]
 
 
 _lib = None

@@ -62,6 +62,13 @@ struct Tile {
     std::uint32_t member, first, kind, end;
 };
 
+// An active member's tiles, written from table position tile0 on: its
+// astrocytes, then its excitatory, then its inhibitory neurons, each cut
+// into workgroup-sized tiles.
+struct TileSpan {
+    std::uint32_t member, na, ne, nn, tile0, pad;
+};
+
 // Everything that does not depend on the state precision.
 struct BatchDev {
     const uint2* in_rs[4];  // (begin, end) per target: neurons (exc, inh, sic), astrocytes (e2a)
@@ -945,6 +952,27 @@ __global__ void rows_kernel(const index_t* rs, index_t n, index_t ebase, uint2* 
     if (t < n) out[t] = make_uint2(ebase + rs[t], ebase + rs[t + 1]);
 }
 
+// One workgroup per active member writes its tiles (the order is that of
+// the host loops that formerly built the table).
+__global__ void tiles_kernel(const TileSpan* spans, std::uint32_t threads, Tile* tiles) {
+    const TileSpan sp = spans[blockIdx.x];
+    const std::uint32_t ta = (sp.na + threads - 1) / threads;
+    const std::uint32_t te = (sp.ne + threads - 1) / threads;
+    const std::uint32_t ti = (sp.nn - sp.ne + threads - 1) / threads;
+    for (std::uint32_t t = threadIdx.x; t < ta + te + ti; t += blockDim.x) {
+        Tile x;
+        x.member = sp.member;
+        if (t < ta) {
+            x.first = t * threads; x.kind = 0u; x.end = sp.na;
+        } else if (t < ta + te) {
+            x.first = (t - ta) * threads; x.kind = 1u; x.end = sp.ne;
+        } else {
+            x.first = sp.ne + (t - ta - te) * threads; x.kind = 2u; x.end = sp.nn;
+        }
+        tiles[sp.tile0 + t] = x;
+    }
+}
+
 __global__ void rebase_kernel(index_t* p, std::size_t n, index_t base) {
     const std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] += base;
@@ -1058,7 +1086,7 @@ struct BatchLayout {
     std::size_t astro[10], ever, neuron[12], refr;
     std::size_t spk_w, spk_v, sic_w, sic_v, nlist, alist, lcount, counters, overflow;
     std::size_t members, tiles, clock, ctx, scratch0, scratch1, scan_tmp;
-    std::size_t scan_tmp_bytes, tiles_cap;
+    std::size_t scan_tmp_bytes, tiles_cap, spans;
     std::size_t total;
 };
 
@@ -1103,6 +1131,7 @@ BatchLayout batch_layout(const BatchCapacity& c, const RunParams& rp) {
     o.members = L.add(std::size_t(c.members) * sizeof(Member));
     o.tiles_cap = (nn + 63) / 64 + (na + 63) / 64 + 3 * std::size_t(c.members);
     o.tiles = L.add(o.tiles_cap * sizeof(Tile));
+    o.spans = L.add(std::size_t(c.members) * sizeof(TileSpan));
     o.clock = L.add(sizeof(std::int64_t));
     o.ctx = L.add(2 * sizeof(IntegrateCtx<double>));  // eager and graph-replay contexts
     const std::uint64_t sm = std::max(nn, na) + 1;
@@ -1169,6 +1198,10 @@ public:
         D_.overflow = (unsigned*)dptr(lay_.overflow);
         D_.members = (const Member*)dptr(lay_.members);
         D_.tiles = (const Tile*)dptr(lay_.tiles);
+        spans_ = (TileSpan*)dptr(lay_.spans);
+        AS_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pin_members_), std::size_t(cap.members) * sizeof(Member)));
+        AS_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pin_spans_), std::size_t(cap.members) * sizeof(TileSpan)));
+        AS_HIP_CHECK(hipEventCreateWithFlags(&tables_copied_, hipEventDisableTiming));
         D_.clock = nullptr;
         clock_ = (std::int64_t*)dptr(lay_.clock);
         D_.ncap = nn;
@@ -1197,6 +1230,9 @@ public:
             (void)hipStreamSynchronize(stream_);
             (void)hipFree(base_);
         }
+        if (pin_members_) (void)hipHostFree(pin_members_);
+        if (pin_spans_) (void)hipHostFree(pin_spans_);
+        if (tables_copied_) (void)hipEventDestroy(tables_copied_);
     }
 
     bool fits(const Network& net) const override {
@@ -1517,27 +1553,35 @@ private:
         rebuild_tiles();
     }
 
+    // The member table and the tile table of the active members. The host
+    // writes the member table and one span per active member into pinned
+    // staging (after the previous copies out of it have completed) and a
+    // kernel writes the tiles, so a change costs a few kilobytes of copy
+    // and no synchronisation however many tiles the batch holds.
     void rebuild_tiles() {
-        std::vector<Tile> tiles;
-        std::vector<Member> ms(hosts_.size());
+        AS_HIP_CHECK(hipEventSynchronize(tables_copied_));
+        std::uint32_t n_active = 0;
+        std::uint64_t total = 0;
+        const std::uint32_t T = static_cast<std::uint32_t>(threads_);
         for (std::size_t i = 0; i < hosts_.size(); ++i) {
             const MemberHost& h = hosts_[i];
-            ms[i] = h.m;
+            pin_members_[i] = h.m;
             if (!h.active) continue;
-            const std::uint32_t mi = std::uint32_t(i);
-            for (index_t f = 0; f < h.m.na; f += threads_) tiles.push_back({mi, f, 0u, h.m.na});
-            for (index_t f = 0; f < h.m.ne; f += threads_) tiles.push_back({mi, f, 1u, h.m.ne});
-            for (index_t f = h.m.ne; f < h.m.nn; f += threads_) tiles.push_back({mi, f, 2u, h.m.nn});
+            const std::uint64_t nt = (h.m.na + T - 1) / T + (h.m.ne + T - 1) / T + (h.m.nn - h.m.ne + T - 1) / T;
+            pin_spans_[n_active++] = TileSpan{std::uint32_t(i), h.m.na, h.m.ne, h.m.nn, std::uint32_t(total), 0u};
+            total += nt;
         }
-        if (tiles.size() > lay_.tiles_cap) throw Error(AS_ERR_NOMEM, "tile table capacity exceeded");
-        n_tiles_ = static_cast<unsigned>(tiles.size());
-        // pageable copies are ordered with the stream and complete before return
-        if (!tiles.empty())
-            AS_HIP_CHECK(hipMemcpyAsync(const_cast<Tile*>(D_.tiles), tiles.data(), tiles.size() * sizeof(Tile),
-                                        hipMemcpyHostToDevice, stream_));
-        AS_HIP_CHECK(hipMemcpyAsync(const_cast<Member*>(D_.members), ms.data(), ms.size() * sizeof(Member),
+        if (total > lay_.tiles_cap) throw Error(AS_ERR_NOMEM, "tile table capacity exceeded");
+        n_tiles_ = static_cast<unsigned>(total);
+        AS_HIP_CHECK(hipMemcpyAsync(const_cast<Member*>(D_.members), pin_members_, hosts_.size() * sizeof(Member),
                                     hipMemcpyHostToDevice, stream_));
-        AS_HIP_CHECK(hipStreamSynchronize(stream_));
+        if (n_active > 0) {
+            AS_HIP_CHECK(hipMemcpyAsync(spans_, pin_spans_, n_active * sizeof(TileSpan), hipMemcpyHostToDevice, stream_));
+            hipLaunchKernelGGL(tiles_kernel, dim3(n_active), dim3(256), 0, stream_, spans_, T,
+                               const_cast<Tile*>(D_.tiles));
+            AS_HIP_CHECK(hipGetLastError());
+        }
+        AS_HIP_CHECK(hipEventRecord(tables_copied_, stream_));
         drop_graphs();
     }
 
@@ -1592,12 +1636,14 @@ private:
         AS_HIP_CHECK(hipGetLastError());
     }
 
+    // The copy reads a converted array that lives in staged_ until the
+    // upload's closing synchronisation.
     void upload_values(R* dst, const std::vector<double>& v, bool reciprocal = false) {
         if (v.empty()) return;
-        std::vector<R> f(v.size());
+        staged_.emplace_back(v.size());
+        std::vector<R>& f = staged_.back();
         for (std::size_t i = 0; i < v.size(); ++i) f[i] = R(reciprocal ? 1.0 / v[i] : v[i]);
         AS_HIP_CHECK(hipMemcpyAsync(dst, f.data(), f.size() * sizeof(R), hipMemcpyHostToDevice, stream_));
-        AS_HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
     // Uploads one member: rebased in-CSRs, out-CSRs built on the device,
@@ -1646,7 +1692,7 @@ private:
                                    const_cast<uint2*>(D_.out_ent[T]));
             AS_HIP_CHECK(hipGetLastError());
             AS_HIP_CHECK(hipMemsetAsync(D_.bcnt[T] + tbase[T], 0, std::size_t(nt) * sizeof(unsigned), stream_));
-            AS_HIP_CHECK(hipStreamSynchronize(stream_));
+            // the scratch arrays are reused by the next type in stream order
         }
         // parameters: FP32c stores 1/tau_IP3
         upload_values(S_.Ca_tot + m.ab, net.Ca_tot);
@@ -1677,7 +1723,9 @@ private:
             fill(D_.sic_w + std::size_t(s) * D_.acap + m.ab, std::uint32_t(0), m.na);
         }
         fill(D_.counters + std::size_t(m.slot) * kCountersPerMember, 0ull, kCountersPerMember);
+        // the network's host arrays and the staged parameters are read by now
         AS_HIP_CHECK(hipStreamSynchronize(stream_));
+        staged_.clear();
     }
 
     RunParams P_;
@@ -1699,6 +1747,11 @@ private:
     IntegrateCtx<R>* ctx_graph_ = nullptr;
     index_t *scratch0_ = nullptr, *scratch1_ = nullptr;
     void* scan_tmp_ = nullptr;
+    TileSpan* spans_ = nullptr;
+    Member* pin_members_ = nullptr;  // pinned staging of the member and span tables
+    TileSpan* pin_spans_ = nullptr;
+    hipEvent_t tables_copied_ = nullptr;
+    std::vector<std::vector<R>> staged_;  // converted parameters until their copies complete
     std::int64_t step_ = 0;
     std::map<std::int64_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
     bool timing_ = false;

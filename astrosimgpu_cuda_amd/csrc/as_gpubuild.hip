@@ -47,10 +47,15 @@ public:
         (void)hipStreamSynchronize(s_);
         (void)hipStreamDestroy(s_);
     }
+    // A failed allocation declines the build (the caller falls back to the
+    // host walk) rather than failing it.
     template <class T>
     T* alloc(u64 n) {
         void* p = nullptr;
-        GB_CHECK(hipMallocAsync(&p, n * sizeof(T), s_));
+        if (hipMallocAsync(&p, n * sizeof(T), s_) != hipSuccess) {
+            (void)hipGetLastError();
+            throw gb::Declined{"device memory exhausted (" + std::to_string(n * sizeof(T)) + " bytes)"};
+        }
         return static_cast<T*>(p);
     }
     void release(void* p) { (void)hipFreeAsync(p, s_); }
@@ -121,9 +126,13 @@ bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::s
     out.cfg = cfg;
     out.seed = seed;
     // the per-cell parameters come from their own stream: drawn meanwhile
-    std::thread params([&] { draw_cell_params(cfg, seed, out); });
-    bool ok = false;
     gb::BuildStats st;
+    std::thread params([&] {
+        const auto tp = std::chrono::steady_clock::now();
+        draw_cell_params(cfg, seed, out);
+        st.params_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tp).count();
+    });
+    bool ok = false;
     try {
         HostRng rng(seed, kSetupStream);
         rng.skip_normals(4ull * cfg.n_astro + 2ull * cfg.n_neurons());
@@ -162,6 +171,50 @@ bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::s
     out.phase_seconds[1] = st.seconds;
     net = std::move(out);
     return true;
+}
+
+void stage_csrs_on_device(Network& net) {
+    if (net.on_device()) return;
+    HostCSR* c[4] = {&net.inc_exc, &net.inc_inh, &net.inc_sic, &net.inc_e2a};
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        return;  // left on the host
+    }
+    auto* held = new std::vector<void*>();
+    bool ok = true;
+    for (int k = 0; k < 4 && ok; ++k) {
+        void* rs = nullptr;
+        void* src = nullptr;
+        const std::size_t nr = c[k]->row_start.size(), ne = c[k]->src.size();
+        ok = hipMallocAsync(&rs, nr * sizeof(index_t), s) == hipSuccess &&
+             hipMallocAsync(&src, std::max<std::size_t>(ne, 1) * sizeof(index_t), s) == hipSuccess;
+        if (rs) held->push_back(rs);
+        if (src) held->push_back(src);
+        if (!ok) break;
+        ok = hipMemcpyAsync(rs, c[k]->row_start.data(), nr * sizeof(index_t), hipMemcpyHostToDevice, s) ==
+                 hipSuccess &&
+             (ne == 0 ||
+              hipMemcpyAsync(src, c[k]->src.data(), ne * sizeof(index_t), hipMemcpyHostToDevice, s) == hipSuccess);
+        net.dev[k] = DeviceCSR{static_cast<const index_t*>(rs), static_cast<const index_t*>(src),
+                               static_cast<index_t>(nr - 1), ne};
+    }
+    ok = hipStreamSynchronize(s) == hipSuccess && ok;
+    if (!ok) {
+        (void)hipGetLastError();
+        for (void* p : *held) (void)hipFreeAsync(p, s);
+        (void)hipStreamSynchronize(s);
+        delete held;
+        for (auto& d : net.dev) d = DeviceCSR{};
+    } else {
+        net.dev_owner = std::shared_ptr<void>(held, [](void* p) {
+            auto* v = static_cast<std::vector<void*>*>(p);
+            for (void* q : *v) (void)hipFreeAsync(q, nullptr);
+            delete v;
+        });
+        for (HostCSR* h : c) *h = HostCSR();
+    }
+    (void)hipStreamDestroy(s);
 }
 
 void materialize_host_csrs(Network& net) {

@@ -42,7 +42,7 @@ struct BuildStats {
     // seconds since the start at: decode, chains, passes 0-2 stitched,
     // counts, edges emitted, the four inversions
     double mark[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    double pools_seconds = 0.0, params_wait_seconds = 0.0;
+    double pools_seconds = 0.0, params_wait_seconds = 0.0, params_seconds = 0.0;
     u64 max_b_draws = 0;      // longest chain walk past the next anchor before it met another chain
     u32 max_repair_rows = 0;  // longest repair walk, in rows
 };

@@ -180,8 +180,10 @@ extern "C" int as_selftest_build_compare(uint32_t n_total, uint64_t seed, int wh
             ok = build_connectivity(bk, cfg, seed, rng.counter(), pool, take, got, st, why, nullptr,
                                     cap_b == 0 ? kCapB : cap_b);
         } else {
-            ok = build_network_gpu(n_total, seed, got, &why, &st);
+            // as the library's own GPU builds: connectivity kept in device memory
+            ok = build_network_gpu(n_total, seed, got, &why, &st, true);
             st.seconds = got.build_seconds;
+            if (ok) materialize_host_csrs(got);
         }
         if (stats) {
             for (int k = 0; k < 11; ++k) stats[16 + k] = st.mark[k];
@@ -189,6 +191,7 @@ extern "C" int as_selftest_build_compare(uint32_t n_total, uint64_t seed, int wh
             stats[28] = st.params_wait_seconds;
             stats[29] = double(st.max_b_draws);
             stats[30] = double(st.max_repair_rows);
+            stats[31] = st.params_seconds;
             const double v[16] = {double(st.counters), double(st.chains), double(st.ambiguous),
                                   double(st.host_rows[0]), double(st.host_rows[1]), double(st.host_rows[2]),
                                   double(st.records[0]), double(st.records[2]), double(st.repairs[0]),

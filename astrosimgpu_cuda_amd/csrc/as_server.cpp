@@ -175,6 +175,8 @@ public:
         s.peak_resident_cells = peak_cells_;
         s.capacity_cells = cap_[0].neurons + cap_[0].astros + cap_[1].neurons + cap_[1].astros;
         s.gpu_busy_s = gpu_busy_s_;
+        s.admit_s = admit_s_;
+        s.retire_s = retire_s_;
         s.cell_steps = cell_steps_;
         return s;
     }
@@ -201,6 +203,8 @@ private:
                 if (!try_gpu || !build_network_gpu(q->in.n_total, q->in.seed, *net, nullptr, nullptr, true)) {
                     if (setup_where_ == AS_BUILD_GPU_ONLY) throw std::runtime_error("GPU build declined");
                     build_network(q->in.n_total, q->in.seed, *net);
+                    // the builder uploads it, not the driver at admission
+                    stage_csrs_on_device(*net);
                 }
             } catch (...) {
                 ok = false;
@@ -304,7 +308,9 @@ private:
                     });
                     if (stop_) return;
                 }
+                const auto a0 = Clock::now();
                 admit();
+                admit_s_ += std::chrono::duration<double>(Clock::now() - a0).count();
                 std::uint64_t cells = 0;
                 int active = 0;
                 const auto g0 = Clock::now();
@@ -322,7 +328,9 @@ private:
                 cell_steps_ += double(cells) * blocks * batch_[0]->block_len();
                 peak_active_ = std::max<std::uint64_t>(peak_active_, active);
                 peak_cells_ = std::max<std::uint64_t>(peak_cells_, cells);
+                const auto r0 = Clock::now();
                 retire();
+                retire_s_ += std::chrono::duration<double>(Clock::now() - r0).count();
             }
         } catch (const std::exception& e) {
             std::lock_guard<std::mutex> g(mu_);
@@ -356,7 +364,7 @@ private:
     std::vector<std::thread> builders_;
     std::thread driver_;
     std::uint64_t peak_active_ = 0, peak_cells_ = 0;
-    double gpu_busy_s_ = 0.0, cell_steps_ = 0.0;
+    double gpu_busy_s_ = 0.0, cell_steps_ = 0.0, admit_s_ = 0.0, retire_s_ = 0.0;
 };
 
 }  // namespace astrosim

@@ -402,27 +402,132 @@ std::vector<index_t> make_pools(const ScalingConfig& cfg, HostRng& rng) {
     return pool;
 }
 
-void draw_cell_params(const ScalingConfig& cfg, std::uint64_t seed, Network& net) {
-    // A fresh stream from counter 0 (src/astrosim_fused.cu:917-933).
+namespace {
+
+// The Gaussian draws of a fresh setup stream, computed window by window on
+// worker threads. normal() number k of HostRng from counter 0 is the cosine
+// (k even) or the sine (k odd) of Box-Muller pair k/2 on counters k/2*2 and
+// k/2*2+1, unless an earlier pair redrew its u1 (u1 <= 1e-300, probability
+// 2^-53 per pair): a window holding such a pair reports it, and the caller
+// then draws sequentially with HostRng.
+class NormalWindows {
+public:
+    NormalWindows(std::uint64_t seed, unsigned threads, std::uint64_t expected)
+        : seed_(seed), threads_(threads), expected_(expected) {}
+    bool normal(double& z) {
+        if (pos_ == buf_.size() && !fill()) return false;
+        z = buf_[pos_++];
+        return true;
+    }
+
+private:
+    bool fill() {
+        // the first window covers the expected draws, later ones a little more
+        const std::uint64_t pairs = next_pair_ == 0 ? expected_ / 2 + 1024
+                                                    : std::max<std::uint64_t>(4096, expected_ / 64);
+        buf_.resize(2 * pairs);
+        std::atomic<bool> redrawn{false};
+        const unsigned T = static_cast<unsigned>(std::max<std::uint64_t>(1, std::min<std::uint64_t>(threads_, pairs / 4096)));
+        parallel_for_threads(T, [&](unsigned t) {
+            for (std::uint64_t i = pairs * t / T; i < pairs * (t + 1) / T; ++i) {
+                const std::uint64_t c = 2 * (next_pair_ + i);
+                const double u1 = rng_uniform(seed_, kSetupStream, c);
+                if (u1 <= 1e-300) {
+                    redrawn = true;
+                    continue;
+                }
+                const double u2 = rng_uniform(seed_, kSetupStream, c + 1);
+                // the expressions of HostRng::normal
+                const double r = std::sqrt(-2.0 * std::log(u1));
+                const double theta = 6.283185307179586476925286766559 * u2;
+                double sn, cs;
+                ::sincos(theta, &sn, &cs);
+                buf_[2 * i] = r * cs;
+                buf_[2 * i + 1] = r * sn;
+            }
+        });
+        if (redrawn) return false;
+        next_pair_ += pairs;
+        pos_ = 0;
+        return true;
+    }
+    std::uint64_t seed_;
+    unsigned threads_;
+    std::uint64_t expected_, next_pair_ = 0;
+    std::vector<double> buf_;
+    std::size_t pos_ = 0;
+};
+
+// draw_scaled over a source of normals `nrm` (bool(double&); false: the
+// source gave out): the same bounds, tries and clamp as HostRng's
+// normal_redraw (src/astrosim_fused.cu:196-216).
+template <class Nrm>
+bool scaled_from(double mean, bool enabled, double lower, double upper, double var, Nrm& nrm, double& out) {
+    if (!enabled || var <= 0.0) {
+        out = mean;
+        return true;
+    }
+    const double lo = mean < 0.0 ? mean * upper : mean * lower;
+    const double hi = mean < 0.0 ? mean * lower : mean * upper;
+    const double stddev = var * std::fabs(mean);
+    for (int i = 0; i < 1000; ++i) {
+        double z;
+        if (!nrm(z)) return false;
+        const double v = mean + stddev * z;
+        if (v >= lo && v <= hi) {
+            out = v;
+            return true;
+        }
+    }
+    out = std::min(std::max(mean, lo), hi);
+    return true;
+}
+
+// The per-cell draws in the reference's order (src/astrosim_fused.cu:917-933).
+template <class Nrm>
+bool cell_params_from(const ScalingConfig& cfg, Nrm& nrm, Network& net) {
     const Biology bio;
     const index_t n_astro = cfg.n_astro, n_neurons = cfg.n_neurons(), n_exc = cfg.n_exc;
-    HostRng prng(seed, kSetupStream);
     net.Ca_tot.resize(n_astro); net.IP3_0.resize(n_astro);
     net.tau_IP3.resize(n_astro); net.delta_IP3.resize(n_astro);
     for (index_t i = 0; i < n_astro; ++i) {
-        net.Ca_tot[i] = draw_scaled(bio.Ca_tot, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, prng);
-        net.IP3_0[i] = draw_scaled(bio.IP3_0, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, prng);
-        net.tau_IP3[i] = draw_scaled(bio.tau_IP3, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, prng);
-        net.delta_IP3[i] = draw_scaled(bio.delta_IP3, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, prng);
+        if (!scaled_from(bio.Ca_tot, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, nrm, net.Ca_tot[i]) ||
+            !scaled_from(bio.IP3_0, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, nrm, net.IP3_0[i]) ||
+            !scaled_from(bio.tau_IP3, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, nrm, net.tau_IP3[i]) ||
+            !scaled_from(bio.delta_IP3, bio.randomize_astro, bio.ra_lower, bio.ra_upper, bio.ra_var, nrm,
+                         net.delta_IP3[i]))
+            return false;
     }
     net.V_reset.resize(n_neurons); net.b.resize(n_neurons);
     for (index_t i = 0; i < n_neurons; ++i) {
         const bool exc = i < n_exc;
-        net.V_reset[i] = draw_scaled(exc ? bio.e_V_reset : bio.i_V_reset, bio.randomize_neuron,
-                                     bio.rn_lower, bio.rn_upper, bio.rn_var, prng);
-        net.b[i] = draw_scaled(exc ? bio.e_b : bio.i_b, bio.randomize_neuron, bio.rn_lower,
-                               bio.rn_upper, bio.rn_var, prng);
+        if (!scaled_from(exc ? bio.e_V_reset : bio.i_V_reset, bio.randomize_neuron, bio.rn_lower, bio.rn_upper,
+                         bio.rn_var, nrm, net.V_reset[i]) ||
+            !scaled_from(exc ? bio.e_b : bio.i_b, bio.randomize_neuron, bio.rn_lower, bio.rn_upper, bio.rn_var, nrm,
+                         net.b[i]))
+            return false;
     }
+    return true;
+}
+
+}  // namespace
+
+void draw_cell_params(const ScalingConfig& cfg, std::uint64_t seed, Network& net) {
+    // A fresh stream from counter 0 (src/astrosim_fused.cu:917-933). About
+    // 1.88 normals per astrocyte parameter (bounds [0.9, 1.05] x mean at 10%)
+    // and 1.05 per neuron parameter ([0.9, 1.1] at 5%).
+    const std::uint64_t expected =
+        static_cast<std::uint64_t>(4.0 * 1.876 * cfg.n_astro + 2.0 * 1.048 * cfg.n_neurons()) + 64;
+    const unsigned threads = std::max(1u, std::min(worker_count(), static_cast<unsigned>(cfg.n_total / 65536u) + 1u));
+    NormalWindows win(seed, threads, expected);
+    auto from_windows = [&](double& z) { return win.normal(z); };
+    if (cell_params_from(cfg, from_windows, net)) return;
+    HostRng prng(seed, kSetupStream);
+    auto sequential = [&](double& z) {
+        z = prng.normal();
+        return true;
+    };
+    cell_params_from(cfg, sequential, net);
 }
 
 void build_network(index_t n_total, std::uint64_t seed, Network& net) {

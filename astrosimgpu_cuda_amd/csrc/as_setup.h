@@ -125,6 +125,11 @@ namespace gb { struct BuildStats; }
 bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::string* why,
                        gb::BuildStats* stats = nullptr, bool keep_on_device = false);
 
+// Moves a host-built network's CSRs into device memory of the current GPU
+// (as build_network_gpu's keep_on_device leaves them; the host CSRs are then
+// emptied). Leaves the network as it was when device memory is short.
+void stage_csrs_on_device(Network& net);
+
 // Fills the host CSRs of a network whose connectivity build_network_gpu
 // kept on the device (a copy; the device CSRs stay). No-op otherwise.
 void materialize_host_csrs(Network& net);

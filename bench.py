@@ -209,6 +209,7 @@ def serving_leg(duration_s: float, blocks_per_round: int = 4) -> dict:
         "peak_resident_cells": st["peak_resident_cells"], "capacity_cells": st["capacity_cells"],
         "hbm_occupancy_peak": st["peak_resident_cells"] / max(1, st["capacity_cells"]),
         "gpu_busy_s": st["gpu_busy_s"], "cell_steps_per_s": st["cell_steps"] / max(1e-9, st["gpu_busy_s"]),
+        "admit_s": st["admit_s"], "retire_s": st["retire_s"],
         "p50_build_s": q(build, 0.5), "p50_queue_s": q(queue, 0.5), "p50_run_s": q(run, 0.5),
         "wall_s": st["wall_s"], "network_setup": "gpu",
     }

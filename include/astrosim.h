@@ -247,6 +247,7 @@ typedef struct as_server_stats {
     uint64_t peak_resident_cells, capacity_cells;
     double gpu_busy_s;            /* wall time of the device rounds */
     double cell_steps;            /* cells x steps advanced */
+    double admit_s, retire_s;     /* driver time admitting and retiring requests between rounds */
 } as_server_stats;
 
 void as_server_config_default(as_server_config* cfg);

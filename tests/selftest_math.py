@@ -108,5 +108,5 @@ def build_compare(n_total: int, seed: int, gpu: bool, cap_b: int = 0) -> tuple[i
     d = dict(zip(BUILD_STATS, list(st)))
     d["marks"] = list(st)[16:27]
     d["pools_seconds"], d["params_wait_seconds"] = st[27], st[28]
-    d["max_b_draws"], d["max_repair_rows"] = st[29], st[30]
+    d["max_b_draws"], d["max_repair_rows"], d["params_seconds"] = st[29], st[30], st[31]
     return rc, d
