@@ -10,6 +10,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <chrono>
+#include <future>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -136,16 +137,28 @@ bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::s
     try {
         HostRng rng(seed, kSetupStream);
         rng.skip_normals(4ull * cfg.n_astro + 2ull * cfg.n_neurons());
-        const std::vector<index_t> pool = make_pools(cfg, rng);
-        const auto t_pools = std::chrono::steady_clock::now();
-        out.phase_seconds[0] = std::chrono::duration<double>(t_pools - t0).count();
-        st.pools_seconds = out.phase_seconds[0];
+        // the pools take at least kPoolSize draws per neuron: the edge draws
+        // start no earlier than that, and the GPU decodes from there while
+        // the pools are drawn
+        const std::uint64_t c0_lo = rng.counter() + std::uint64_t(ScalingConfig::kPoolSize) * cfg.n_neurons();
+        auto pools = std::async(std::launch::async, [&cfg, rng, t0, &out, &st]() mutable {
+            std::vector<index_t> pool = make_pools(cfg, rng);
+            out.phase_seconds[0] = st.pools_seconds =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            return std::make_pair(rng.counter(), std::move(pool));
+        });
+        auto pools_fn = [&pools](std::vector<index_t>& pool) {
+            auto r = pools.get();
+            pool = std::move(r.second);
+            return r.first;
+        };
         std::vector<void*> kept;
         {
             HipBackend bk;  // its destructor waits for the build's stream
-            ok = gb::build_connectivity(bk, cfg, seed, rng.counter(), pool, ScalingConfig::kPoolSize, out, st,
-                                        reason, keep_on_device ? &kept : nullptr);
+            ok = gb::build_connectivity(bk, cfg, seed, c0_lo, pools_fn, ScalingConfig::kPoolSize, out, st, reason,
+                                        keep_on_device ? &kept : nullptr);
         }
+        if (pools.valid()) pools.wait();
         if (ok && keep_on_device) {
             auto* held = new std::vector<void*>(std::move(kept));
             out.dev_owner = std::shared_ptr<void>(held, [](void* p) {

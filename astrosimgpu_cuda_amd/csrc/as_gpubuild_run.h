@@ -35,6 +35,7 @@ struct BuildStats {
     u64 chains = 0;
     u32 ambiguous = 0;     // skips decided by the C library
     u32 host_rows[3] = {0, 0, 0};  // rows walked on the host before a pass joined the chains
+    u64 host_walked = 0;           // rows walked on the host where a chain met no later one
     u32 records[3] = {0, 0, 0};    // rows with a target near their own position
     u32 repairs[3] = {0, 0, 0};    // autapses that moved the walk off the chains
     int max_shift = 0;
@@ -125,17 +126,19 @@ constexpr u32 kCapB = 2048;
 constexpr u32 kAmbCap = 1u << 16;
 
 // Builds the four target-major CSRs of `net` (cfg and seed already set) from
-// the setup stream whose edge draws start at counter c0, with the astrocyte
-// pools `pool` ([neuron][take]). Returns false with a reason when the
+// the setup stream whose edge draws start at a counter c0 >= c0_lo. The
+// astrocyte pools come from pools_fn(pool) (which fills [neuron][take] and
+// returns c0); it is first called once the draws are decoded and the chains
+// walked from c0_lo, so the pools can be drawn meanwhile. Returns false with a reason when the
 // network is outside what the method handles (the caller then builds it on
 // the host); the result, when true, equals build_network's bit for bit.
 // With `kept`, the CSRs stay in the backend's memory (net.dev; the host CSRs
 // are left empty) and their allocations are appended to *kept for the
 // caller to free.
-template <class BK>
-bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, const std::vector<index_t>& pool,
-                        u32 take, Network& net, BuildStats& st, std::string& why,
-                        std::vector<void*>* kept = nullptr, u32 cap_b = kCapB) {
+template <class BK, class Pools>
+bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0_lo, Pools&& pools_fn, u32 take,
+                        Network& net, BuildStats& st, std::string& why, std::vector<void*>* kept = nullptr,
+                        u32 cap_b = kCapB) {
     const auto t0 = std::chrono::steady_clock::now();
     auto mark = [&](int k) {
         st.mark[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -159,6 +162,7 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
         }
         auto margin = [](double var) { return u64(16.0 * std::sqrt(var)) + 4 * kW + 65536; };
         u64 r0[3], rend[3];
+        const u64 c0 = c0_lo;  // regions and the table start here; the true start is not earlier
         double start_est = double(c0), var_start = 0.0;
         for (int q = 0; q < 3; ++q) {
             const u64 lo_m = q == 0 ? 0 : margin(var_start);
@@ -250,15 +254,45 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
         }
         mark(1);
 
+        // ---- the pools, and with them the first edge draw's counter ----
+        std::vector<index_t> pool;
+        const u64 c_first = pools_fn(pool);
+        if (c_first < c0_lo) throw Declined{"pools ended before their lower bound"};
+
         // ---- per pass: stitch, repair, final row starts ----
         u64* F[3] = {nullptr, nullptr, nullptr};
-        u64 s = c0;
+        u64 s = c_first;
         std::map<u64, std::vector<u32>> a_cache;
         for (int q = 0; q < 3; ++q) {
             const PassDesc& Pq = P[q];
             const ChainPass& cp = C.pass[q];
             const bool has_self = Pq.self_base != kNone;
             const u64 need = u64(Pq.n_rows) + 1 + (has_self ? u64(kK) + 2 : 0);
+
+            // the list of row starts chain g recorded (fetched once)
+            auto a_list = [&](u64 g) -> const std::vector<u32>& {
+                auto it = a_cache.find(g);
+                if (it == a_cache.end()) {
+                    std::vector<u32> a(nA[g]);
+                    bk.d2h(a.data(), C.A + g * C.capA, a.size() * sizeof(u32));
+                    it = a_cache.emplace(g, std::move(a)).first;
+                }
+                return it->second;
+            };
+            // whether x is a row start some chain of this pass recorded: its chain and rank
+            auto recorded = [&](u64 x, u64& g_out, u64& r_out) {
+                if (x < cp.r0 || x >= rend[q]) return false;
+                const u64 k = (x - cp.r0) / kW;
+                const u64 g = cp.chain0 + k;
+                if (sa[g] != kChainOk) return false;
+                const std::vector<u32>& a = a_list(g);
+                const u32 off = static_cast<u32>(x - (cp.r0 + k * kW));
+                const auto pos = std::lower_bound(a.begin(), a.end(), off);
+                if (pos == a.end() || *pos != off) return false;
+                g_out = g;
+                r_out = static_cast<u64>(pos - a.begin());
+                return true;
+            };
 
             // exact host walk from the pass's first row start until it meets a chain
             std::vector<u64> host_rows;
@@ -271,25 +305,7 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
                     walked_all = true;
                     break;
                 }
-                if (x >= cp.r0 && x < rend[q]) {
-                    const u64 k = (x - cp.r0) / kW;
-                    const u64 g = cp.chain0 + k;
-                    if (sa[g] == kChainOk) {
-                        auto it = a_cache.find(g);
-                        if (it == a_cache.end()) {
-                            std::vector<u32> a(nA[g]);
-                            bk.d2h(a.data(), C.A + g * C.capA, a.size() * sizeof(u32));
-                            it = a_cache.emplace(g, std::move(a)).first;
-                        }
-                        const u32 off = static_cast<u32>(x - (cp.r0 + k * kW));
-                        const auto pos = std::lower_bound(it->second.begin(), it->second.end(), off);
-                        if (pos != it->second.end() && *pos == off) {
-                            g_in = g;
-                            r_in = static_cast<u64>(pos - it->second.begin());
-                            break;
-                        }
-                    }
-                }
+                if (recorded(x, g_in, r_in)) break;
                 if (host_rows.size() > (1u << 20)) throw Declined{"pass start did not meet the chains"};
                 const u32 self = has_self ? Pq.self_base + static_cast<u32>(host_rows.size()) : kNone;
                 host_rows.push_back(x);
@@ -311,6 +327,7 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
 
             // the path through the chains
             std::vector<PathEntry> ent;
+            std::vector<std::pair<u64, std::vector<u64>>> walked;  // (list position, starts) walked on the host
             u64 total = m;
             u64 g = g_in, r = r_in;
             while (total < need) {
@@ -319,22 +336,45 @@ bool build_connectivity(BK& bk, const ScalingConfig& cfg, u64 seed, u64 c0, cons
                 PathEntry pe{static_cast<u32>(g), static_cast<u32>(r), 0u, 0u, total};
                 pe.na = static_cast<u32>(std::min<u64>(nA[g] - r, need - total));
                 total += pe.na;
-                if (total < need) {
-                    if (sb[g] != kChainOk)
-                        throw Declined{"chains did not couple on the path (status " + std::to_string(sb[g]) +
-                                       ", pass " + std::to_string(q) + ", chain " + std::to_string(g - cp.chain0) +
-                                       " of " + std::to_string(cp.n_chains) + ")"};
+                if (total < need && sb[g] == kChainOk) {
                     pe.nb = static_cast<u32>(std::min<u64>(nB[g], need - total));
                     total += pe.nb;
+                    ent.push_back(pe);
+                    if (total >= need) break;
+                    r = cr[g];
+                    g = cp.chain0 + ck[g];
+                    continue;
                 }
                 ent.push_back(pe);
                 if (total >= need) break;
-                const u64 g_next = cp.chain0 + ck[g];
-                r = cr[g];
-                g = g_next;
+                // chain g met no later chain before its region ended (or is the
+                // last one): the host walks on from its last recorded start, as
+                // the chains walk (no own-target check), until it stands on a
+                // start a later chain recorded, or has enough rows
+                const std::vector<u32>& a = a_list(g);
+                u64 xw = cp.r0 + (g - cp.chain0) * kW + a.back();
+                std::vector<u64> piece;
+                bool met = false;
+                while (total + piece.size() < need) {
+                    Walker w{xw, 0, 0, 0, 0};
+                    while (!(walker_step(w, host_entry(seed, w.c, log1m_p, take), Pq, kNone) & kEvRowEnd)) {
+                    }
+                    xw = w.c;
+                    if (recorded(xw, g, r)) {
+                        met = true;
+                        break;
+                    }
+                    piece.push_back(xw);
+                }
+                st.host_walked += piece.size();
+                total += piece.size();
+                walked.emplace_back(total - piece.size(), std::move(piece));
+                if (!met) break;
             }
             u64* L = ar.template get<u64>(need);
             if (m > 0) bk.h2d(L, host_rows.data(), m * sizeof(u64));
+            for (const auto& pw : walked)
+                if (!pw.second.empty()) bk.h2d(L + pw.first, pw.second.data(), pw.second.size() * sizeof(u64));
             PathEntry* dent = ar.template get<PathEntry>(ent.size());
             bk.h2d(dent, ent.data(), ent.size() * sizeof(PathEntry));
             bk.zero(flags + 1, sizeof(u32));

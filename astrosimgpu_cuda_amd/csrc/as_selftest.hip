@@ -174,10 +174,14 @@ extern "C" int as_selftest_build_compare(uint32_t n_total, uint64_t seed, int wh
             got.seed = seed;
             HostRng rng(seed, kSetupStream);
             rng.skip_normals(4ull * cfg.n_astro + 2ull * cfg.n_neurons());
-            const std::vector<index_t> pool = make_pools(cfg, rng);
             const u32 take = static_cast<u32>(std::min<index_t>(ScalingConfig::kPoolSize, cfg.n_astro));
+            const u64 c0_lo = rng.counter() + u64(take) * cfg.n_neurons();
+            auto pools_fn = [&](std::vector<index_t>& pool) {
+                pool = make_pools(cfg, rng);
+                return rng.counter();
+            };
             HostBackend bk;
-            ok = build_connectivity(bk, cfg, seed, rng.counter(), pool, take, got, st, why, nullptr,
+            ok = build_connectivity(bk, cfg, seed, c0_lo, pools_fn, take, got, st, why, nullptr,
                                     cap_b == 0 ? kCapB : cap_b);
         } else {
             // as the library's own GPU builds: connectivity kept in device memory
