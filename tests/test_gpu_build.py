@@ -60,3 +60,17 @@ def test_device_resident_network_simulates_like_host_built(method, precision):
     for which in ("exc", "sic"):
         for x, y in zip(host.csr(which), gpu.csr(which)):
             assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("method", ["plain", "blocked"])
+def test_cli_summary_same_with_gpu_and_host_setup(method):
+    # the drop-in CLI generates the network on the GPU by default: its
+    # summary must equal the one from the host walk (timing lines aside)
+    args = ["--n", "150000", "--seed", "9", "--pre-ms", "20", "--sim-ms", "60", "--method", method]
+    timing = ("wall time", "steps/s")
+    out = {}
+    for setup in ("gpu", "host"):
+        r = A.run_cli(args + ["--setup", setup], timeout=300)
+        assert r.returncode == 0, r.stderr
+        out[setup] = [l.split(", built")[0] for l in r.stdout.splitlines() if not l.startswith(timing)]
+    assert out["gpu"] == out["host"]
