@@ -136,7 +136,23 @@ def cpu_baseline(net, seed: int, n_steps: int) -> dict:
     t0 = time.perf_counter()
     osim.step(n_steps)
     wall = time.perf_counter() - t0
+    del osim
+    # the same stepper on one core, over a few steps
+    one = O.OracleSim(onet, threads=1)
+    one.step(1)
+    n1 = max(1, n_steps // 16)
+    t1 = time.perf_counter()
+    one.step(n1)
+    wall1 = time.perf_counter() - t1
+    del one
+    model = ""
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except Exception:
+        pass
+    log(f"cpu baseline: one core {n1 / wall1:.4f} steps/s ({model})")
     return {"value": n_steps / wall, "unit": "steps/s", "cores": threads, "kind": "port",
+            "single_core": {"value": n1 / wall1, "steps": n1}, "cpu_model": model,
             "sample": f"FP64 host oracle stepper, {n_steps} steps of the same N_total={net.n_total} "
                       f"network (after 1 warm step), OpenMP over cells"}
 
