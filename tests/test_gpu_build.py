@@ -21,6 +21,15 @@ def test_gpu_build_equals_host_build(n_total, seed):
     assert rc == 0, (rc, st)
 
 
+# random sizes and seeds (one of them has a skip the C library decides)
+@pytest.mark.parametrize("n_total,seed", [(465821, 601051018), (1843701, 838826498), (136123, 869349200),
+                                          (1692482, 67822819), (110773, 498145836), (547300, 711299772)])
+def test_gpu_build_random_networks(n_total, seed):
+    print(f"gpu build compare N={n_total} seed={seed}", flush=True)
+    rc, st = S.build_compare(n_total, seed, gpu=True)
+    assert rc == 0, (rc, st)
+
+
 def test_network_setup_gpu_through_c_abi():
     host = A.Network(30011, 5, setup="host")
     gpu = A.Network(30011, 5, setup="gpu-only")
