@@ -78,7 +78,7 @@ typedef enum as_build_where {
     AS_BUILD_GPU_ONLY = 2  /* parallel generator only: AS_ERR_INVALID when it declines */
 } as_build_where;
 
-/* ---- network construction (host only, no GPU needed) ---- */
+/* ---- network construction (as_network_build: host only, no GPU needed) ---- */
 
 /* Scaling law, connectivity and per-cell parameters for (n_total, seed).
  * Replaces src/astrosim_fused.cu:846 (ScalingConfig::make), :867
@@ -206,11 +206,15 @@ void as_batch_destroy(as_batch* b);
 
 /* ---- continuous-batching server ----
  *
- * No counterpart in the reference. Requests are built on host worker
- * threads, admitted into resident FP32c / FP64 batches (capacities sized from
- * a fraction of free HBM) as soon as their footprint fits, advanced a few
- * blocks at a time with both batches' streams overlapping on the device, and
- * retired when they reach their step count. Results equal a solo run. */
+ * No counterpart in the reference. Requests are built by worker threads (by
+ * default with the GPU generator from N_total = 100000 on, see
+ * as_network_build_on), admitted into resident FP32c / FP64 batches
+ * (capacities sized from a fraction of free HBM) as soon as their footprint
+ * fits, advanced a few blocks at a time with both batches' streams
+ * overlapping on the device, and retired when they reach their step count.
+ * Results equal a solo run. The process needs more hardware queues than
+ * build_threads + 4 (GPU_MAX_HW_QUEUES, read when HIP initialises): streams
+ * that share a queue wait for each other's work. */
 typedef struct as_server as_server;
 typedef struct as_server_config {
     double dt;                /* ms */
