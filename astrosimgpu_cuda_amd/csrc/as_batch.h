@@ -9,6 +9,7 @@
 #pragma once
 
 #include <cstdint>
+#include <vector>
 #include <memory>
 
 #include "as_engine.h"
@@ -32,6 +33,15 @@ public:
     // Admits `net` to run `n_steps` steps starting at the current global
     // step, which must be a block boundary. Returns the member slot.
     virtual int add(const Network& net, std::int64_t n_steps) = 0;
+    // Reserves `net`'s ranges and slot and uploads it on the batch's upload
+    // stream, after the work enqueued so far and beside later steps; the
+    // member joins at the first block boundary after the upload completes
+    // (activate_ready), then runs `n_steps` steps. `net` must outlive the
+    // activation. Returns the member slot.
+    virtual int add_async(const Network& net, std::int64_t n_steps) = 0;
+    // At a block boundary: activates every member whose upload has completed
+    // and appends its slot to *out. Returns how many.
+    virtual int activate_ready(std::vector<int>* out) = 0;
     // Retires a member; its ranges are reused once no in-flight block can
     // still deliver its emissions.
     virtual void remove(int member) = 0;

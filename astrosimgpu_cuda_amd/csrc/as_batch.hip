@@ -1145,6 +1145,9 @@ BatchLayout batch_layout(const BatchCapacity& c, const RunParams& rp) {
 
 struct MemberHost {
     bool used = false, active = false;
+    bool staging = false;             // uploaded by add_async, not yet active
+    std::int64_t pending_steps = 0;   // its steps, counted from its activation
+    hipEvent_t up_done = nullptr;     // its upload's completion
     Member m{};
     std::uint64_t ebase[4] = {0, 0, 0, 0}, ecount[4] = {0, 0, 0, 0};
     std::int64_t free_at_block = -1;  // ranges released once the global block reaches this
@@ -1199,6 +1202,9 @@ public:
         D_.members = (const Member*)dptr(lay_.members);
         D_.tiles = (const Tile*)dptr(lay_.tiles);
         spans_ = (TileSpan*)dptr(lay_.spans);
+        AS_HIP_CHECK(hipStreamCreateWithFlags(&up_, hipStreamNonBlocking));
+        AS_HIP_CHECK(hipEventCreateWithFlags(&enqueued_, hipEventDisableTiming));
+        staged_by_slot_.resize(cap.members);
         AS_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pin_members_), std::size_t(cap.members) * sizeof(Member)));
         AS_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pin_spans_), std::size_t(cap.members) * sizeof(TileSpan)));
         AS_HIP_CHECK(hipEventCreateWithFlags(&tables_copied_, hipEventDisableTiming));
@@ -1230,6 +1236,13 @@ public:
             (void)hipStreamSynchronize(stream_);
             (void)hipFree(base_);
         }
+        if (up_) {
+            (void)hipStreamSynchronize(up_);
+            (void)hipStreamDestroy(up_);
+        }
+        for (auto& h : hosts_)
+            if (h.up_done) (void)hipEventDestroy(h.up_done);
+        if (enqueued_) (void)hipEventDestroy(enqueued_);
         if (pin_members_) (void)hipHostFree(pin_members_);
         if (pin_spans_) (void)hipHostFree(pin_spans_);
         if (tables_copied_) (void)hipEventDestroy(tables_copied_);
@@ -1245,6 +1258,14 @@ public:
 
     int add(const Network& net, std::int64_t n_steps) override {
         if (step_ % geo_.L != 0) throw Error(AS_ERR_INVALID, "members join at block boundaries");
+        const int slot = add_async(net, n_steps);
+        AS_HIP_CHECK(hipEventSynchronize(hosts_[slot].up_done));
+        activate(slot);
+        rebuild_tiles();
+        return slot;
+    }
+
+    int add_async(const Network& net, std::int64_t n_steps) override {
         if (n_steps < 0) throw Error(AS_ERR_INVALID, "n_steps must be >= 0");
         release_due();
         if (!fits(net)) throw Error(AS_ERR_NOMEM, "network does not fit in the batch's free capacity");
@@ -1259,18 +1280,42 @@ public:
         m.ab = static_cast<index_t>(aalloc_.take(na));
         m.na = na;
         m.step0 = step_;
-        m.step_end = step_ + n_steps;
+        m.step_end = step_;
         m.slot = static_cast<std::uint32_t>(slot);
         for (int i = 0; i < 4; ++i) {
             h.ecount[i] = net.edges(i);
             h.ebase[i] = ealloc_[i].take(h.ecount[i]);
         }
-        upload_member(net, h);
         h.used = true;
-        h.active = true;
+        h.active = false;
+        h.staging = true;
+        h.pending_steps = n_steps;
         h.free_at_block = -1;
-        rebuild_tiles();
+        // the ranges may have been released by a member whose last emissions
+        // are still being delivered by enqueued work: upload after it
+        AS_HIP_CHECK(hipEventRecord(enqueued_, stream_));
+        AS_HIP_CHECK(hipStreamWaitEvent(up_, enqueued_, 0));
+        upload_member(net, h, up_, staged_by_slot_[slot]);
+        if (h.up_done == nullptr) AS_HIP_CHECK(hipEventCreateWithFlags(&h.up_done, hipEventDisableTiming));
+        AS_HIP_CHECK(hipEventRecord(h.up_done, up_));
         return slot;
+    }
+
+    int activate_ready(std::vector<int>* out) override {
+        if (step_ % geo_.L != 0) return 0;
+        int n = 0;
+        for (std::size_t i = 0; i < hosts_.size(); ++i) {
+            MemberHost& h = hosts_[i];
+            if (!h.staging) continue;
+            const hipError_t q = hipEventQuery(h.up_done);
+            if (q == hipErrorNotReady) continue;
+            AS_HIP_CHECK(q);
+            activate(int(i));
+            if (out) out->push_back(int(i));
+            ++n;
+        }
+        if (n > 0) rebuild_tiles();
+        return n;
     }
 
     void remove(int member) override {
@@ -1435,6 +1480,7 @@ public:
 
     std::int64_t member_steps(int member) const override {
         const MemberHost& h = host(member);
+        if (h.staging) return 0;
         return std::max<std::int64_t>(0, std::min(step_, h.m.step_end) - h.m.step0);
     }
     std::int64_t member_target(int member) const override { return host(member).m.step_end - host(member).m.step0; }
@@ -1483,6 +1529,16 @@ private:
         return hosts_[member];
     }
     MemberHost& host(int member) { return const_cast<MemberHost&>(static_cast<const BatchImpl*>(this)->host(member)); }
+
+    // An uploaded member joins now (a block boundary) for its pending steps.
+    void activate(int slot) {
+        MemberHost& h = hosts_[slot];
+        h.m.step0 = step_;
+        h.m.step_end = step_ + h.pending_steps;
+        h.staging = false;
+        h.active = true;
+        staged_by_slot_[slot].clear();
+    }
 
     int free_slot() const {
         for (std::size_t i = 0; i < hosts_.size(); ++i)
@@ -1592,7 +1648,9 @@ private:
                 nalloc_.give(h.m.nb, h.m.nn);
                 aalloc_.give(h.m.ab, h.m.na);
                 for (int i = 0; i < 4; ++i) ealloc_[i].give(h.ebase[i], h.ecount[i]);
+                const hipEvent_t ev = h.up_done;  // reused by the slot's next member
                 h = MemberHost();
+                h.up_done = ev;
             }
         }
     }
@@ -1630,25 +1688,28 @@ private:
     }
 
     template <class T>
-    void fill(T* p, T v, std::size_t n) {
+    void fill(T* p, T v, std::size_t n) { fill_on(stream_, p, v, n); }
+    template <class T>
+    void fill_on(hipStream_t s, T* p, T v, std::size_t n) {
         if (n == 0) return;
-        hipLaunchKernelGGL(fill_kernel<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream_, p, v, n);
+        hipLaunchKernelGGL(fill_kernel<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, p, v, n);
         AS_HIP_CHECK(hipGetLastError());
     }
 
-    // The copy reads a converted array that lives in staged_ until the
-    // upload's closing synchronisation.
-    void upload_values(R* dst, const std::vector<double>& v, bool reciprocal = false) {
+    // The copy reads a converted array that lives in `staged` until the
+    // member's activation (its upload has completed by then).
+    void upload_values(R* dst, const std::vector<double>& v, bool reciprocal, hipStream_t s,
+                       std::vector<std::vector<R>>& staged) {
         if (v.empty()) return;
-        staged_.emplace_back(v.size());
-        std::vector<R>& f = staged_.back();
+        staged.emplace_back(v.size());
+        std::vector<R>& f = staged.back();
         for (std::size_t i = 0; i < v.size(); ++i) f[i] = R(reciprocal ? 1.0 / v[i] : v[i]);
-        AS_HIP_CHECK(hipMemcpyAsync(dst, f.data(), f.size() * sizeof(R), hipMemcpyHostToDevice, stream_));
+        AS_HIP_CHECK(hipMemcpyAsync(dst, f.data(), f.size() * sizeof(R), hipMemcpyHostToDevice, s));
     }
 
     // Uploads one member: rebased in-CSRs, out-CSRs built on the device,
     // parameters and initial state (src/astrosim_fused.cu:981-1018).
-    void upload_member(const Network& net, MemberHost& h) {
+    void upload_member(const Network& net, MemberHost& h, hipStream_t s, std::vector<std::vector<R>>& staged) {
         const Member& m = h.m;
         const HostCSR* c[4] = {&net.inc_exc, &net.inc_inh, &net.inc_sic, &net.inc_e2a};
         // source-id base and source-space size of each type (global ids)
@@ -1665,67 +1726,64 @@ private:
             index_t* src = const_cast<index_t*>(D_.in_src[T]) + eb;
             if (ne) {
                 AS_HIP_CHECK(hipMemcpyAsync(src, dev ? net.dev[T].src : c[T]->src.data(), ne * sizeof(index_t),
-                                            dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
+                                            dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
                 if (sbase[T] != 0)
-                    hipLaunchKernelGGL(rebase_kernel, dim3(unsigned((ne + 255) / 256)), dim3(256), 0, stream_, src, ne,
+                    hipLaunchKernelGGL(rebase_kernel, dim3(unsigned((ne + 255) / 256)), dim3(256), 0, s, src, ne,
                                        sbase[T]);
             }
             AS_HIP_CHECK(hipMemcpyAsync(scratch0_, dev ? net.dev[T].row_start : c[T]->row_start.data(),
                                         (std::size_t(nt) + 1) * sizeof(index_t),
-                                        dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
+                                        dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
             uint2* in_rs = const_cast<uint2*>(D_.in_rs[T]) + tbase[T];
-            if (nt) hipLaunchKernelGGL(rows_kernel, dim3((nt + 255) / 256), dim3(256), 0, stream_, scratch0_, nt, eb, in_rs);
+            if (nt) hipLaunchKernelGGL(rows_kernel, dim3((nt + 255) / 256), dim3(256), 0, s, scratch0_, nt, eb, in_rs);
             // out-CSR over the member's source space
             const index_t ns = nsrc[T];
-            AS_HIP_CHECK(hipMemsetAsync(scratch0_, 0, (std::size_t(ns) + 1) * sizeof(index_t), stream_));
+            AS_HIP_CHECK(hipMemsetAsync(scratch0_, 0, (std::size_t(ns) + 1) * sizeof(index_t), s));
             if (ne)
-                hipLaunchKernelGGL(count_sources_kernel, dim3(unsigned((ne + 255) / 256)), dim3(256), 0, stream_,
+                hipLaunchKernelGGL(count_sources_kernel, dim3(unsigned((ne + 255) / 256)), dim3(256), 0, s,
                                    D_.in_src[T] + eb, ne, sbase[T], ns, scratch0_);
             std::size_t tmp = lay_.scan_tmp_bytes;
-            AS_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp_, tmp, scratch0_, scratch1_, int(ns) + 1, stream_));
+            AS_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp_, tmp, scratch0_, scratch1_, int(ns) + 1, s));
             if (nspace[T])
-                hipLaunchKernelGGL(out_rows_kernel, dim3((nspace[T] + 255) / 256), dim3(256), 0, stream_, scratch1_, ns,
+                hipLaunchKernelGGL(out_rows_kernel, dim3((nspace[T] + 255) / 256), dim3(256), 0, s, scratch1_, ns,
                                    nspace[T], sbase[T], eb, const_cast<uint2*>(D_.out_rs[T]));
             if (nt)
-                hipLaunchKernelGGL(fill_out_kernel, dim3((nt + 255) / 256), dim3(256), 0, stream_, D_.in_rs[T],
+                hipLaunchKernelGGL(fill_out_kernel, dim3((nt + 255) / 256), dim3(256), 0, s, D_.in_rs[T],
                                    D_.in_src[T], tbase[T], nt, sbase[T], ns, eb, scratch1_,
                                    const_cast<uint2*>(D_.out_ent[T]));
             AS_HIP_CHECK(hipGetLastError());
-            AS_HIP_CHECK(hipMemsetAsync(D_.bcnt[T] + tbase[T], 0, std::size_t(nt) * sizeof(unsigned), stream_));
+            AS_HIP_CHECK(hipMemsetAsync(D_.bcnt[T] + tbase[T], 0, std::size_t(nt) * sizeof(unsigned), s));
             // the scratch arrays are reused by the next type in stream order
         }
         // parameters: FP32c stores 1/tau_IP3
-        upload_values(S_.Ca_tot + m.ab, net.Ca_tot);
-        upload_values(S_.IP3_0 + m.ab, net.IP3_0);
-        upload_values(S_.tau_IP3 + m.ab, net.tau_IP3, sizeof(R) == 4);
-        upload_values(S_.delta_IP3 + m.ab, net.delta_IP3);
-        upload_values(S_.V_reset + m.nb, net.V_reset);
-        upload_values(S_.b + m.nb, net.b);
+        upload_values(S_.Ca_tot + m.ab, net.Ca_tot, false, s, staged);
+        upload_values(S_.IP3_0 + m.ab, net.IP3_0, false, s, staged);
+        upload_values(S_.tau_IP3 + m.ab, net.tau_IP3, sizeof(R) == 4, s, staged);
+        upload_values(S_.delta_IP3 + m.ab, net.delta_IP3, false, s, staged);
+        upload_values(S_.V_reset + m.nb, net.V_reset, false, s, staged);
+        upload_values(S_.b + m.nb, net.b, false, s, staged);
         const Biology bio;
-        fill(S_.Ca + m.ab, R(bio.Ca_init), m.na);
-        fill(S_.IP3 + m.ab, R(bio.IP3_init), m.na);
-        fill(S_.h + m.ab, R(bio.h_init), m.na);
+        fill_on(s, S_.Ca + m.ab, R(bio.Ca_init), m.na);
+        fill_on(s, S_.IP3 + m.ab, R(bio.IP3_init), m.na);
+        fill_on(s, S_.h + m.ab, R(bio.h_init), m.na);
         if constexpr (sizeof(R) == 4) {  // rounding residue of the initial values
-            fill(S_.Ca_lo + m.ab, R(bio.Ca_init - double(float(bio.Ca_init))), m.na);
-            fill(S_.IP3_lo + m.ab, R(bio.IP3_init - double(float(bio.IP3_init))), m.na);
-            fill(S_.h_lo + m.ab, R(bio.h_init - double(float(bio.h_init))), m.na);
+            fill_on(s, S_.Ca_lo + m.ab, R(bio.Ca_init - double(float(bio.Ca_init))), m.na);
+            fill_on(s, S_.IP3_lo + m.ab, R(bio.IP3_init - double(float(bio.IP3_init))), m.na);
+            fill_on(s, S_.h_lo + m.ab, R(bio.h_init - double(float(bio.h_init))), m.na);
         }
-        fill(S_.V + m.nb, R(bio.e_V_m), m.ne);
-        fill(S_.V + m.nb + m.ne, R(bio.i_V_m), m.nn - m.ne);
-        for (R* p : {S_.w, S_.g_ex, S_.dg_ex, S_.g_in, S_.dg_in, S_.I_sic}) fill(p + m.nb, R(0), m.nn);
-        fill(S_.stp_x + m.nb, R(1), m.nn);
-        fill(S_.stp_u + m.nb, R(P_.stp_U), m.nn);
-        fill(S_.stp_t + m.nb, R(-1e12), m.nn);
-        fill(D_.refr + m.nb, std::int32_t(0), m.nn);
-        fill(D_.ever + m.ab, std::uint8_t(0), m.na);
-        for (int s = 0; s < geo_.H; ++s) {
-            fill(D_.spk_w + std::size_t(s) * D_.ncap + m.nb, std::uint32_t(0), m.nn);
-            fill(D_.sic_w + std::size_t(s) * D_.acap + m.ab, std::uint32_t(0), m.na);
+        fill_on(s, S_.V + m.nb, R(bio.e_V_m), m.ne);
+        fill_on(s, S_.V + m.nb + m.ne, R(bio.i_V_m), m.nn - m.ne);
+        for (R* p : {S_.w, S_.g_ex, S_.dg_ex, S_.g_in, S_.dg_in, S_.I_sic}) fill_on(s, p + m.nb, R(0), m.nn);
+        fill_on(s, S_.stp_x + m.nb, R(1), m.nn);
+        fill_on(s, S_.stp_u + m.nb, R(P_.stp_U), m.nn);
+        fill_on(s, S_.stp_t + m.nb, R(-1e12), m.nn);
+        fill_on(s, D_.refr + m.nb, std::int32_t(0), m.nn);
+        fill_on(s, D_.ever + m.ab, std::uint8_t(0), m.na);
+        for (int hs = 0; hs < geo_.H; ++hs) {
+            fill_on(s, D_.spk_w + std::size_t(hs) * D_.ncap + m.nb, std::uint32_t(0), m.nn);
+            fill_on(s, D_.sic_w + std::size_t(hs) * D_.acap + m.ab, std::uint32_t(0), m.na);
         }
-        fill(D_.counters + std::size_t(m.slot) * kCountersPerMember, 0ull, kCountersPerMember);
-        // the network's host arrays and the staged parameters are read by now
-        AS_HIP_CHECK(hipStreamSynchronize(stream_));
-        staged_.clear();
+        fill_on(s, D_.counters + std::size_t(m.slot) * kCountersPerMember, 0ull, kCountersPerMember);
     }
 
     RunParams P_;
@@ -1751,7 +1809,9 @@ private:
     Member* pin_members_ = nullptr;  // pinned staging of the member and span tables
     TileSpan* pin_spans_ = nullptr;
     hipEvent_t tables_copied_ = nullptr;
-    std::vector<std::vector<R>> staged_;  // converted parameters until their copies complete
+    hipStream_t up_ = nullptr;          // member uploads (add_async)
+    hipEvent_t enqueued_ = nullptr;     // the batch stream's work at an add_async
+    std::vector<std::vector<std::vector<R>>> staged_by_slot_;  // converted parameters until activation
     std::int64_t step_ = 0;
     std::map<std::int64_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
     bool timing_ = false;

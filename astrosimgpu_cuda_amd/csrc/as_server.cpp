@@ -253,14 +253,35 @@ private:
                     cv_done_.notify_all();
                     continue;
                 }
-                q->member = B.add(*q->net, q->in.n_steps);
+                // uploaded beside the running steps; it joins once the upload is done
+                q->member = B.add_async(*q->net, q->in.n_steps);
                 std::lock_guard<std::mutex> g(mu_);
                 ready_[b].pop_front();
+                staging_[b].push_back(q);
+            }
+        }
+    }
+
+    // Requests whose upload has completed join their batch (at the current
+    // block boundary) and start running.
+    void activate() {
+        for (int b = 0; b < 2; ++b) {
+            if (staging_[b].empty()) continue;
+            std::vector<int> slots;
+            if (batch_[b]->activate_ready(&slots) == 0) continue;
+            std::vector<std::shared_ptr<Req>> still;
+            std::lock_guard<std::mutex> g(mu_);
+            for (auto& q : staging_[b]) {
+                if (std::find(slots.begin(), slots.end(), q->member) == slots.end()) {
+                    still.push_back(q);
+                    continue;
+                }
                 q->net.reset();  // resident in HBM now
                 q->out.status = AS_REQ_RUNNING;
                 q->out.t_admit = now();
                 running_[b].push_back(q);
             }
+            staging_[b].swap(still);
         }
     }
 
@@ -304,12 +325,13 @@ private:
                     std::unique_lock<std::mutex> g(mu_);
                     cv_ready_.wait_for(g, std::chrono::milliseconds(2), [&] {
                         return stop_ || !ready_[0].empty() || !ready_[1].empty() || !running_[0].empty() ||
-                               !running_[1].empty();
+                               !running_[1].empty() || !staging_[0].empty() || !staging_[1].empty();
                     });
                     if (stop_) return;
                 }
                 const auto a0 = Clock::now();
                 admit();
+                activate();
                 admit_s_ += std::chrono::duration<double>(Clock::now() - a0).count();
                 std::uint64_t cells = 0;
                 int active = 0;
@@ -322,7 +344,12 @@ private:
                     active += int(running_[b].size());
                     batch_[b]->enqueue_steps(std::int64_t(blocks) * batch_[b]->block_len());  // async
                 }
-                if (!any) continue;
+                if (!any) {
+                    // only uploads in flight: poll them without spinning
+                    if (!staging_[0].empty() || !staging_[1].empty())
+                        std::this_thread::sleep_for(std::chrono::microseconds(200));
+                    continue;
+                }
                 for (int b = 0; b < 2; ++b) AS_HIP_CHECK(hipStreamSynchronize(stream_[b]));
                 gpu_busy_s_ += std::chrono::duration<double>(Clock::now() - g0).count();
                 cell_steps_ += double(cells) * blocks * batch_[0]->block_len();
@@ -356,6 +383,7 @@ private:
     std::condition_variable cv_build_, cv_ready_, cv_done_;
     std::vector<std::shared_ptr<Req>> reqs_;
     std::deque<std::shared_ptr<Req>> build_q_, ready_[2];
+    std::vector<std::shared_ptr<Req>> staging_[2];  // admitted, upload in flight
     std::vector<std::shared_ptr<Req>> running_[2];
     std::size_t finished_ = 0;
     bool stop_ = false;

@@ -39,10 +39,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 # Hardware queues per process, read when HIP initialises (nothing has yet).
-# The serving leg runs the two batch streams beside up to 12 network-build
-# streams; with HIP's default of 4 queues those streams share queues, and
-# every batch synchronisation then also waits for unrelated build kernels.
-os.environ["GPU_MAX_HW_QUEUES"] = "16"
+# The serving leg runs the two batch streams and their two upload streams
+# beside up to 12 network-build streams; with HIP's default of 4 queues those
+# streams share queues, and every batch synchronisation then also waits for
+# unrelated build kernels.
+os.environ["GPU_MAX_HW_QUEUES"] = "24"
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
