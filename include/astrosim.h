@@ -213,8 +213,9 @@ void as_batch_destroy(as_batch* b);
  * fits, advanced a few blocks at a time with both batches' streams
  * overlapping on the device, and retired when they reach their step count.
  * Results equal a solo run. The process needs more hardware queues than
- * build_threads + 4 (GPU_MAX_HW_QUEUES, read when HIP initialises): streams
- * that share a queue wait for each other's work. */
+ * build_threads + 4 (GPU_MAX_HW_QUEUES, read when HIP initialises; each
+ * batch has a block stream and an upload stream): streams that share a
+ * queue wait for each other's work. */
 typedef struct as_server as_server;
 typedef struct as_server_config {
     double dt;                /* ms */
