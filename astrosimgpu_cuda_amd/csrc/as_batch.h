@@ -63,6 +63,13 @@ public:
     virtual void read_ever_active(int member, std::uint8_t* out) = 0;
     virtual void read_raster(int member, std::int64_t first_local, int n, std::uint8_t* out) = 0;
     virtual std::int64_t member_steps(int member) const = 0;  // local steps enqueued
+    // Local steps the member has run once the device reaches global step g.
+    virtual std::int64_t member_steps_at(int member, std::int64_t g) const = 0;
+    // A finished member's spike total and final calcium (hi + lo), read on
+    // the batch's read stream: later blocks do not touch a member past its
+    // last step, so the read runs beside them. The caller has waited for the
+    // work that ran the member's last step.
+    virtual void read_final(int member, std::uint64_t* spikes, double* ca) = 0;
     virtual std::int64_t member_target(int member) const = 0;
     virtual bool member_active(int member) const = 0;
     virtual int active_members() const = 0;

@@ -1203,6 +1203,7 @@ public:
         D_.tiles = (const Tile*)dptr(lay_.tiles);
         spans_ = (TileSpan*)dptr(lay_.spans);
         AS_HIP_CHECK(hipStreamCreateWithFlags(&up_, hipStreamNonBlocking));
+        AS_HIP_CHECK(hipStreamCreateWithFlags(&rd_, hipStreamNonBlocking));
         AS_HIP_CHECK(hipEventCreateWithFlags(&enqueued_, hipEventDisableTiming));
         staged_by_slot_.resize(cap.members);
         AS_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pin_members_), std::size_t(cap.members) * sizeof(Member)));
@@ -1240,6 +1241,7 @@ public:
             (void)hipStreamSynchronize(up_);
             (void)hipStreamDestroy(up_);
         }
+        if (rd_) (void)hipStreamDestroy(rd_);
         for (auto& h : hosts_)
             if (h.up_done) (void)hipEventDestroy(h.up_done);
         if (enqueued_) (void)hipEventDestroy(enqueued_);
@@ -1483,6 +1485,31 @@ public:
         if (h.staging) return 0;
         return std::max<std::int64_t>(0, std::min(step_, h.m.step_end) - h.m.step0);
     }
+    std::int64_t member_steps_at(int member, std::int64_t g) const override {
+        const MemberHost& h = host(member);
+        if (h.staging) return 0;
+        return std::max<std::int64_t>(0, std::min(g, h.m.step_end) - h.m.step0);
+    }
+
+    void read_final(int member, std::uint64_t* spikes, double* ca) override {
+        const MemberHost& h = host(member);
+        unsigned long long c[kCountersPerMember];
+        AS_HIP_CHECK(hipMemcpyAsync(c, D_.counters + std::size_t(h.m.slot) * kCountersPerMember, sizeof c,
+                                    hipMemcpyDeviceToHost, rd_));
+        const std::size_t n = h.m.na;
+        std::vector<R> hv(n), lv;
+        if (n) AS_HIP_CHECK(hipMemcpyAsync(hv.data(), S_.Ca + h.m.ab, n * sizeof(R), hipMemcpyDeviceToHost, rd_));
+        if (sizeof(R) == 4 && n) {
+            lv.resize(n);
+            AS_HIP_CHECK(hipMemcpyAsync(lv.data(), S_.Ca_lo + h.m.ab, n * sizeof(R), hipMemcpyDeviceToHost, rd_));
+        }
+        AS_HIP_CHECK(hipStreamSynchronize(rd_));
+        std::uint64_t t = 0;
+        for (auto v : c) t += v;
+        *spikes = t;
+        for (std::size_t i = 0; i < n; ++i) ca[i] = double(hv[i]) + (lv.empty() ? 0.0 : double(lv[i]));
+    }
+
     std::int64_t member_target(int member) const override { return host(member).m.step_end - host(member).m.step0; }
     bool member_active(int member) const override { return host(member).active; }
     int active_members() const override {
@@ -1810,6 +1837,7 @@ private:
     TileSpan* pin_spans_ = nullptr;
     hipEvent_t tables_copied_ = nullptr;
     hipStream_t up_ = nullptr;          // member uploads (add_async)
+    hipStream_t rd_ = nullptr;          // reads of finished members (read_final)
     hipEvent_t enqueued_ = nullptr;     // the batch stream's work at an add_async
     std::vector<std::vector<std::vector<R>>> staged_by_slot_;  // converted parameters until activation
     std::int64_t step_ = 0;

@@ -89,6 +89,10 @@ public:
         cap_[1] = capacity_for_budget(std::uint64_t(budget * share64), rp_[1], members);
         for (int i = 0; i < 2; ++i) {
             AS_HIP_CHECK(hipStreamCreateWithFlags(&stream_[i], hipStreamNonBlocking));
+            for (int k = 0; k < 2; ++k) {
+                AS_HIP_CHECK(hipEventCreate(&round0_[k][i]));
+                AS_HIP_CHECK(hipEventCreate(&round1_[k][i]));
+            }
             batch_[i] = make_batch(rp_[i], cap_[i], stream_[i]);
         }
         setup_where_ = c.setup_where;
@@ -108,6 +112,10 @@ public:
         if (driver_.joinable()) driver_.join();
         for (int i = 0; i < 2; ++i) {
             batch_[i].reset();
+            for (int k = 0; k < 2; ++k) {
+                if (round0_[k][i]) (void)hipEventDestroy(round0_[k][i]);
+                if (round1_[k][i]) (void)hipEventDestroy(round1_[k][i]);
+            }
             if (stream_[i]) (void)hipStreamDestroy(stream_[i]);
         }
     }
@@ -285,18 +293,20 @@ private:
         }
     }
 
+    // Retires the requests that have run all their steps by each batch's
+    // last completed round.
     void retire() {
         for (int b = 0; b < 2; ++b) {
             Batch& B = *batch_[b];
             std::vector<std::shared_ptr<Req>> still;
             for (auto& q : running_[b]) {
-                if (B.member_steps(q->member) < q->in.n_steps) {
+                if (B.member_steps_at(q->member, done_step_[b]) < q->in.n_steps) {
                     still.push_back(q);
                     continue;
                 }
-                const std::uint64_t spikes = B.spike_count(q->member);
+                std::uint64_t spikes = 0;
                 std::vector<double> ca(q->out.n_astro);
-                if (!ca.empty()) B.read_state(q->member, 0, ca.data());
+                B.read_final(q->member, &spikes, ca.data());
                 double mean = 0.0;
                 std::uint32_t active = 0;
                 for (double c : ca) {
@@ -317,47 +327,82 @@ private:
         }
     }
 
+    // Rounds are pipelined two deep: round k+1 is enqueued before the driver
+    // waits for round k, so the device runs k+1 while the host admits,
+    // retires (the results of members that finished by the end of k are read
+    // on each batch's read stream) and rebuilds the tile tables for k+2.
     void drive_loop() {
         const int blocks = std::max(1, cfg_.blocks_per_round);
+        int cur = 0, prev = 0;
+        bool inflight = false;  // round `prev` is enqueued and not yet completed
         try {
             for (;;) {
                 {
                     std::unique_lock<std::mutex> g(mu_);
                     cv_ready_.wait_for(g, std::chrono::milliseconds(2), [&] {
-                        return stop_ || !ready_[0].empty() || !ready_[1].empty() || !running_[0].empty() ||
-                               !running_[1].empty() || !staging_[0].empty() || !staging_[1].empty();
+                        return stop_ || inflight || !ready_[0].empty() || !ready_[1].empty() ||
+                               !running_[0].empty() || !running_[1].empty() || !staging_[0].empty() ||
+                               !staging_[1].empty();
                     });
                     if (stop_) return;
                 }
-                const auto a0 = Clock::now();
-                admit();
+                auto a0 = Clock::now();
                 activate();
                 admit_s_ += std::chrono::duration<double>(Clock::now() - a0).count();
-                std::uint64_t cells = 0;
-                int active = 0;
-                const auto g0 = Clock::now();
-                bool any = false;
+                Round& R = rounds_[cur];
+                R = Round();
                 for (int b = 0; b < 2; ++b) {
                     if (running_[b].empty()) continue;
-                    any = true;
-                    for (auto& q : running_[b]) cells += q->out.n_neurons + q->out.n_astro;
-                    active += int(running_[b].size());
+                    R.ran[b] = true;
+                    for (auto& q : running_[b]) R.cells += q->out.n_neurons + q->out.n_astro;
+                    R.active += int(running_[b].size());
+                    AS_HIP_CHECK(hipEventRecord(round0_[cur][b], stream_[b]));
                     batch_[b]->enqueue_steps(std::int64_t(blocks) * batch_[b]->block_len());  // async
+                    AS_HIP_CHECK(hipEventRecord(round1_[cur][b], stream_[b]));
+                    R.end_step[b] = batch_[b]->global_step();
                 }
-                if (!any) {
-                    // only uploads in flight: poll them without spinning
-                    if (!staging_[0].empty() || !staging_[1].empty())
+                const bool now_ran = R.ran[0] || R.ran[1];
+                // Admission's host work runs while the rounds execute; its
+                // uploads are ordered after them (Batch::add_async) and the
+                // members join at a later round's start.
+                a0 = Clock::now();
+                admit();
+                admit_s_ += std::chrono::duration<double>(Clock::now() - a0).count();
+                if (!inflight) {
+                    if (now_ran) {
+                        inflight = true;  // keep it queued and enqueue the next
+                        prev = cur;
+                        cur ^= 1;
+                    } else if (!staging_[0].empty() || !staging_[1].empty()) {
+                        // only uploads in flight: poll them without spinning
                         std::this_thread::sleep_for(std::chrono::microseconds(200));
+                    }
                     continue;
                 }
-                for (int b = 0; b < 2; ++b) AS_HIP_CHECK(hipStreamSynchronize(stream_[b]));
-                gpu_busy_s_ += std::chrono::duration<double>(Clock::now() - g0).count();
-                cell_steps_ += double(cells) * blocks * batch_[0]->block_len();
-                peak_active_ = std::max<std::uint64_t>(peak_active_, active);
-                peak_cells_ = std::max<std::uint64_t>(peak_cells_, cells);
+                // complete round `prev`; the one just enqueued keeps the device busy
+                const Round& P = rounds_[prev];
+                float round_ms = 0.0f;
+                for (int b = 0; b < 2; ++b) {
+                    if (!P.ran[b]) continue;
+                    AS_HIP_CHECK(hipEventSynchronize(round1_[prev][b]));
+                    float ms = 0.0f;
+                    AS_HIP_CHECK(hipEventElapsedTime(&ms, round0_[prev][b], round1_[prev][b]));
+                    round_ms = std::max(round_ms, ms);
+                    done_step_[b] = P.end_step[b];
+                }
+                gpu_busy_s_ += 1e-3 * double(round_ms);  // the longer of the two batches' spans
+                cell_steps_ += double(P.cells) * blocks * batch_[0]->block_len();
+                peak_active_ = std::max<std::uint64_t>(peak_active_, P.active);
+                peak_cells_ = std::max<std::uint64_t>(peak_cells_, P.cells);
                 const auto r0 = Clock::now();
                 retire();
                 retire_s_ += std::chrono::duration<double>(Clock::now() - r0).count();
+                if (now_ran) {
+                    prev = cur;
+                    cur ^= 1;
+                } else {
+                    inflight = false;
+                }
             }
         } catch (const std::exception& e) {
             std::lock_guard<std::mutex> g(mu_);
@@ -378,6 +423,16 @@ private:
     RunParams rp_[2];
     BatchCapacity cap_[2];
     hipStream_t stream_[2] = {nullptr, nullptr};
+    // A round: up to `blocks_per_round` blocks of each batch with members.
+    struct Round {
+        bool ran[2] = {false, false};
+        std::int64_t end_step[2] = {0, 0};  // each batch's global step at its end
+        std::uint64_t cells = 0;
+        int active = 0;
+    };
+    Round rounds_[2];
+    hipEvent_t round0_[2][2] = {}, round1_[2][2] = {};  // [round slot][batch]: the round's span
+    std::int64_t done_step_[2] = {0, 0};                // each batch's step at its last completed round
     std::unique_ptr<Batch> batch_[2];
     std::mutex mu_;
     std::condition_variable cv_build_, cv_ready_, cv_done_;

@@ -250,7 +250,8 @@ typedef struct as_server_stats {
     uint64_t device_bytes;        /* HBM held by the resident batches */
     uint64_t peak_active;         /* most requests resident at once */
     uint64_t peak_resident_cells, capacity_cells;
-    double gpu_busy_s;            /* wall time of the device rounds */
+    double gpu_busy_s;            /* device time of the rounds (per round, the
+                                     longer batch's span) */
     double cell_steps;            /* cells x steps advanced */
     double admit_s, retire_s;     /* driver time admitting and retiring requests between rounds */
 } as_server_stats;
