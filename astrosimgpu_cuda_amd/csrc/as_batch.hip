@@ -1205,7 +1205,7 @@ public:
         AS_HIP_CHECK(hipStreamCreateWithFlags(&up_, hipStreamNonBlocking));
         AS_HIP_CHECK(hipStreamCreateWithFlags(&rd_, hipStreamNonBlocking));
         AS_HIP_CHECK(hipEventCreateWithFlags(&enqueued_, hipEventDisableTiming));
-        staged_by_slot_.resize(cap.members);
+        pin_slot_.resize(cap.members);
         AS_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pin_members_), std::size_t(cap.members) * sizeof(Member)));
         AS_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pin_spans_), std::size_t(cap.members) * sizeof(TileSpan)));
         AS_HIP_CHECK(hipEventCreateWithFlags(&tables_copied_, hipEventDisableTiming));
@@ -1244,6 +1244,11 @@ public:
         if (rd_) (void)hipStreamDestroy(rd_);
         for (auto& h : hosts_)
             if (h.up_done) (void)hipEventDestroy(h.up_done);
+        if (rd_) (void)hipStreamSynchronize(rd_);
+        for (Pinned& b : pin_slot_)
+            if (b.p) (void)hipHostFree(b.p);
+        for (Pinned& b : pin_free_) (void)hipHostFree(b.p);
+        if (pin_read_.p) (void)hipHostFree(pin_read_.p);
         if (enqueued_) (void)hipEventDestroy(enqueued_);
         if (pin_members_) (void)hipHostFree(pin_members_);
         if (pin_spans_) (void)hipHostFree(pin_spans_);
@@ -1297,7 +1302,11 @@ public:
         // are still being delivered by enqueued work: upload after it
         AS_HIP_CHECK(hipEventRecord(enqueued_, stream_));
         AS_HIP_CHECK(hipStreamWaitEvent(up_, enqueued_, 0));
-        upload_member(net, h, up_, staged_by_slot_[slot]);
+        const std::size_t pin_need = pinned_bytes(net.Ca_tot.size()) + pinned_bytes(net.IP3_0.size()) +
+                                     pinned_bytes(net.tau_IP3.size()) + pinned_bytes(net.delta_IP3.size()) +
+                                     pinned_bytes(net.V_reset.size()) + pinned_bytes(net.b.size());
+        pin_slot_[slot] = take_pinned(pin_need);
+        upload_member(net, h, up_, pin_slot_[slot]);
         if (h.up_done == nullptr) AS_HIP_CHECK(hipEventCreateWithFlags(&h.up_done, hipEventDisableTiming));
         AS_HIP_CHECK(hipEventRecord(h.up_done, up_));
         return slot;
@@ -1331,6 +1340,7 @@ public:
     }
 
     void enqueue_steps(std::int64_t k) override {
+        flush_tables();
         while (k > 0) {
             if (step_ % geo_.L == 0) release_due();
             const std::int64_t bidx = step_ / geo_.L;
@@ -1356,6 +1366,7 @@ public:
         if (blocks <= 0) return;
         if (step_ % geo_.L != 0) throw Error(AS_ERR_INVALID, "graph replays start at a block boundary");
         release_due();
+        flush_tables();
         hipGraphExec_t exec = graph_for(blocks);
         const std::int64_t b0 = step_ / geo_.L;
         AS_HIP_CHECK(hipMemcpyAsync(clock_, &b0, sizeof b0, hipMemcpyHostToDevice, stream_));
@@ -1493,21 +1504,31 @@ public:
 
     void read_final(int member, std::uint64_t* spikes, double* ca) override {
         const MemberHost& h = host(member);
-        unsigned long long c[kCountersPerMember];
-        AS_HIP_CHECK(hipMemcpyAsync(c, D_.counters + std::size_t(h.m.slot) * kCountersPerMember, sizeof c,
-                                    hipMemcpyDeviceToHost, rd_));
         const std::size_t n = h.m.na;
-        std::vector<R> hv(n), lv;
-        if (n) AS_HIP_CHECK(hipMemcpyAsync(hv.data(), S_.Ca + h.m.ab, n * sizeof(R), hipMemcpyDeviceToHost, rd_));
-        if (sizeof(R) == 4 && n) {
-            lv.resize(n);
-            AS_HIP_CHECK(hipMemcpyAsync(lv.data(), S_.Ca_lo + h.m.ab, n * sizeof(R), hipMemcpyDeviceToHost, rd_));
+        constexpr std::size_t kCountBytes = kCountersPerMember * sizeof(unsigned long long);
+        const std::size_t need = kCountBytes + 2 * pinned_bytes(n);
+        if (pin_read_.bytes < need) {
+            if (pin_read_.p) AS_HIP_CHECK(hipHostFree(pin_read_.p));
+            pin_read_ = Pinned();
+            const std::size_t bytes = std::max<std::size_t>(need + need / 2, std::size_t(1) << 20);
+            AS_HIP_CHECK(hipHostMalloc(&pin_read_.p, bytes, hipHostMallocDefault));
+            pin_read_.bytes = bytes;
         }
+        char* base = static_cast<char*>(pin_read_.p);
+        auto* c = reinterpret_cast<unsigned long long*>(base);
+        R* hv = reinterpret_cast<R*>(base + kCountBytes);
+        R* lv = reinterpret_cast<R*>(base + kCountBytes + pinned_bytes(n));
+        const bool has_lo = sizeof(R) == 4;
+        AS_HIP_CHECK(hipMemcpyAsync(c, D_.counters + std::size_t(h.m.slot) * kCountersPerMember, kCountBytes,
+                                    hipMemcpyDeviceToHost, rd_));
+        if (n) AS_HIP_CHECK(hipMemcpyAsync(hv, S_.Ca + h.m.ab, n * sizeof(R), hipMemcpyDeviceToHost, rd_));
+        if (has_lo && n)
+            AS_HIP_CHECK(hipMemcpyAsync(lv, S_.Ca_lo + h.m.ab, n * sizeof(R), hipMemcpyDeviceToHost, rd_));
         AS_HIP_CHECK(hipStreamSynchronize(rd_));
         std::uint64_t t = 0;
-        for (auto v : c) t += v;
+        for (int i = 0; i < kCountersPerMember; ++i) t += c[i];
         *spikes = t;
-        for (std::size_t i = 0; i < n; ++i) ca[i] = double(hv[i]) + (lv.empty() ? 0.0 : double(lv[i]));
+        for (std::size_t i = 0; i < n; ++i) ca[i] = double(hv[i]) + (has_lo ? double(lv[i]) : 0.0);
     }
 
     std::int64_t member_target(int member) const override { return host(member).m.step_end - host(member).m.step0; }
@@ -1564,7 +1585,8 @@ private:
         h.m.step_end = step_ + h.pending_steps;
         h.staging = false;
         h.active = true;
-        staged_by_slot_[slot].clear();
+        give_pinned(pin_slot_[slot]);  // its upload has completed
+        pin_slot_[slot] = Pinned();
     }
 
     int free_slot() const {
@@ -1641,7 +1663,21 @@ private:
     // staging (after the previous copies out of it have completed) and a
     // kernel writes the tiles, so a change costs a few kilobytes of copy
     // and no synchronisation however many tiles the batch holds.
+    // Membership changed: the tables are rewritten before the next launch
+    // (flush_tables), once however many members joined or left meanwhile.
+    // Writing them waits for the previous copy out of the pinned staging;
+    // deferred to the next launch, that copy precedes the blocks already
+    // enqueued instead of following them.
     void rebuild_tiles() {
+        tables_dirty_ = true;
+        drop_graphs();
+    }
+    void flush_tables() {
+        if (!tables_dirty_) return;
+        write_tables();
+        tables_dirty_ = false;
+    }
+    void write_tables() {
         AS_HIP_CHECK(hipEventSynchronize(tables_copied_));
         std::uint32_t n_active = 0;
         std::uint64_t total = 0;
@@ -1665,7 +1701,6 @@ private:
             AS_HIP_CHECK(hipGetLastError());
         }
         AS_HIP_CHECK(hipEventRecord(tables_copied_, stream_));
-        drop_graphs();
     }
 
     void release_due() {
@@ -1723,20 +1758,49 @@ private:
         AS_HIP_CHECK(hipGetLastError());
     }
 
-    // The copy reads a converted array that lives in `staged` until the
-    // member's activation (its upload has completed by then).
-    void upload_values(R* dst, const std::vector<double>& v, bool reciprocal, hipStream_t s,
-                       std::vector<std::vector<R>>& staged) {
+    // Pinned host memory (hipHostMalloc): copies from and to it run on their
+    // own stream beside other streams' kernels, where a copy through
+    // pageable memory waited for them here.
+    struct Pinned {
+        void* p = nullptr;
+        std::size_t bytes = 0, used = 0;
+    };
+    static std::size_t pinned_bytes(std::size_t n) { return (n * sizeof(R) + 255) & ~std::size_t(255); }
+    Pinned take_pinned(std::size_t need) {
+        std::size_t best = pin_free_.size();
+        for (std::size_t i = 0; i < pin_free_.size(); ++i)
+            if (pin_free_[i].bytes >= need && (best == pin_free_.size() || pin_free_[i].bytes < pin_free_[best].bytes))
+                best = i;
+        Pinned b;
+        if (best < pin_free_.size()) {
+            b = pin_free_[best];
+            pin_free_.erase(pin_free_.begin() + std::ptrdiff_t(best));
+        } else {
+            b.bytes = std::max<std::size_t>(need, std::size_t(1) << 20);
+            AS_HIP_CHECK(hipHostMalloc(&b.p, b.bytes, hipHostMallocDefault));
+        }
+        b.used = 0;
+        return b;
+    }
+    void give_pinned(const Pinned& b) {
+        if (b.p) pin_free_.push_back(b);
+    }
+
+    // The copy reads a converted array in the member's pinned staging, which
+    // is returned to the pool at its activation (its upload has completed).
+    void upload_values(R* dst, const std::vector<double>& v, bool reciprocal, hipStream_t s, Pinned& pb) {
         if (v.empty()) return;
-        staged.emplace_back(v.size());
-        std::vector<R>& f = staged.back();
+        const std::size_t bytes = pinned_bytes(v.size());
+        if (pb.used + bytes > pb.bytes) throw Error(AS_ERR_INVALID, "pinned staging smaller than the network's parameters");
+        R* f = reinterpret_cast<R*>(static_cast<char*>(pb.p) + pb.used);
+        pb.used += bytes;
         for (std::size_t i = 0; i < v.size(); ++i) f[i] = R(reciprocal ? 1.0 / v[i] : v[i]);
-        AS_HIP_CHECK(hipMemcpyAsync(dst, f.data(), f.size() * sizeof(R), hipMemcpyHostToDevice, s));
+        AS_HIP_CHECK(hipMemcpyAsync(dst, f, v.size() * sizeof(R), hipMemcpyHostToDevice, s));
     }
 
     // Uploads one member: rebased in-CSRs, out-CSRs built on the device,
     // parameters and initial state (src/astrosim_fused.cu:981-1018).
-    void upload_member(const Network& net, MemberHost& h, hipStream_t s, std::vector<std::vector<R>>& staged) {
+    void upload_member(const Network& net, MemberHost& h, hipStream_t s, Pinned& staged) {
         const Member& m = h.m;
         const HostCSR* c[4] = {&net.inc_exc, &net.inc_inh, &net.inc_sic, &net.inc_e2a};
         // source-id base and source-space size of each type (global ids)
@@ -1823,6 +1887,7 @@ private:
     std::vector<MemberHost> hosts_;
     int threads_;
     unsigned n_tiles_ = 0;
+    bool tables_dirty_ = false;  // membership changed since the tables were written
     BatchLayout lay_{};
     void* base_ = nullptr;
     BatchDev D_{};
@@ -1839,7 +1904,9 @@ private:
     hipStream_t up_ = nullptr;          // member uploads (add_async)
     hipStream_t rd_ = nullptr;          // reads of finished members (read_final)
     hipEvent_t enqueued_ = nullptr;     // the batch stream's work at an add_async
-    std::vector<std::vector<std::vector<R>>> staged_by_slot_;  // converted parameters until activation
+    std::vector<Pinned> pin_slot_;  // per slot: the converted parameters of its upload, until activation
+    std::vector<Pinned> pin_free_;  // pinned staging buffers not in use
+    Pinned pin_read_;               // read_final's destination
     std::int64_t step_ = 0;
     std::map<std::int64_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
     bool timing_ = false;

@@ -482,7 +482,7 @@ void as_server_config_default(as_server_config* c) {
     c->hbm_fraction = 0.5;
     c->fp64_share = 0.3;
     c->build_threads = 8;
-    c->blocks_per_round = 4;
+    c->blocks_per_round = 16;
     c->max_members = 512;
     c->setup_where = AS_BUILD_GPU;
 }

@@ -39,8 +39,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 # Hardware queues per process, read when HIP initialises (nothing has yet).
-# The serving leg runs the two batch streams and their two upload streams
-# beside up to 12 network-build streams; with HIP's default of 4 queues those
+# The serving leg runs the two batch streams with their upload and read
+# streams beside up to 12 network-build streams; with HIP's default of 4 queues those
 # streams share queues, and every batch synchronisation then also waits for
 # unrelated build kernels.
 os.environ["GPU_MAX_HW_QUEUES"] = "24"
@@ -199,7 +199,7 @@ def ensemble_leg(pre_ms: float = 100.0, sim_ms: float = 500.0, n_sample: int = 8
     }
 
 
-def serving_leg(duration_s: float, blocks_per_round: int = 4) -> dict:
+def serving_leg(duration_s: float, blocks_per_round: int = 16) -> dict:
     """Serving load (40 req/s Poisson arrivals, N_total log-uniform in
     [1e3, 1e6], sim time uniform in [200, 2000] ms, 20% FP64, own seeds)
     replayed in real time for `duration_s` seconds of arrivals against the
@@ -281,7 +281,7 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=10.0,
                     help="arrival window of the serving load (0 skips it; the full load is 120 s)")
-    ap.add_argument("--serving-blocks", type=int, default=4,
+    ap.add_argument("--serving-blocks", type=int, default=16,
                     help="1 ms blocks the server advances between admission checks")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the known-size copy that HBM counter readings are scaled by")

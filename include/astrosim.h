@@ -210,12 +210,12 @@ void as_batch_destroy(as_batch* b);
  * default with the GPU generator from N_total = 100000 on, see
  * as_network_build_on), admitted into resident FP32c / FP64 batches
  * (capacities sized from a fraction of free HBM) as soon as their footprint
- * fits, advanced a few blocks at a time with both batches' streams
+ * fits, advanced in rounds of blocks with both batches' streams
  * overlapping on the device, and retired when they reach their step count.
  * Results equal a solo run. The process needs more hardware queues than
- * build_threads + 4 (GPU_MAX_HW_QUEUES, read when HIP initialises; each
- * batch has a block stream and an upload stream): streams that share a
- * queue wait for each other's work. */
+ * build_threads + 6 (GPU_MAX_HW_QUEUES, read when HIP initialises; each
+ * batch has a block stream, an upload stream and a read stream): streams
+ * that share a queue wait for each other's work. */
 typedef struct as_server as_server;
 typedef struct as_server_config {
     double dt;                /* ms */
@@ -223,7 +223,9 @@ typedef struct as_server_config {
     double hbm_fraction;      /* share of free HBM given to the two batches */
     double fp64_share;        /* share of that budget given to the FP64 batch */
     int32_t build_threads;    /* host network builders */
-    int32_t blocks_per_round; /* blocks advanced between admission checks */
+    int32_t blocks_per_round; /* blocks per round (default 16); rounds are
+                                 pipelined two deep, members join and leave
+                                 between rounds */
     int32_t max_members;      /* member slots per batch */
     int32_t setup_where;      /* as_build_where of the request networks (default AS_BUILD_GPU) */
 } as_server_config;
