@@ -20,14 +20,21 @@ def solo(n_total, seed, steps, precision):
     return out
 
 
-def test_server_results_equal_solo_runs():
+# (blocks per round, largest N_total, arena share of free HBM, load seed):
+# short rounds with host-built networks; the default 16-block rounds with
+# four networks of 125k-148k cells generated on the GPU (one FP64)
+CASES = [(2, 30000, 0.002, 9), (16, 150000, 0.02, 13)]
+
+
+@pytest.mark.parametrize("blocks,n_hi,hbm,seed", CASES)
+def test_server_results_equal_solo_runs(blocks, n_hi, hbm, seed):
     # a near-simultaneous burst, so requests arrive while others run
-    load = serving.poisson_load(rate_hz=400.0, duration_s=0.05, n_lo=1000, n_hi=30000, sim_lo_ms=100.0,
-                                sim_hi_ms=300.0, fp64_fraction=0.3, seed=9)
-    print(f"[server] {len(load)} requests", flush=True)
+    load = serving.poisson_load(rate_hz=400.0, duration_s=0.05, n_lo=1000, n_hi=n_hi, sim_lo_ms=100.0,
+                                sim_hi_ms=300.0, fp64_fraction=0.3, seed=seed)
+    print(f"[server] {len(load)} requests, {blocks} blocks per round", flush=True)
     assert len(load) >= 8
     # a small arena so that later requests must wait for earlier ones to leave
-    with A.Server(hbm_fraction=0.002, fp64_share=0.3, build_threads=4, blocks_per_round=2) as srv:
+    with A.Server(hbm_fraction=hbm, fp64_share=0.3, build_threads=4, blocks_per_round=blocks) as srv:
         out = serving.replay(srv, load, timeout_s=240.0)
     st = out["stats"]
     print(f"[server] stats {st}", flush=True)
