@@ -4,10 +4,11 @@
 // host threads builds each request's network (connectivity and parameters);
 // a driver thread admits built networks into one of two resident batches
 // (FP32c and FP64, each on its own HIP stream, capacities sized from an HBM
-// budget) as soon as their ranges fit, advances both batches a few blocks at
-// a time (the two streams overlap on the device), and retires members that
-// reached their step count, reading their results and freeing their ranges
-// for the next admissions. No request waits for a batch to drain.
+// budget) as soon as their ranges fit, advances both batches in rounds of
+// blocks (the two streams overlap on the device; rounds are pipelined two
+// deep), and retires members that reached their step count, reading their
+// results and freeing their ranges for the next admissions. No request waits
+// for a batch to drain.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -320,6 +321,7 @@ private:
                 q->out.active_astro = active;
                 q->out.status = AS_REQ_DONE;
                 q->out.t_done = now();
+                cell_steps_ += double(q->out.n_neurons + q->out.n_astro) * double(q->in.n_steps);
                 ++finished_;
                 cv_done_.notify_all();
             }
@@ -391,7 +393,6 @@ private:
                     done_step_[b] = P.end_step[b];
                 }
                 gpu_busy_s_ += 1e-3 * double(round_ms);  // the longer of the two batches' spans
-                cell_steps_ += double(P.cells) * blocks * batch_[0]->block_len();
                 peak_active_ = std::max<std::uint64_t>(peak_active_, P.active);
                 peak_cells_ = std::max<std::uint64_t>(peak_cells_, P.cells);
                 const auto r0 = Clock::now();

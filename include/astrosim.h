@@ -254,7 +254,7 @@ typedef struct as_server_stats {
     uint64_t peak_resident_cells, capacity_cells;
     double gpu_busy_s;            /* device time of the rounds (per round, the
                                      longer batch's span) */
-    double cell_steps;            /* cells x steps advanced */
+    double cell_steps;            /* cells x steps of the completed requests */
     double admit_s, retire_s;     /* driver time admitting and retiring requests between rounds */
 } as_server_stats;
 
