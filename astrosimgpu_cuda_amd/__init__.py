@@ -524,10 +524,11 @@ class Server:
     requests are built by worker threads (networks from N_total = 100000 on
     with the GPU connectivity generator, kept in device memory; smaller ones
     by the host walk; ``setup_where`` as in include/astrosim.h), admitted into
-    resident FP32c/FP64 batches as their footprint fits, advanced with both
-    batches overlapping on the device, and retired when done. Give the process
-    more hardware queues than build threads + 4 (GPU_MAX_HW_QUEUES, read when
-    HIP initialises): streams that share a queue wait for each other's work."""
+    resident FP32c/FP64 batches as their footprint fits, advanced in rounds
+    of blocks pipelined two deep with both batches overlapping on the device,
+    and retired when done. Give the process more hardware queues than build
+    threads + 6 (GPU_MAX_HW_QUEUES, read when HIP initialises): streams that
+    share a queue wait for each other's work."""
 
     def __init__(self, **kw):
         lib = load()
