@@ -117,8 +117,11 @@ void build_network(index_t n_total, std::uint64_t seed, Network& net);
 // for its draw statistics, or a rare draw pattern it does not resolve); the
 // caller then uses build_network.
 // Smallest N_total for which the automatic choice (AS_BUILD_GPU) takes the
-// GPU generator: its latency is about constant, the host walk's grows with
-// the network, and they cross near here on one MI355X and its host.
+// GPU generator. Its latency is about constant (0.08-0.11 s from 20,000 to
+// 100,000 cells on one MI355X), the host walk's grows with the network
+// (0.07 s at 20,000, 0.21 s at 100,000), and alone they cross near 25,000.
+// GPU builds share the device with running simulations, though: serving
+// with the threshold at 30,000 measured slower than at 100,000.
 constexpr index_t kGpuBuildAutoMin = 100000;
 
 namespace gb { struct BuildStats; }
