@@ -1233,18 +1233,15 @@ public:
 
     ~BatchImpl() override {
         drop_graphs();
-        if (base_) {
-            (void)hipStreamSynchronize(stream_);
-            (void)hipFree(base_);
-        }
-        if (up_) {
-            (void)hipStreamSynchronize(up_);
-            (void)hipStreamDestroy(up_);
-        }
+        // the block, upload and read streams finish before anything they use is freed
+        (void)hipStreamSynchronize(stream_);
+        if (up_) (void)hipStreamSynchronize(up_);
+        if (rd_) (void)hipStreamSynchronize(rd_);
+        if (base_) (void)hipFree(base_);
+        if (up_) (void)hipStreamDestroy(up_);
         if (rd_) (void)hipStreamDestroy(rd_);
         for (auto& h : hosts_)
             if (h.up_done) (void)hipEventDestroy(h.up_done);
-        if (rd_) (void)hipStreamSynchronize(rd_);
         for (Pinned& b : pin_slot_)
             if (b.p) (void)hipHostFree(b.p);
         for (Pinned& b : pin_free_) (void)hipHostFree(b.p);
