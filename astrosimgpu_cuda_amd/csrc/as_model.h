@@ -219,15 +219,23 @@ struct NeuronRegs {
     int refractory;
 };
 
+// A neuron's reset potential and spike-triggered adaptation increment.
+struct ResetPair {
+    double V_reset, b;
+};
+
 // One communication step of one neuron after its inputs are known: the
 // substep loop with the exact alpha-cascade propagator, the refractory clamp
 // (Euler on w only), RK4 on (V, w), spike detection and reset.
 // `ex_decay`/`in_decay` are as_exp(-h_step/tau_syn_*) (population constants),
-// h_sixth is h_step / 6.0.
+// h_sixth is h_step / 6.0. `reset()` returns the neuron's ResetPair; it is
+// called only on the refractory clamp and on a spike, the two branches that
+// use them, so a caller can load them lazily.
 // Returns true if the neuron spiked. Reference: src/astrosim_fused.cu:508-553.
-AS_HD bool neuron_substeps(const NeuronConst& p, NeuronRegs& r, double I_ext, double V_reset,
-                           double b, double h_step, double h_sixth, int substeps, double ex_decay,
-                           double in_decay) {
+template <class Reset>
+AS_HD bool neuron_substeps_lazy(const NeuronConst& p, NeuronRegs& r, double I_ext, Reset&& reset,
+                                double h_step, double h_sixth, int substeps, double ex_decay,
+                                double in_decay) {
     bool spiked = false;
     for (int sub = 0; sub < substeps; ++sub) {
         const double g_ex_next = ex_decay * (r.g_ex + h_step * r.dg_ex);
@@ -235,7 +243,7 @@ AS_HD bool neuron_substeps(const NeuronConst& p, NeuronRegs& r, double I_ext, do
         const double g_in_next = in_decay * (r.g_in + h_step * r.dg_in);
         const double dg_in_next = in_decay * r.dg_in;
         if (r.refractory > 0) {
-            r.V = V_reset;
+            r.V = reset().V_reset;
             double dV, dw;
             neuron_rhs(p, r.V, r.w, r.g_ex, r.g_in, I_ext, dV, dw);
             r.w += h_step * dw;
@@ -260,13 +268,23 @@ AS_HD bool neuron_substeps(const NeuronConst& p, NeuronRegs& r, double I_ext, do
         if (r.refractory > 0) {
             --r.refractory;
         } else if (r.V >= p.V_peak) {
-            r.V = V_reset;
-            r.w += b;
+            const ResetPair rb = reset();
+            r.V = rb.V_reset;
+            r.w += rb.b;
             r.refractory = p.refractory_steps;
             spiked = true;
         }
     }
     return spiked;
+}
+
+// The same with the reset values passed in.
+AS_HD bool neuron_substeps(const NeuronConst& p, NeuronRegs& r, double I_ext, double V_reset,
+                           double b, double h_step, double h_sixth, int substeps, double ex_decay,
+                           double in_decay) {
+    return neuron_substeps_lazy(
+        p, r, I_ext, [&]() { return ResetPair{V_reset, b}; }, h_step, h_sixth, substeps, ex_decay,
+        in_decay);
 }
 
 // Per-presynaptic-neuron Tsodyks-Markram update on a spike at t_now; returns

@@ -70,7 +70,10 @@ def test_fp32c_config2_statistics_match_fp64():
           f"active {act64:.4f} vs {act32:.4f}; engaged {eng64:.4f} vs {eng32:.4f}", flush=True)
     for f in ("V", "Ca", "IP3", "h"):
         d = np.abs(e32[f] - e64[f])
-        print(f"[fp32c] state at 100 ms {f}: median |diff| {np.median(d):.3e} max {d.max():.3e}", flush=True)
+        r = d / np.maximum(np.abs(e64[f]), 1e-3)
+        print(f"[fp32c] state at 100 ms {f}: |diff| median {np.median(d):.3e} p99 {np.quantile(d, 0.99):.3e} "
+              f"max {d.max():.3e}; rel median {np.median(r):.3e} p99 {np.quantile(r, 0.99):.3e} "
+              f"max {r.max():.3e}", flush=True)
     assert abs(rate32 / rate64 - 1.0) < 0.01
     assert abs(act32 - act64) < 0.01
     assert abs(eng32 - eng64) < 0.01

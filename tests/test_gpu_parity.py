@@ -94,6 +94,20 @@ def test_tiny_networks(method, n):
     run_pair(n, 2, 300, method=method)
 
 
+def test_plain_split_launches_bit_exact():
+    # from 2^16 neurons the plain method runs a gather kernel and a cell
+    # kernel per step (summary bits in LDS confirmed against the flag words)
+    run_pair(70000, 13, 300, method="plain", chunk=10)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_poisson_normal_approximation_bit_exact(method):
+    # dt 12 ms: the excitatory Poisson mean is 2700 Hz * 12 ms = 32.4 > 30,
+    # so rng_poisson takes its normal-approximation branch (src:160-165);
+    # the inhibitory mean is exactly 30 (Knuth, long chains); one-step delays
+    run_pair(3000, 17, 60, dt=12.0, chunk=1, method=method)
+
+
 def test_blocked_larger_network_bit_exact():
     # 50k cells, 200 ms: exercises bucket overflow fallbacks at the onset
     run_pair(50000, 11, 2000, method="blocked", chunk=10)
