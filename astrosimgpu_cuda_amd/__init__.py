@@ -124,6 +124,7 @@ SIGNATURES = [
     ("as_sim_read_state", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     ("as_sim_read_refractory", C.c_int, [C.c_void_p, C.c_void_p]),
     ("as_sim_read_ever_active", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("as_sim_read_spike_counts", C.c_int, [C.c_void_p, C.c_void_p]),
     ("as_sim_read_raster", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
     ("as_sim_bytes", C.c_uint64, [C.c_void_p]),
     ("as_sim_set_timing", C.c_int, [C.c_void_p, C.c_int]),
@@ -147,6 +148,7 @@ SIGNATURES = [
     ("as_batch_read_state", C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     ("as_batch_read_ever_active", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     ("as_batch_read_raster", C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_int32, C.c_void_p]),
+    ("as_batch_read_spike_counts", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     ("as_batch_set_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("as_batch_kernel_time", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     ("as_batch_destroy", None, [C.c_void_p]),
@@ -353,6 +355,12 @@ class Simulation:
         _check(load().as_sim_read_ever_active(self.handle, _ptr(out)))
         return out
 
+    def spike_counts(self) -> np.ndarray:
+        """Spikes of every neuron since creation."""
+        out = np.empty(self.n_neurons, dtype=np.uint32)
+        _check(load().as_sim_read_spike_counts(self.handle, _ptr(out)))
+        return out
+
     def raster(self, first_step: int, n_steps: int) -> np.ndarray:
         out = np.empty((n_steps, self.n_neurons), dtype=np.uint8)
         _check(load().as_sim_read_raster(self.handle, C.c_int64(first_step), C.c_int32(n_steps), _ptr(out)))
@@ -488,6 +496,11 @@ class Batch:
         out = np.empty((n_steps, self._sizes[member][0]), dtype=np.uint8)
         _check(load().as_batch_read_raster(self.handle, member, C.c_int64(first_step), C.c_int32(n_steps),
                                            _ptr(out)))
+        return out
+
+    def spike_counts(self, member: int) -> np.ndarray:
+        out = np.empty(self._sizes[member][0], dtype=np.uint32)
+        _check(load().as_batch_read_spike_counts(self.handle, member, _ptr(out)))
         return out
 
     def set_timing(self, on: bool) -> None:

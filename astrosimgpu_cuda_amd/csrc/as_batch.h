@@ -61,6 +61,7 @@ public:
     virtual void read_state(int member, int field, double* out) = 0;
     virtual void read_refractory(int member, std::int32_t* out) = 0;
     virtual void read_ever_active(int member, std::uint8_t* out) = 0;
+    virtual void read_spike_counts(int member, std::uint32_t* out) = 0;  // spikes per neuron
     virtual void read_raster(int member, std::int64_t first_local, int n, std::uint8_t* out) = 0;
     virtual std::int64_t member_steps(int member) const = 0;  // local steps enqueued
     // Local steps the member has run once the device reaches global step g.

@@ -79,6 +79,7 @@ struct BatchDev {
     uint4* bent[4];  // [target][kBucketCap] bucket entries (see entry layout below)
     std::uint8_t* ever;
     std::int32_t* refr;
+    std::uint32_t* spk_n;  // spikes per neuron since admission
     std::uint32_t* spk_w;  // [H][ncap]
     std::uint32_t* sic_w;  // [H][acap]
     std::uint32_t* nlist;  // [H][ncap]
@@ -868,6 +869,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     at(S.g_in, on) = g_in;
     at(S.dg_in, on) = dg_in;
     at(D.refr, o4) = refr;
+    if (nspikes != 0u) at(D.spk_n, o4) += nspikes;
     at(S.I_sic, on) = I_sic;
     at(D.spk_w + std::size_t(B.wslot) * D.ncap, o4) = wout;
     if (B.last) {
@@ -1083,7 +1085,7 @@ private:
 
 struct BatchLayout {
     std::size_t in_rs[4], in_src[4], out_rs[4], out_ent[4], bcnt[4], bent[4];
-    std::size_t astro[10], ever, neuron[12], refr;
+    std::size_t astro[10], ever, neuron[12], refr, spk_n;
     std::size_t spk_w, spk_v, sic_w, sic_v, nlist, alist, lcount, counters, overflow;
     std::size_t members, tiles, clock, ctx, scratch0, scratch1, scan_tmp;
     std::size_t scan_tmp_bytes, tiles_cap, spans;
@@ -1119,6 +1121,7 @@ BatchLayout batch_layout(const BatchCapacity& c, const RunParams& rp) {
     o.ever = L.add(na);
     for (int i = 0; i < 12; ++i) o.neuron[i] = L.add(nn * rsize);
     o.refr = L.add(nn * sizeof(std::int32_t));
+    o.spk_n = L.add(nn * sizeof(std::uint32_t));
     o.spk_w = L.add(std::size_t(g.H) * nn * 4);
     o.spk_v = L.add(std::size_t(g.Rv) * nn * rsize);
     o.sic_w = L.add(std::size_t(g.H) * na * 4);
@@ -1192,6 +1195,7 @@ public:
         S_.sic_v = (R*)dptr(lay_.sic_v);
         D_.ever = (std::uint8_t*)dptr(lay_.ever);
         D_.refr = (std::int32_t*)dptr(lay_.refr);
+        D_.spk_n = (std::uint32_t*)dptr(lay_.spk_n);
         D_.spk_w = (std::uint32_t*)dptr(lay_.spk_w);
         D_.sic_w = (std::uint32_t*)dptr(lay_.sic_w);
         D_.nlist = (std::uint32_t*)dptr(lay_.nlist);
@@ -1456,6 +1460,12 @@ public:
     void read_refractory(int member, std::int32_t* out) override {
         const MemberHost& h = host(member);
         AS_HIP_CHECK(hipMemcpyAsync(out, D_.refr + h.m.nb, h.m.nn * 4, hipMemcpyDeviceToHost, stream_));
+        AS_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    void read_spike_counts(int member, std::uint32_t* out) override {
+        const MemberHost& h = host(member);
+        AS_HIP_CHECK(hipMemcpyAsync(out, D_.spk_n + h.m.nb, h.m.nn * 4, hipMemcpyDeviceToHost, stream_));
         AS_HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
@@ -1866,6 +1876,7 @@ private:
         fill_on(s, S_.stp_u + m.nb, R(P_.stp_U), m.nn);
         fill_on(s, S_.stp_t + m.nb, R(-1e12), m.nn);
         fill_on(s, D_.refr + m.nb, std::int32_t(0), m.nn);
+        fill_on(s, D_.spk_n + m.nb, std::uint32_t(0), m.nn);
         fill_on(s, D_.ever + m.ab, std::uint8_t(0), m.na);
         for (int hs = 0; hs < geo_.H; ++hs) {
             fill_on(s, D_.spk_w + std::size_t(hs) * D_.ncap + m.nb, std::uint32_t(0), m.nn);
@@ -1934,6 +1945,7 @@ public:
     void read_state(int field, double* out) override { batch_->read_state(member_, field, out); }
     void read_refractory(std::int32_t* out) override { batch_->read_refractory(member_, out); }
     void read_ever_active(std::uint8_t* out) override { batch_->read_ever_active(member_, out); }
+    void read_spike_counts(std::uint32_t* out) override { batch_->read_spike_counts(member_, out); }
     void read_raster(std::int64_t first, int n, std::uint8_t* out) override {
         batch_->read_raster(member_, first, n, out);
     }

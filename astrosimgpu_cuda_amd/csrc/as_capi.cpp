@@ -251,6 +251,13 @@ as_status as_sim_read_ever_active(as_sim* sim, uint8_t* out) {
     });
 }
 
+as_status as_sim_read_spike_counts(as_sim* sim, uint32_t* out) {
+    return guarded([&] {
+        require(sim && out, "NULL argument");
+        sim->eng->read_spike_counts(out);
+    });
+}
+
 as_status as_sim_read_raster(as_sim* sim, int64_t first_step, int32_t n_steps, uint8_t* out) {
     return guarded([&] {
         require(sim && out, "NULL argument");
@@ -437,6 +444,13 @@ as_status as_batch_read_ever_active(as_batch* b, int32_t member, uint8_t* out) {
     return guarded([&] {
         require(b && out, "NULL argument");
         b->b->read_ever_active(member, out);
+    });
+}
+
+as_status as_batch_read_spike_counts(as_batch* b, int32_t member, uint32_t* out) {
+    return guarded([&] {
+        require(b && out, "NULL argument");
+        b->b->read_spike_counts(member, out);
     });
 }
 

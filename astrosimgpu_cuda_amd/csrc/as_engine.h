@@ -80,6 +80,7 @@ public:
     virtual void read_state(int field, double* out) = 0;  // synchronises
     virtual void read_refractory(std::int32_t* out) = 0;
     virtual void read_ever_active(std::uint8_t* out) = 0;
+    virtual void read_spike_counts(std::uint32_t* out) = 0;  // spikes per neuron since creation
     virtual void read_raster(std::int64_t first, int n, std::uint8_t* out) = 0;
     virtual std::uint64_t device_bytes() const = 0;
 

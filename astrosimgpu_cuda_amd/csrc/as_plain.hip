@@ -56,6 +56,7 @@ struct PlainDev {
     std::uint8_t* ever;
     double *V, *w, *g_ex, *dg_ex, *g_in, *dg_in, *I_sic;
     std::int32_t* refr;
+    std::uint32_t* spk_n;  // spikes per neuron since creation
     const double *V_reset, *b;
     double *stp_x, *stp_u, *stp_t;
     std::uint64_t *flag_n, *flag_s;  // [R][wn], [R][wa] flag words per 64 cells
@@ -359,6 +360,7 @@ __device__ __forceinline__ bool neuron_cell(const PlainDev& D, const RunParams& 
     D.dg_in[j] = g.dg_in;
     if (g.refractory != refr0) D.refr[j] = g.refractory;
     if (spiked) {
+        D.spk_n[j] += 1u;
         if (P.stp_enabled) {
             double x = D.stp_x[j], u = D.stp_u[j], tl = D.stp_t[j];
             out = stp_on_spike(x, u, tl, static_cast<double>(S.step) * P.dt, P.stp_U, P.stp_tau_rec, P.stp_tau_fac);
@@ -537,7 +539,7 @@ __global__ void fill_kernel(double* p, double v, std::size_t n) {
 struct PlainLayout {
     std::size_t csr_rs[4], csr_src[4];
     std::size_t astro[7], ever;
-    std::size_t neuron[12], refr;
+    std::size_t neuron[12], refr, spk_n;
     std::size_t flag_n, flag_s, val_n, val_s, sum_n, sum_s, counters;
     std::size_t in[4], mask[4];  // split mode: exc, inh, sic, e2a
     bool split;
@@ -578,6 +580,7 @@ PlainLayout plain_layout(const Network& net, const RunParams& rp) {
     o.ever = L.add(na);
     for (int i = 0; i < 12; ++i) o.neuron[i] = L.add(nn * sizeof(double));
     o.refr = L.add(nn * sizeof(std::int32_t));
+    o.spk_n = L.add(nn * sizeof(std::uint32_t));
     const std::size_t R = std::size_t(rp.ring_slots);
     o.wn = ceil_div(nn, 64);
     o.wa = ceil_div(na, 64);
@@ -665,6 +668,7 @@ public:
         D_.I_sic = N[6]; D_.stp_x = N[7]; D_.stp_u = N[8]; D_.stp_t = N[9];
         D_.V_reset = N[10]; D_.b = N[11];
         D_.refr = (std::int32_t*)dptr(lay_.refr);
+        D_.spk_n = (std::uint32_t*)dptr(lay_.spk_n);
         D_.flag_n = (std::uint64_t*)dptr(lay_.flag_n);
         D_.flag_s = (std::uint64_t*)dptr(lay_.flag_s);
         D_.val_n = (double*)dptr(lay_.val_n);
@@ -739,6 +743,7 @@ public:
         fill(N[8], rp.stp_U, n_neurons_);
         fill(N[9], -1e12, n_neurons_);
         AS_HIP_CHECK(hipMemsetAsync(D_.refr, 0, n_neurons_ * sizeof(std::int32_t), stream_));
+        AS_HIP_CHECK(hipMemsetAsync(D_.spk_n, 0, n_neurons_ * sizeof(std::uint32_t), stream_));
         AS_HIP_CHECK(hipMemsetAsync(D_.ever, 0, n_astro_, stream_));
         // Flags, values and summaries: everything from the ring on is zero at step 0.
         AS_HIP_CHECK(hipMemsetAsync(dptr(lay_.flag_n), 0, lay_.total - lay_.flag_n, stream_));
@@ -819,6 +824,11 @@ public:
 
     void read_refractory(std::int32_t* out) override {
         AS_HIP_CHECK(hipMemcpyAsync(out, D_.refr, n_neurons_ * sizeof(std::int32_t), hipMemcpyDeviceToHost, stream_));
+        AS_HIP_CHECK(hipStreamSynchronize(stream_));
+    }
+
+    void read_spike_counts(std::uint32_t* out) override {
+        AS_HIP_CHECK(hipMemcpyAsync(out, D_.spk_n, n_neurons_ * sizeof(std::uint32_t), hipMemcpyDeviceToHost, stream_));
         AS_HIP_CHECK(hipStreamSynchronize(stream_));
     }
 

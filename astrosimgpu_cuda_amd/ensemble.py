@@ -63,7 +63,7 @@ def run_batched(nets: list[Network], steps: int, precision: str = "fp32c", graph
     info = batch.info()
     batch.close()
     return {"wall_s": wall, "setup_s": t0 - t_create, "sims": len(nets), "sims_per_s": len(nets) / wall,
-            "steps": steps, "block_threads": chosen, "device_bytes": info["device_bytes"],
+            "steps": steps, "block_threads": chosen, "graph_blocks": graph_blocks, "device_bytes": info["device_bytes"],
             "bucket_overflows": info["bucket_overflows"], "spikes": spikes}
 
 

@@ -2,26 +2,44 @@
 pipeline on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--seed S]
-                    [--legs fp64:plain,...] [--no-cpu-baseline]
+                    [--legs fp64:plain,...] [--no-cpu-baseline] [--no-ensemble]
+                    [--serving-seconds S] [--no-config1] [--no-activity]
 
 Workload (default): the full-scale single network, N_total = 10,000,000
 (8M excitatory + 2M inhibitory neurons, 2M astrocytes, ~1.32G int32 in-edges),
 seed 42, dt 0.1 ms. A "step" is one simulated 0.1 ms step of every cell. The
 network is generated (on the GPU, identical to the host walk of the setup
-stream) and resident in HBM before timing starts;
-W untimed steps, then exactly K timed steps bracketed by device syncs. The
-default K = 11,000 is the reference's run at this size (pre 100 ms + sim
-1000 ms; its steps/s covers both phases), onset transient included. 
+stream) and resident in HBM before timing starts. Each leg (precision and
+method) runs W untimed steps, then exactly K timed steps bracketed by device
+syncs. The default K = 11,000 is the reference's run at this size (pre 100
+ms + sim 1000 ms; its steps/s covers both phases), onset transient included.
 
-One JSON line is printed. `value` is the headline leg's (the last leg's)
-steps/s; every leg's figures are in `legs`. `roofline` is for the headline
-leg's step kernel: algorithmic bytes per launch (the reference layout's bytes
-per step, with this network's exact edge counts, times the steps one launch
-advances) over the average launch time measured with HIP events on the
-simulation's stream; `traffic` is the rocprofv3-measured HBM bytes per launch
-(FETCH_SIZE and WRITE_SIZE scaled by a known-size copy) recorded under
-profiles/. `cpu_baseline` times the FP64 host oracle's stepper on this same
-network on the host cores for a bounded number of steps.
+One JSON line is printed. `value` is the headline leg's (the last leg's:
+FP32c, blocked method) steps/s; every leg's figures are in `legs`:
+  - steps/s over the timed window, and synaptic events/s: every spike of the
+    window times its neuron's out-degree over the connections that deliver it
+    (excitatory, inhibitory and neuron-to-astrocyte in-CSRs, counted by
+    source id), over the window's wall time;
+  - the dominant kernel's time per FULL launch (one step for the plain
+    method, one 10-step block for the blocked method): its time summed over
+    the timed window (HIP events around every launch on the simulation's
+    stream) per 10 (blocked) or 1 (plain) steps, so that partial launches at
+    the window's ends do not change the basis; and, separately, the average
+    of 20 (blocked) or 100 (plain) whole launches right after the window;
+  - that kernel's own algorithmic HBM bytes per launch (see
+    kernel_bytes_per_launch), the GB/s they give at that launch time and the
+    fraction of the 8 TB/s peak, beside the rocprofv3-measured HBM bytes
+    (profiles/traffic_*.json: FETCH_SIZE and WRITE_SIZE scaled by a
+    known-size copy) and, for the blocked kernels, the vector-ALU busy
+    fraction (profiles/valu_*.json, SQ counters).
+`roofline` repeats the headline leg's figures. `fp32c_vs_fp64` runs both
+blocked legs over the reference's whole run (1,000 + 10,000 steps, untimed)
+and compares the recorded window's firing rate and SIC fractions; the FP64
+blocked path is bit-identical to the FP64 host oracle. `cpu_baseline` times
+the FP64 host oracle's stepper on the same network. `config1` times the
+reference's default-size run (N_total = 500, 60,000 steps), `activity` the
+two FP64 methods on a 1M-cell network at normal and at SIC-driven high
+activity, `ensemble` the 256-network batch and `serving` the 40 req/s load.
 
     python bench.py --copy-calibration BYTES   # the known-size copy alone
 """
@@ -39,12 +57,13 @@ sys.path.insert(0, str(ROOT))
 
 # Hardware queues per process, read when HIP initialises (nothing has yet).
 # The serving leg runs the two batch streams with their upload and read
-# streams beside up to 12 network-build streams; with HIP's default of 4 queues those
-# streams share queues, and every batch synchronisation then also waits for
-# unrelated build kernels.
-os.environ["GPU_MAX_HW_QUEUES"] = "24"
+# streams beside the network-build streams; with HIP's default of 4 queues
+# those streams share queues, and every batch synchronisation then also
+# waits for unrelated build kernels.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "24")
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+SIC_TH = 0.19669
 
 
 def log(msg: str) -> None:
@@ -67,14 +86,16 @@ def reference_bytes_per_step(net, prec: str) -> float:
 
 
 def kernel_bytes_per_launch(net, prec: str, method: str) -> float:
-    """Algorithmic HBM bytes of one launch of the dominant step kernel.
+    """Algorithmic HBM bytes of one launch of the method's dominant kernel.
 
-    plain: the reference-layout step (reference_bytes_per_step).
-    blocked: one integrate launch advances every cell one block; per neuron
-    it reads and writes V, w, g_ex, dg_ex, g_in, dg_in, I_sic (14s), reads
-    V_reset and b (2s), reads and writes the refractory count (8), reads
-    three bucket counts and clears them (24) and writes its spike word (4);
-    per astrocyte Ca, IP3, h r/w (6s, plus 6s of compensation terms in
+    plain (one step of the gather and cell kernels): the reference layout's
+    step (reference_bytes_per_step): the plain method streams every in-edge
+    every step.
+    blocked (one integrate launch advances every cell one 10-step block): per
+    neuron it reads and writes V, w, g_ex, dg_ex, g_in, dg_in, I_sic (14s),
+    reads V_reset and b (2s), reads and writes the refractory count (8),
+    reads three bucket counts and clears them (24) and writes its spike word
+    (4); per astrocyte Ca, IP3, h r/w (6s, plus 6s of compensation terms in
     FP32c), four parameters (4s), bucket count r/w (8), SIC word (4) and the
     ever-active flag (1). Bucket entries, emitted values and STP updates
     scale with activity (a few per cent of cells per block) and are left out.
@@ -87,39 +108,100 @@ def kernel_bytes_per_launch(net, prec: str, method: str) -> float:
     return float(net.n_neurons * neuron + net.n_astro * astro)
 
 
-def run_leg(A, net, prec: str, method: str, steps: int, warmup: int, block: int = 128) -> dict:
+def out_degrees(net):
+    """Per neuron, the in-CSR entries that deliver its spikes: excitatory and
+    neuron-to-astrocyte rows by source id, and inhibitory rows, whose stored
+    local ids index the neuron ring as they are (src/astrosim_fused.cu:461)."""
+    import numpy as np
+    deg = np.zeros(net.n_neurons, dtype=np.int64)
+    for which in ("exc", "inh", "e2a"):
+        _, src = net.csr(which)
+        deg += np.bincount(src, minlength=net.n_neurons)[: net.n_neurons]
+        del src
+    return deg
+
+
+def run_leg(A, net, prec: str, method: str, steps: int, warmup: int, block: int, deg) -> dict:
+    import numpy as np
     log(f"leg {prec}/{method}: create")
     t0 = time.perf_counter()
     sim = A.Simulation(net, precision=prec, method=method, block=block)
     sim.sync()
-    create_s = time.perf_counter() - t0  # host-to-device upload of the network + device setup
+    create_s = time.perf_counter() - t0  # device-to-device copy of the network + device setup
     sim.step(warmup)
     sim.sync()
+    counts0 = sim.spike_counts() if deg is not None else None
     sim.set_timing(True)
     t0 = time.perf_counter()
     sim.step(steps)
     sim.sync()
     wall = time.perf_counter() - t0
-    kms, launches = sim.kernel_time()
+    kms_window, launches_window = sim.kernel_time()
     sim.set_timing(False)
     spikes = sim.spike_count()
-    # SIC fractions as the reference's summary reports them (src/astrosim_fused.cu:
-    # 1117-1125): Ca above threshold at the final step, and ever above it
+    events = None
+    if deg is not None:
+        d = sim.spike_counts().astype(np.int64) - counts0.astype(np.int64)
+        events = int(d @ deg)
+    # full launches right after the window: align to a block, then time whole blocks
+    L = 10 if method == "blocked" else 1
+    n_full = 20 if method == "blocked" else 100
+    sim.step((-sim.steps) % L)
+    sim.set_timing(True)
+    sim.step(n_full * L)
+    kms, launches = sim.kernel_time()
+    sim.set_timing(False)
     ca = sim.state("Ca")
-    sic_active = float(((ca - 0.19669) * 1000.0 > 1.0).mean())
-    sic_engaged = float(sim.ever_active().astype(bool).mean())
     out = {
-        "precision": prec, "method": method, "block": block, "steps": steps, "wall_s": wall,
+        "precision": prec, "method": method, "block": block, "steps": steps, "warmup": warmup, "wall_s": wall,
         "steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
-        "kernel_ms_total": kms, "launches": launches,
-        "kernel_ms_per_launch": kms / max(1, launches),
-        "steps_per_launch": steps / max(1, launches),
-        "device_bytes": sim.device_bytes(), "spikes_total": spikes,
-        "sic_active_fraction": sic_active, "sic_engaged_fraction": sic_engaged,
+        "spikes_in_window": None if deg is None else int(d.sum()),
+        "synaptic_events_per_s": None if events is None else events / wall,
+        # the window's kernel time per L steps: a full launch's time, whatever
+        # partial launches the window starts or ends with (the basis of the roofline)
+        "kernel_ms_per_full_launch": kms_window * L / steps,
+        "kernel_ms_window_total": kms_window, "launches_in_window": launches_window,
+        "kernel_ms_per_full_launch_after_window": kms / max(1, launches), "full_launches_after_window": launches,
+        "steps_per_launch": L, "device_bytes": sim.device_bytes(), "spikes_total": spikes,
+        "sic_active_fraction": float(((ca - SIC_TH) * 1000.0 > 1.0).mean()),
         "create_s": create_s,
     }
     sim.close()
-    log(f"leg {prec}/{method}: {out['steps_per_s']:.1f} steps/s, {out['kernel_ms_per_launch']:.3f} ms/launch")
+    log(f"leg {prec}/{method}: {out['steps_per_s']:.1f} steps/s, {out['kernel_ms_per_full_launch']:.3f} ms "
+        f"per full launch" + ("" if events is None else f", {out['synaptic_events_per_s']:.3e} synaptic events/s"))
+    return out
+
+
+def reference_run(A, net, prec: str, pre: int = 1000, sim_steps: int = 10000) -> dict:
+    """The reference's run (pre steps, ever-active reset, sim steps) on the
+    blocked method: recorded-window rate and SIC fractions
+    (src/astrosim_fused.cu:1066-1076, 1117-1129)."""
+    import numpy as np
+    sim = A.Simulation(net, precision=prec, method="blocked")
+    sim.step(pre)
+    before = sim.spike_count()
+    sim.reset_ever_active()
+    sim.step(sim_steps)
+    spikes = sim.spike_count() - before
+    ca = sim.state("Ca")
+    out = {"rate_hz": spikes / (net.n_neurons * sim_steps * 0.1e-3),
+           "sic_active": float(np.mean((ca - SIC_TH) * 1000.0 > 1.0)),
+           "sic_engaged": float(np.mean(sim.ever_active()))}
+    sim.close()
+    return out
+
+
+def accuracy(A, net) -> dict:
+    r64 = reference_run(A, net, "fp64")
+    r32 = reference_run(A, net, "fp32c")
+    out = {"fp64": r64, "fp32c": r32, "rate_rel": r32["rate_hz"] / r64["rate_hz"] - 1.0, "rate_tolerance": 0.01,
+           "sic_active_diff": r32["sic_active"] - r64["sic_active"],
+           "sic_engaged_diff": r32["sic_engaged"] - r64["sic_engaged"], "sic_tolerance": 0.01,
+           "window_steps": 11000,
+           "note": "both blocked legs over pre 1,000 + sim 10,000 steps, untimed; FP64 blocked is "
+                   "bit-identical to the FP64 host oracle"}
+    log(f"fp32c vs fp64 (11,000 steps): rate {100 * out['rate_rel']:+.4f}%, SIC active "
+        f"{out['sic_active_diff']:+.6f}, engaged {out['sic_engaged_diff']:+.6f}")
     return out
 
 
@@ -157,6 +239,72 @@ def cpu_baseline(net, seed: int, n_steps: int) -> dict:
                       f"network (after 1 warm step), OpenMP over cells"}
 
 
+def config1_leg(A) -> dict:
+    """The reference's default-size run: N_total = 500, seed 1, dt 0.1 ms,
+    pre 1000 + sim 5000 ms = 60,000 steps, FP64, launch-bound."""
+    net = A.Network(500, 1)
+    out = {"workload": "N_total=500 seed=1 dt=0.1ms, 60,000 steps (pre 1000 ms + sim 5000 ms), FP64"}
+    for method in ("plain", "blocked"):
+        sim = A.Simulation(net, method=method)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(60000)
+        sim.sync()
+        wall = time.perf_counter() - t0
+        out[method] = {"steps_per_s": 60000 / wall, "us_per_step": 1e6 * wall / 60000, "wall_s": wall,
+                       "spikes": sim.spike_count()}
+        sim.close()
+        log(f"config 1 {method}: {60000 / wall:.0f} steps/s ({1e6 * wall / 60000:.2f} us/step)")
+    return out
+
+
+def activity_leg(A, n: int = 1_000_000, seed: int = 7, warm: int = 2000, steps: int = 1000) -> dict:
+    """The two FP64 methods on one network at normal activity and at high,
+    SIC-driven activity (n2a and a2n weights raised: w_n2a 100, w_a2n 5000).
+    The plain method's cost is nearly independent of activity; the blocked
+    method's scatter grows with the emitters and its buckets overflow into
+    whole-row walks. Times `steps` steps after `warm`."""
+    from astrosimgpu_cuda_amd import Batch
+    net = A.Network(n, seed, setup="gpu")
+    out = {"workload": f"N_total={n} seed={seed}, FP64, steps {warm}..{warm + steps}", "cases": []}
+    for label, kw in (("normal", {}), ("high", {"w_n2a": 100.0, "w_a2n": 5000.0})):
+        case = {"activity": label, **kw}
+        sim = A.Simulation(net, method="plain", **kw)
+        sim.step(warm)
+        sim.sync()
+        s0 = sim.spike_count()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.sync()
+        wall = time.perf_counter() - t0
+        case["plain_steps_per_s"] = steps / wall
+        case["rate_hz"] = (sim.spike_count() - s0) / (net.n_neurons * steps * 0.1e-3)
+        sim.close()
+        b = Batch([net], **kw)
+        m = b.add(net, warm + steps)
+        b.step(warm)
+        b.sync()
+        ov0 = b.info()["bucket_overflows"]
+        b.set_timing(True)
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        wall = time.perf_counter() - t0
+        kms, launches = b.kernel_time()
+        case["blocked_steps_per_s"] = steps / wall
+        case["blocked_integrate_ms_per_block"] = kms / max(1, launches)
+        case["blocked_other_ms_per_block"] = 1e3 * wall / max(1, launches) - kms / max(1, launches)
+        case["blocked_bucket_overflows_per_block"] = (b.info()["bucket_overflows"] - ov0) / max(1, launches)
+        case["blocked_over_plain"] = case["blocked_steps_per_s"] / case["plain_steps_per_s"]
+        b.close()
+        out["cases"].append(case)
+        log(f"activity {label}: {case['rate_hz']:.2f} Hz; plain {case['plain_steps_per_s']:.0f} steps/s, "
+            f"blocked {case['blocked_steps_per_s']:.0f} steps/s (integrate "
+            f"{case['blocked_integrate_ms_per_block']:.3f} + other {case['blocked_other_ms_per_block']:.3f} ms "
+            f"per block, {case['blocked_bucket_overflows_per_block']:.0f} overflows per block)")
+    return out
+
+
 def ensemble_leg(pre_ms: float = 100.0, sim_ms: float = 500.0, n_sample: int = 8) -> dict:
     """Batched ensemble: 256 networks, N_total log-uniform in [500, 200000]
     from seed 3, seeds 1000+i, dt 0.1 ms, pre 100 ms + sim 500 ms, FP32c, in
@@ -172,8 +320,9 @@ def ensemble_leg(pre_ms: float = 100.0, sim_ms: float = 500.0, n_sample: int = 8
     build_s = time.perf_counter() - t0
     log(f"ensemble: built 256 networks ({sum(sizes)} cells) in {build_s:.1f} s")
     batched = E.run_batched(nets, steps, precision="fp32c")
+    incl = len(nets) / (batched["wall_s"] + batched["setup_s"])
     log(f"ensemble: batched {batched['sims_per_s']:.2f} sims/s ({batched['wall_s']:.2f} s, "
-        f"tile {batched['block_threads']})")
+        f"tile {batched['block_threads']}); with admission {incl:.2f} sims/s")
     order = sorted(range(len(nets)), key=lambda i: sizes[i])
     pick = [order[int((k + 0.5) * len(order) / n_sample)] for k in range(n_sample)]
     sample = [nets[i] for i in pick]
@@ -186,15 +335,17 @@ def ensemble_leg(pre_ms: float = 100.0, sim_ms: float = 500.0, n_sample: int = 8
         "networks": 256, "cells": sum(sizes) + sum(max(1, round(n / 5)) for n in sizes),
         "steps_per_sim": steps, "precision": "fp32c",
         "batched_sims_per_s": batched["sims_per_s"], "batched_wall_s": batched["wall_s"],
-        "batched_setup_s": batched["setup_s"], "tile_threads": batched["block_threads"],
+        "batched_setup_s": batched["setup_s"], "batched_sims_per_s_with_admission": incl,
+        "tile_threads": batched["block_threads"], "graph_blocks": batched["graph_blocks"],
         "network_build_s": build_s, "network_setup": "gpu",
         "bucket_overflows": batched["bucket_overflows"], "device_bytes": batched["device_bytes"],
         "one_at_a_time_plain_fp64_sims_per_s": plain["sims_per_s"],
         "one_at_a_time_blocked_fp32c_sims_per_s": blk["sims_per_s"],
         "speedup_vs_plain_fp64": batched["sims_per_s"] / plain["sims_per_s"],
+        "speedup_vs_plain_fp64_with_admission": incl / plain["sims_per_s"],
         "speedup_vs_blocked_one_at_a_time": batched["sims_per_s"] / blk["sims_per_s"],
         "sample": f"{n_sample} networks at the size quantiles, N={[sizes[i] for i in pick]}",
-        "members_equal_alone": same,
+        "sampled_members_equal_alone": same,
     }
 
 
@@ -217,7 +368,7 @@ def serving_leg(duration_s: float, blocks_per_round: int = 16) -> dict:
     q = lambda v, p: float(sorted(v)[min(len(v) - 1, int(p * len(v)))]) if v else None
     return {
         "arrival_rate_hz": 40.0, "arrival_window_s": duration_s, "requests": len(load),
-        "blocks_per_round": blocks_per_round,
+        "blocks_per_round": blocks_per_round, "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
         "completed": st["completed"], "rejected": st["rejected"], "all_done": st["all_done"],
         "p50_latency_s": st["p50_latency_s"], "p99_latency_s": st["p99_latency_s"],
         "mean_latency_s": st["mean_latency_s"], "sims_per_s": st["sims_per_s"],
@@ -231,11 +382,11 @@ def serving_leg(duration_s: float, blocks_per_round: int = 16) -> dict:
     }
 
 
-def load_traffic(workload: str) -> float | None:
-    p = ROOT / "profiles" / f"traffic_{workload}.json"
+def load_profile(kind: str, workload: str) -> dict | None:
+    p = ROOT / "profiles" / f"{kind}_{workload}.json"
     if p.exists():
         try:
-            return float(json.loads(p.read_text())["hbm_bytes_per_launch"])
+            return json.loads(p.read_text())
         except Exception:
             return None
     return None
@@ -265,6 +416,37 @@ def copy_calibration(nbytes: int) -> int:
     return 0
 
 
+KERNELS = {"plain": "plain_gather_kernel + plain_cell_kernel (one step)",
+           "blocked": "integrate_kernel (one 10-step block)"}
+
+
+def add_roofline(net, leg: dict, workload: str) -> dict:
+    """The leg's dominant kernel against the HBM roofline, on the kernel's
+    own algorithmic bytes and its full-launch time."""
+    prec, method = leg["precision"], leg["method"]
+    launch_s = leg["kernel_ms_per_full_launch"] * 1e-3
+    own = kernel_bytes_per_launch(net, prec, method)
+    achieved = own / launch_s / 1e9
+    traffic = load_profile("traffic", f"{method}_{prec}_{workload}")
+    valu = load_profile("valu", f"{method}_{prec}_{workload}")
+    tb = None if traffic is None else float(traffic["hbm_bytes_per_launch"])
+    r = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": achieved / HBM_PEAK_GBS, "traffic": tb,
+         "kernel": KERNELS[method], "algorithmic_bytes_per_launch": own,
+         "kernel_ms_per_full_launch": leg["kernel_ms_per_full_launch"],
+         "measured_gbs": None if tb is None else tb / launch_s / 1e9,
+         "measured_frac": None if tb is None else tb / launch_s / 1e9 / HBM_PEAK_GBS,
+         "traffic_over_algorithmic": None if tb is None else tb / own,
+         "valu_busy_frac": None if valu is None else valu.get("valu_busy_frac"),
+         "reference_layout_bytes_per_step": reference_bytes_per_step(net, prec)}
+    if method == "blocked":
+        r["note"] = ("bound by vector-instruction issue (valu_busy_frac), not by HBM: the kernel reads each "
+                     "cell's state once per 10-step block and its inputs from buckets filled by the scatter")
+    else:
+        r["note"] = "the plain method streams every in-edge every step: the reference layout's bytes"
+    return r
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,8 +461,12 @@ def main() -> int:
     ap.add_argument("--block", type=int, default=128, help="threads per workgroup (the CLI's --block)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--serving-seconds", type=float, default=10.0,
-                    help="arrival window of the serving load (0 skips it; the full load is 120 s)")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-events", action="store_true", help="skip the synaptic-events count (out-degrees)")
+    ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--no-activity", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=120.0,
+                    help="arrival window of the serving load (0 skips it)")
     ap.add_argument("--serving-blocks", type=int, default=16,
                     help="1 ms blocks the server advances between admission checks")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -301,45 +487,34 @@ def main() -> int:
     setup_s = time.perf_counter() - t0
     log(f"network built in {setup_s:.1f} s ({'GPU' if net.built_on_gpu else 'host'} generator, "
         f"connectivity and parameters {net.build_seconds:.2f} s), edges {net.n_edges}")
+    deg = None if args.no_events else out_degrees(net)
 
     legs = [tuple(x.split(":")) for x in args.legs.split(",")]
     results = [run_leg(A, net, p, m, args.plain_steps if (m == "plain" and args.plain_steps > 0) else args.steps,
-                       args.warmup, args.block)
+                       args.warmup, args.block, deg)
                for p, m in legs]
-    head = results[-1]
     workload = f"n{args.n}_seed{args.seed}"
-    spl = head["steps_per_launch"]
-    launch_s = head["kernel_ms_per_launch"] * 1e-3
-    # the contract's algorithmic bytes: the reference layout's per-step
-    # bytes (CSR read once per step) times the steps one launch advances
-    ref_bpl = reference_bytes_per_step(net, head["precision"]) * spl
-    achieved = ref_bpl / launch_s / 1e9
-    own_bpl = kernel_bytes_per_launch(net, head["precision"], head["method"])
-    traffic = load_traffic(f"{head['method']}_{head['precision']}_{workload}")
-    for leg in results:  # the same figures for every leg
-        b = reference_bytes_per_step(net, leg["precision"]) * leg["steps_per_launch"]
-        leg["reference_layout_gbs"] = b / (leg["kernel_ms_per_launch"] * 1e-3) / 1e9
-        leg["hbm_bytes_per_launch"] = load_traffic(f"{leg['method']}_{leg['precision']}_{workload}")
-    # FP32c against FP64 over the same timed window of the same network
-    # (the FP64 blocked leg is bit-identical to the FP64 oracle)
-    accuracy = None
+    for leg in results:
+        leg["roofline"] = add_roofline(net, leg, workload)
+    head = results[-1]
     by = {(r["precision"], r["method"]): r for r in results}
-    a64, a32 = by.get(("fp64", "blocked")) or by.get(("fp64", "plain")), by.get(("fp32c", "blocked"))
-    if a64 and a32 and a64["steps"] == a32["steps"] and a64["spikes_total"] > 0:
-        accuracy = {"rate_rel": a32["spikes_total"] / a64["spikes_total"] - 1.0, "rate_tolerance": 0.01,
-                    "sic_active_diff": a32["sic_active_fraction"] - a64["sic_active_fraction"],
-                    "sic_engaged_diff": a32["sic_engaged_fraction"] - a64["sic_engaged_fraction"],
-                    "sic_tolerance": 0.01, "window_steps": a32["steps"] + args.warmup}
-        log(f"fp32c vs fp64: rate {100 * accuracy['rate_rel']:+.4f}%, SIC active "
-            f"{accuracy['sic_active_diff']:+.5f}, engaged {accuracy['sic_engaged_diff']:+.5f}")
+    ratios = {}
+    if ("fp32c", "blocked") in by:
+        for key in (("fp64", "blocked"), ("fp64", "plain")):
+            if key in by:
+                ratios[f"fp32c_blocked_over_{key[0]}_{key[1]}"] = (by[("fp32c", "blocked")]["steps_per_s"]
+                                                                  / by[key]["steps_per_s"])
+    acc = None if args.no_accuracy else accuracy(A, net)
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: FP64 host oracle")
         cpu = cpu_baseline(net, args.seed, args.cpu_steps)
         log(f"cpu baseline: {cpu['value']:.3f} steps/s on {cpu['cores']} threads")
-    ens = None
-    if not args.no_ensemble:
-        ens = ensemble_leg()
+    net.free()
+    del deg
+    c1 = None if args.no_config1 else config1_leg(A)
+    act = None if args.no_activity else activity_leg(A)
+    ens = None if args.no_ensemble else ensemble_leg()
     srv = serving_leg(args.serving_seconds, args.serving_blocks) if args.serving_seconds > 0 else None
     line = {
         "metric": "steps/s at N=10M FP32 vs FP64 (1 GPU); ensemble sims/s + p50/p99 latency",
@@ -359,28 +534,15 @@ def main() -> int:
                    "precision": head["precision"], "method": head["method"],
                    "parallelism": "single GPU"},
         "legs": results,
+        "speedups": ratios,
+        "synaptic_events_per_s": head["synaptic_events_per_s"],
         "network_setup": {"where": "gpu" if net.built_on_gpu else "host", "build_s": net.build_seconds,
                           "wall_s": setup_s},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic,
-                     "kernel": "integrate_kernel" if head["method"] == "blocked" else "plain_step_kernel",
-                     "algorithmic_bytes_per_launch": ref_bpl,
-                     "algorithmic_bytes_basis": "reference layout, CSR ids read once per step, "
-                                                f"x {spl:g} steps per launch",
-                     "kernel_ms_per_launch": head["kernel_ms_per_launch"],
-                     "steps_per_launch": spl,
-                     "measured_gbs": None if traffic is None else traffic / launch_s / 1e9,
-                     "measured_frac": None if traffic is None else traffic / launch_s / 1e9 / HBM_PEAK_GBS,
-                     "kernel_state_bytes_per_launch": own_bpl,
-                     "note": ("frac > 1: the blocked, event-driven kernel does not stream the CSR every "
-                              "step, so it exceeds the reference layout's HBM ceiling; its own HBM "
-                              "traffic (measured_gbs) is far below peak because it is bound by "
-                              "vector-instruction throughput") if head["method"] == "blocked" else
-                             ("the plain kernel walks ~116 dependent id->value loads per lane and "
-                              "step: latency-bound, far from the HBM ceiling")},
+        "roofline": head["roofline"],
         "cpu_baseline": cpu,
-        "fp32c_vs_fp64": accuracy,
+        "fp32c_vs_fp64": acc,
+        "config1": c1,
+        "activity": act,
         "ensemble": ens,
         "serving": srv,
     }

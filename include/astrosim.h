@@ -131,6 +131,11 @@ as_status as_sim_read_state(as_sim* sim, int field, double* out);
 as_status as_sim_read_refractory(as_sim* sim, int32_t* out);
 /* Per-astrocyte ever-active flags (src/astrosim_fused.cu:1121-1123). */
 as_status as_sim_read_ever_active(as_sim* sim, uint8_t* out);
+/* Spikes of every neuron since creation (n_neurons values; synchronises).
+ * The reference keeps only the total (src/astrosim_fused.cu:584); per-neuron
+ * counts give the recorded window's synaptic events (each spike times its
+ * neuron's out-degree) and per-neuron rates. */
+as_status as_sim_read_spike_counts(as_sim* sim, uint32_t* out);
 /* Spike raster of steps [first_step, first_step + n_steps): n_steps*n_neurons
  * bytes, 1 = spiked. Only the most recent steps are held (at least the
  * minimum synaptic delay); AS_ERR_RANGE otherwise. Test and analysis aid,
@@ -200,6 +205,8 @@ as_status as_batch_reset_ever_active(as_batch* b, int32_t member);
 as_status as_batch_read_state(as_batch* b, int32_t member, int32_t field, double* out);
 as_status as_batch_read_ever_active(as_batch* b, int32_t member, uint8_t* out);
 as_status as_batch_read_raster(as_batch* b, int32_t member, int64_t first_step, int32_t n_steps, uint8_t* out);
+/* Spikes of every neuron of the member since its admission. */
+as_status as_batch_read_spike_counts(as_batch* b, int32_t member, uint32_t* out);
 as_status as_batch_set_timing(as_batch* b, int enable);
 as_status as_batch_kernel_time(as_batch* b, double* total_ms, int64_t* launches);
 void as_batch_destroy(as_batch* b);

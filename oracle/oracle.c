@@ -508,28 +508,13 @@ static inline void neuron_rhs(const nrn_par* p, double V, double w, double g_ex,
     *dw = (p->a * (Vc - p->E_L) - w) / p->tau_w;
 }
 
-/* astro_step_kernel body for astrocyte a (src:374-427) */
-static void astro_cell(orc_sim* s, uint32_t a, int64_t step) {
-    const orc_net* net = s->net;
-    const csr_t* c = &net->csr[3];
-    int read_slot = (int)((step + s->R - s->d_e) % s->R);
-    const double* row = s->ring_neuron + (size_t)read_slot * net->n_neurons;
-    double gathered = 0.0;
-    for (uint32_t k = c->row_start[a]; k < c->row_start[a + 1]; ++k) gathered += row[c->src[k]];
-    double ip3_input = s->w_n2a * gathered;
-    const double rate = 3.029715502945856;
-    if (rate > 0.0) {
-        double lambda = rate * s->dt * 1e-3;
-        int ev = orc_rng_poisson(net->seed ^ 0xC2B2AE3D27D4EB4FULL, (uint64_t)step * 1000003ULL + a, lambda);
-        if (ev > 0) ip3_input += 1.0 * (double)ev;
-    }
-    double noise = 0.0003 * orc_rng_normal(net->seed ^ 0x9E3779B97F4A7C15ULL, (uint64_t)step * 1000003ULL + a);
-    double ca = s->Ca[a], ip3 = s->IP3[a], hh = s->h[a];
-    double Ca_tot = net->Ca_tot[a], IP3_0 = net->IP3_0[a], tau = net->tau_IP3[a];
-    /* astro_advance (src:255-278) */
-    if (ip3_input != 0.0) ip3 += net->delta_IP3[a] * ip3_input;
-    double hs = s->h_step;
-    for (int sub = 0; sub < s->substeps; ++sub) {
+/* astro_advance (src:255-278): IP3 jump, then `substeps` classic RK4 steps
+ * of size hs, each followed by the clamps. */
+static void astro_advance(double Ca_tot, double IP3_0, double tau, double delta_IP3, double ip3_input,
+                          double noise, double hs, int substeps, double* Ca, double* IP3, double* h) {
+    double ca = *Ca, ip3 = *IP3, hh = *h;
+    if (ip3_input != 0.0) ip3 += delta_IP3 * ip3_input;
+    for (int sub = 0; sub < substeps; ++sub) {
         double k1C, k1I, k1h, k2C, k2I, k2h, k3C, k3I, k3h, k4C, k4I, k4h;
         astro_rhs(Ca_tot, IP3_0, tau, ca, ip3, hh, noise, &k1C, &k1I, &k1h);
         astro_rhs(Ca_tot, IP3_0, tau, ca + 0.5 * hs * k1C, ip3 + 0.5 * hs * k1I, hh + 0.5 * hs * k1h,
@@ -545,57 +530,58 @@ static void astro_cell(orc_sim* s, uint32_t a, int64_t step) {
         hh = hh < 0.0 ? 0.0 : (hh > 1.0 ? 1.0 : hh);
         ip3 = ip3 < 0.0 ? 0.0 : ip3;
     }
-    s->Ca[a] = ca; s->IP3[a] = ip3; s->h[a] = hh;
+    *Ca = ca; *IP3 = ip3; *h = hh;
+}
+
+/* SIC factor (src:416-417) */
+static double sic_factor(double ca) {
     double y = (ca - B_SIC_TH) * 1000.0;
-    double factor = (y <= 1.0) ? 0.0 : B_SIC_SCALE * as_log(y);
+    return (y <= 1.0) ? 0.0 : B_SIC_SCALE * as_log(y);
+}
+
+/* A gather: sum of the ring row's values over the CSR row of `target`, in
+ * row order (src:384-390, 450-463, 482-490). */
+static double gather_sum(const csr_t* c, const double* row, uint32_t target) {
+    double sum = 0.0;
+    for (uint32_t k = c->row_start[target]; k < c->row_start[target + 1]; ++k) sum += row[c->src[k]];
+    return sum;
+}
+
+/* astro_step_kernel body for astrocyte a (src:374-427) */
+static void astro_cell(orc_sim* s, uint32_t a, int64_t step) {
+    const orc_net* net = s->net;
+    const csr_t* c = &net->csr[3];
+    int read_slot = (int)((step + s->R - s->d_e) % s->R);
+    const double* row = s->ring_neuron + (size_t)read_slot * net->n_neurons;
+    double gathered = gather_sum(c, row, a);
+    double ip3_input = s->w_n2a * gathered;
+    const double rate = 3.029715502945856;
+    if (rate > 0.0) {
+        double lambda = rate * s->dt * 1e-3;
+        int ev = orc_rng_poisson(net->seed ^ 0xC2B2AE3D27D4EB4FULL, (uint64_t)step * 1000003ULL + a, lambda);
+        if (ev > 0) ip3_input += 1.0 * (double)ev;
+    }
+    double noise = 0.0003 * orc_rng_normal(net->seed ^ 0x9E3779B97F4A7C15ULL, (uint64_t)step * 1000003ULL + a);
+    double ca = s->Ca[a], ip3 = s->IP3[a], hh = s->h[a];
+    astro_advance(net->Ca_tot[a], net->IP3_0[a], net->tau_IP3[a], net->delta_IP3[a], ip3_input, noise, s->h_step,
+                  s->substeps, &ca, &ip3, &hh);
+    s->Ca[a] = ca; s->IP3[a] = ip3; s->h[a] = hh;
+    double factor = sic_factor(ca);
     if (factor > 0.0) s->ever_active[a] = 1;
     s->ring_sic[(size_t)(step % s->R) * net->n_astro + a] = factor;
 }
 
-/* neuron_step_kernel body for neuron j (src:433-587); returns 1 on a spike */
-static int neuron_cell(orc_sim* s, uint32_t j, int64_t step, uint64_t noise_idx, int sic_refresh) {
-    const orc_net* net = s->net;
-    int excitatory = j < net->n_exc;
-    const nrn_par* p = excitatory ? &s->pe : &s->pi;
-    const csr_t* ce = &net->csr[0];
-    const csr_t* ci = &net->csr[1];
-    const csr_t* cs = &net->csr[2];
-    int slot_ex = (int)((step + s->R - s->d_e) % s->R);
-    const double* row_ex = s->ring_neuron + (size_t)slot_ex * net->n_neurons;
-    double exc_sum = 0.0;
-    for (uint32_t k = ce->row_start[j]; k < ce->row_start[j + 1]; ++k) exc_sum += row_ex[ce->src[k]];
-    int slot_in = (int)((step + s->R - s->d_i) % s->R);
-    const double* row_in = s->ring_neuron + (size_t)slot_in * net->n_neurons;
-    double inh_sum = 0.0;
-    /* the stored source id is the local inhibitory index, used as-is (src:461-462) */
-    for (uint32_t k = ci->row_start[j]; k < ci->row_start[j + 1]; ++k) inh_sum += row_in[ci->src[k]];
-    double exc_input = s->w_e * exc_sum;
-    double inh_input = (-s->w_i) * inh_sum;
-    double rate = excitatory ? 2700.0 : 2500.0;
-    if (rate > 0.0) {
-        double lambda = rate * s->dt * 1e-3;
-        int ev = orc_rng_poisson(net->seed ^ 0x2545F4914F6CDD1DULL, (uint64_t)step * 1000003ULL + j, lambda);
-        if (ev > 0) exc_input += 1.0 * (double)ev;
-    }
-    if (sic_refresh) {
-        int slot_sic = (int)((step + s->R - s->d_a2n) % s->R);
-        const double* row_sic = s->ring_sic + (size_t)slot_sic * net->n_astro;
-        double sic_sum = 0.0;
-        for (uint32_t k = cs->row_start[j]; k < cs->row_start[j + 1]; ++k) sic_sum += row_sic[cs->src[k]];
-        s->I_sic_hold[j] = s->w_a2n * sic_sum;
-    }
-    double I_ext = s->I_sic_hold[j];
-    const double noise_std = 100.0;
-    if (noise_std > 0.0) {
-        I_ext += noise_std * orc_rng_normal(net->seed ^ 0x9E3779B185EBCA87ULL, noise_idx * 1000033ULL + (uint64_t)j);
-    }
-    double V = s->V[j], w = s->w[j];
-    double g_ex = s->g_ex[j], dg_ex = s->dg_ex[j] + exc_input * p->psc_ex;
-    double g_in = s->g_in[j], dg_in = s->dg_in[j] + inh_input * p->psc_in;
-    int refractory = s->refr[j];
+/* One neuron's substep loop once its inputs are known (src:508-553): the
+ * exact alpha-cascade propagator, the refractory clamp (Euler on w only),
+ * RK4 on (V, w) with the conductances at t, the midpoint and t+h, spike
+ * detection and reset. st = {V, w, g_ex, dg_ex, g_in, dg_in}. Returns 1 on
+ * a spike. */
+static int neuron_substeps(const nrn_par* p, double* st, int32_t* refr, double I_ext, double V_reset, double b,
+                           double hs, int substeps) {
+    double V = st[0], w = st[1], g_ex = st[2], dg_ex = st[3], g_in = st[4], dg_in = st[5];
+    int refractory = *refr;
     int spiked = 0;
-    double hs = s->h_step;
-    for (int sub = 0; sub < s->substeps; ++sub) {
+    for (int sub = 0; sub < substeps; ++sub) {
         double ex_decay = as_exp(-hs / p->tau_syn_ex);
         double in_decay = as_exp(-hs / p->tau_syn_in);
         double g_ex_next = ex_decay * (g_ex + hs * dg_ex);
@@ -603,7 +589,7 @@ static int neuron_cell(orc_sim* s, uint32_t j, int64_t step, uint64_t noise_idx,
         double g_in_next = in_decay * (g_in + hs * dg_in);
         double dg_in_next = in_decay * dg_in;
         if (refractory > 0) {
-            V = net->V_reset[j];
+            V = V_reset;
             double dV, dw;
             neuron_rhs(p, V, w, g_ex, g_in, I_ext, &dV, &dw);
             w += hs * dw;
@@ -623,29 +609,77 @@ static int neuron_cell(orc_sim* s, uint32_t j, int64_t step, uint64_t noise_idx,
         if (refractory > 0) {
             --refractory;
         } else if (V >= p->V_peak) {
-            V = net->V_reset[j];
-            w += net->b[j];
+            V = V_reset;
+            w += b;
             refractory = (int)(p->t_ref / hs + 0.5);
             spiked = 1;
         }
     }
-    s->V[j] = V; s->w[j] = w;
-    s->g_ex[j] = g_ex; s->dg_ex[j] = dg_ex;
-    s->g_in[j] = g_in; s->dg_in[j] = dg_in;
+    st[0] = V; st[1] = w; st[2] = g_ex; st[3] = dg_ex; st[4] = g_in; st[5] = dg_in;
+    *refr = refractory;
+    return spiked;
+}
+
+/* Tsodyks-Markram update of a presynaptic neuron on a spike at t_now
+ * (src:565-576); returns the release multiplier x*u. */
+static double stp_on_spike(double* x, double* u, double* t_last, double t_now, double U, double tau_rec,
+                           double tau_fac) {
+    double elapsed = t_now - *t_last;
+    double x_decay = as_exp(-elapsed / tau_rec);
+    double u_decay = tau_fac < 1e-10 ? 0.0 : as_exp(-elapsed / tau_fac);
+    *x = 1.0 + (*x - *x * *u - 1.0) * x_decay;
+    *u = U + *u * (1.0 - U) * u_decay;
+    *t_last = t_now;
+    return *x * *u;
+}
+
+/* neuron_step_kernel body for neuron j (src:433-587); returns 1 on a spike */
+static int neuron_cell(orc_sim* s, uint32_t j, int64_t step, uint64_t noise_idx, int sic_refresh) {
+    const orc_net* net = s->net;
+    int excitatory = j < net->n_exc;
+    const nrn_par* p = excitatory ? &s->pe : &s->pi;
+    const csr_t* ce = &net->csr[0];
+    const csr_t* ci = &net->csr[1];
+    const csr_t* cs = &net->csr[2];
+    int slot_ex = (int)((step + s->R - s->d_e) % s->R);
+    const double* row_ex = s->ring_neuron + (size_t)slot_ex * net->n_neurons;
+    double exc_sum = gather_sum(ce, row_ex, j);
+    int slot_in = (int)((step + s->R - s->d_i) % s->R);
+    const double* row_in = s->ring_neuron + (size_t)slot_in * net->n_neurons;
+    /* the stored source id is the local inhibitory index, used as-is (src:461-462) */
+    double inh_sum = gather_sum(ci, row_in, j);
+    double exc_input = s->w_e * exc_sum;
+    double inh_input = (-s->w_i) * inh_sum;
+    double rate = excitatory ? 2700.0 : 2500.0;
+    if (rate > 0.0) {
+        double lambda = rate * s->dt * 1e-3;
+        int ev = orc_rng_poisson(net->seed ^ 0x2545F4914F6CDD1DULL, (uint64_t)step * 1000003ULL + j, lambda);
+        if (ev > 0) exc_input += 1.0 * (double)ev;
+    }
+    if (sic_refresh) {
+        int slot_sic = (int)((step + s->R - s->d_a2n) % s->R);
+        const double* row_sic = s->ring_sic + (size_t)slot_sic * net->n_astro;
+        s->I_sic_hold[j] = s->w_a2n * gather_sum(cs, row_sic, j);
+    }
+    double I_ext = s->I_sic_hold[j];
+    const double noise_std = 100.0;
+    if (noise_std > 0.0) {
+        I_ext += noise_std * orc_rng_normal(net->seed ^ 0x9E3779B185EBCA87ULL, noise_idx * 1000033ULL + (uint64_t)j);
+    }
+    double st[6] = {s->V[j], s->w[j], s->g_ex[j], s->dg_ex[j] + exc_input * p->psc_ex,
+                    s->g_in[j], s->dg_in[j] + inh_input * p->psc_in};
+    int32_t refractory = s->refr[j];
+    int spiked = neuron_substeps(p, st, &refractory, I_ext, net->V_reset[j], net->b[j], s->h_step, s->substeps);
+    s->V[j] = st[0]; s->w[j] = st[1];
+    s->g_ex[j] = st[2]; s->dg_ex[j] = st[3];
+    s->g_in[j] = st[4]; s->dg_in[j] = st[5];
     s->refr[j] = refractory;
     double out = 0.0;
     if (spiked) {
-        double x = s->stp_x[j], u = s->stp_u[j];
         if (s->stp_enabled) {
-            double t_now = (double)step * s->dt;
-            double elapsed = t_now - s->stp_t_last[j];
             const double U = 0.5, tau_rec = 800.0, tau_fac = 0.0; /* src:768 */
-            double x_decay = as_exp(-elapsed / tau_rec);
-            double u_decay = tau_fac < 1e-10 ? 0.0 : as_exp(-elapsed / tau_fac);
-            x = 1.0 + (x - x * u - 1.0) * x_decay;
-            u = U + u * (1.0 - U) * u_decay;
-            s->stp_x[j] = x; s->stp_u[j] = u; s->stp_t_last[j] = t_now;
-            out = x * u;
+            out = stp_on_spike(&s->stp_x[j], &s->stp_u[j], &s->stp_t_last[j], (double)step * s->dt, U, tau_rec,
+                               tau_fac);
         } else {
             out = 1.0;
         }
@@ -704,4 +738,55 @@ void orc_sim_read_refractory(const orc_sim* s, int32_t* out) {
 }
 void orc_sim_read_ever_active(const orc_sim* s, uint8_t* out) {
     memcpy(out, s->ever_active, s->net->n_astro);
+}
+
+/* ======================================================================
+ * Per-cell entry points for the known-answer tests of the step arithmetic
+ * (the same functions the stepper above calls).
+ * ====================================================================== */
+void orc_astro_rhs(double Ca_tot, double IP3_0, double tau_IP3, double Ca, double IP3, double h, double noise,
+                   double out[3]) {
+    astro_rhs(Ca_tot, IP3_0, tau_IP3, Ca, IP3, h, noise, &out[0], &out[1], &out[2]);
+}
+
+void orc_astro_advance(double Ca_tot, double IP3_0, double tau_IP3, double delta_IP3, double ip3_input,
+                       double noise, double h_step, int substeps, double state[3]) {
+    astro_advance(Ca_tot, IP3_0, tau_IP3, delta_IP3, ip3_input, noise, h_step, substeps, &state[0], &state[1],
+                  &state[2]);
+}
+
+double orc_sic_factor(double Ca) { return sic_factor(Ca); }
+
+void orc_neuron_rhs(int excitatory, double V, double w, double g_ex, double g_in, double I_ext, double out[2]) {
+    nrn_par p = excitatory ? exc_par() : inh_par();
+    neuron_rhs(&p, V, w, g_ex, g_in, I_ext, &out[0], &out[1]);
+}
+
+int orc_neuron_substeps(int excitatory, double state[6], int32_t* refractory, double I_ext, double V_reset,
+                        double b, double h_step, int substeps) {
+    nrn_par p = excitatory ? exc_par() : inh_par();
+    return neuron_substeps(&p, state, refractory, I_ext, V_reset, b, h_step, substeps);
+}
+
+double orc_psc_init(int excitatory, int inhibitory_synapse) {
+    nrn_par p = excitatory ? exc_par() : inh_par();
+    return inhibitory_synapse ? p.psc_in : p.psc_ex;
+}
+
+double orc_stp_on_spike(double state[3], double t_now) {
+    return stp_on_spike(&state[0], &state[1], &state[2], t_now, 0.5, 800.0, 0.0);
+}
+
+double orc_gather(const uint32_t* row_start, const uint32_t* src, uint32_t n_targets, const double* ring_row,
+                  uint32_t target) {
+    csr_t c;
+    c.row_start = (uint32_t*)row_start;
+    c.src = (uint32_t*)src;
+    c.n_targets = n_targets;
+    c.n_edges = row_start[n_targets];
+    return gather_sum(&c, ring_row, target);
+}
+
+void orc_sim_params(const orc_sim* s, int32_t out[5]) {
+    out[0] = s->d_e; out[1] = s->d_i; out[2] = s->d_a2n; out[3] = s->R; out[4] = s->steps_per_noise;
 }

@@ -21,8 +21,12 @@
  * uniform identity, Knuth-Poisson brute force and moments, Box-Muller pair
  * identities, scaling-law closed forms, connectivity invariants (no
  * autapses, row order, pool and out-degree caps, in-degree statistics),
- * the alpha-cascade closed form, the Li-Rinzel fixed point and RK4 order,
- * the STP recursion closed form, serial == threaded bitwise.
+ * serial == threaded bitwise; and, for the step arithmetic
+ * (tests/test_oracle_physics.py), through the per-cell entry points below:
+ * Li-Rinzel values at degenerate points and its fixed point, RK4's fourth
+ * order, the clamps, the AdEx values at threshold, at rest and above the
+ * peak, the alpha-cascade peak, the refractory clamp, the STP first spike
+ * and recursion, and zero-term invariance of the gathers.
  */
 #ifndef ASTROSIM_ORACLE_H
 #define ASTROSIM_ORACLE_H
@@ -89,6 +93,34 @@ void orc_sim_reset_ever_active(orc_sim* sim);
 void orc_sim_read(const orc_sim* sim, int field, double* out);
 void orc_sim_read_refractory(const orc_sim* sim, int32_t* out);
 void orc_sim_read_ever_active(const orc_sim* sim, uint8_t* out);
+
+/* ---- per-cell step arithmetic, exported for known-answer tests ---- */
+/* Li-Rinzel right-hand side (src:236-253): out = {dCa, dIP3, dh}. */
+void orc_astro_rhs(double Ca_tot, double IP3_0, double tau_IP3, double Ca, double IP3, double h, double noise,
+                   double out[3]);
+/* IP3 jump, `substeps` RK4 steps of h_step and clamps (src:255-278);
+ * state = {Ca, IP3, h}, updated in place. */
+void orc_astro_advance(double Ca_tot, double IP3_0, double tau_IP3, double delta_IP3, double ip3_input,
+                       double noise, double h_step, int substeps, double state[3]);
+/* SIC factor of a calcium level (src:416-417). */
+double orc_sic_factor(double Ca);
+/* AdEx right-hand side of the excitatory or inhibitory population
+ * (src:281-290): out = {dV, dw}. */
+void orc_neuron_rhs(int excitatory, double V, double w, double g_ex, double g_in, double I_ext, double out[2]);
+/* Substep loop (src:508-553); state = {V, w, g_ex, dg_ex, g_in, dg_in}
+ * updated in place. Returns 1 on a spike. */
+int orc_neuron_substeps(int excitatory, double state[6], int32_t* refractory, double I_ext, double V_reset,
+                        double b, double h_step, int substeps);
+/* e / tau_syn of the population's excitatory (0) or inhibitory (1) synapses (src:944-951). */
+double orc_psc_init(int excitatory, int inhibitory_synapse);
+/* STP on a spike at t_now (src:565-576); state = {x, u, t_last} in place;
+ * returns x*u. */
+double orc_stp_on_spike(double state[3], double t_now);
+/* Sum of ring_row over the CSR row of `target`, in row order. */
+double orc_gather(const uint32_t* row_start, const uint32_t* src, uint32_t n_targets, const double* ring_row,
+                  uint32_t target);
+/* {d_e, d_i, d_a2n, ring depth R, steps_per_noise} of a simulation (src:954-957, 1043). */
+void orc_sim_params(const orc_sim* sim, int32_t out[5]);
 
 #ifdef __cplusplus
 }

@@ -56,6 +56,16 @@ def load():
         "orc_sim_read": (None, [C.c_void_p, C.c_int, C.c_void_p]),
         "orc_sim_read_refractory": (None, [C.c_void_p, C.c_void_p]),
         "orc_sim_read_ever_active": (None, [C.c_void_p, C.c_void_p]),
+        "orc_astro_rhs": (None, [C.c_double] * 7 + [C.c_void_p]),
+        "orc_astro_advance": (None, [C.c_double] * 7 + [C.c_int, C.c_void_p]),
+        "orc_sic_factor": (C.c_double, [C.c_double]),
+        "orc_neuron_rhs": (None, [C.c_int] + [C.c_double] * 5 + [C.c_void_p]),
+        "orc_neuron_substeps": (C.c_int, [C.c_int, C.c_void_p, C.POINTER(C.c_int32), C.c_double, C.c_double,
+                                          C.c_double, C.c_double, C.c_int]),
+        "orc_psc_init": (C.c_double, [C.c_int, C.c_int]),
+        "orc_stp_on_spike": (C.c_double, [C.c_void_p, C.c_double]),
+        "orc_gather": (C.c_double, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]),
+        "orc_sim_params": (None, [C.c_void_p, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -203,3 +213,62 @@ def hostrng_normals(seed, stream, n):
     out = np.empty(n, dtype=np.float64)
     load().orc_hostrng_normals(seed, stream, n, _p(out))
     return out
+
+
+# ---- per-cell step arithmetic (known-answer tests) ----
+
+def astro_rhs(Ca_tot, IP3_0, tau_IP3, Ca, IP3, h, noise=0.0):
+    """{dCa, dIP3, dh} of the Li-Rinzel right-hand side."""
+    out = np.empty(3, dtype=np.float64)
+    load().orc_astro_rhs(Ca_tot, IP3_0, tau_IP3, Ca, IP3, h, noise, _p(out))
+    return out
+
+
+def astro_advance(Ca_tot, IP3_0, tau_IP3, delta_IP3, state, ip3_input=0.0, noise=0.0, h_step=0.1, substeps=1):
+    """(Ca, IP3, h) after the IP3 jump, `substeps` RK4 steps and the clamps."""
+    st = np.array(state, dtype=np.float64)
+    load().orc_astro_advance(Ca_tot, IP3_0, tau_IP3, delta_IP3, ip3_input, noise, h_step, substeps, _p(st))
+    return st
+
+
+def sic_factor(ca):
+    return load().orc_sic_factor(ca)
+
+
+def neuron_rhs(excitatory, V, w, g_ex=0.0, g_in=0.0, I_ext=0.0):
+    """{dV, dw} of the AdEx right-hand side of the population."""
+    out = np.empty(2, dtype=np.float64)
+    load().orc_neuron_rhs(1 if excitatory else 0, V, w, g_ex, g_in, I_ext, _p(out))
+    return out
+
+
+def neuron_substeps(excitatory, state, refractory=0, I_ext=0.0, V_reset=-50.0, b=300.0, h_step=0.1, substeps=1):
+    """(state {V, w, g_ex, dg_ex, g_in, dg_in}, refractory, spiked) after the substep loop."""
+    st = np.array(state, dtype=np.float64)
+    r = C.c_int32(refractory)
+    sp = load().orc_neuron_substeps(1 if excitatory else 0, _p(st), C.byref(r), I_ext, V_reset, b, h_step, substeps)
+    return st, r.value, bool(sp)
+
+
+def psc_init(excitatory, inhibitory_synapse):
+    return load().orc_psc_init(1 if excitatory else 0, 1 if inhibitory_synapse else 0)
+
+
+def stp_on_spike(state, t_now):
+    """(release x*u, new {x, u, t_last})."""
+    st = np.array(state, dtype=np.float64)
+    out = load().orc_stp_on_spike(_p(st), t_now)
+    return out, st
+
+
+def gather(row_start, src, ring_row, target):
+    rs = np.ascontiguousarray(row_start, dtype=np.uint32)
+    sr = np.ascontiguousarray(src if len(src) else [0], dtype=np.uint32)
+    ring = np.ascontiguousarray(ring_row, dtype=np.float64)
+    return load().orc_gather(_p(rs), _p(sr), len(rs) - 1, _p(ring), target)
+
+
+def sim_params(sim: "OracleSim") -> dict:
+    out = np.empty(5, dtype=np.int32)
+    load().orc_sim_params(sim._h, _p(out))
+    return dict(zip(("d_e", "d_i", "d_a2n", "R", "steps_per_noise"), (int(x) for x in out)))

@@ -41,3 +41,18 @@ def test_blocked_kernel_state_bytes():
     assert bench.kernel_bytes_per_launch(net, "fp32c", "blocked") == 1000 * (16 * 4 + 36) + 200 * (16 * 4 + 13)
     # the plain kernel's algorithmic bytes are the reference layout's
     assert bench.kernel_bytes_per_launch(net, "fp64", "plain") == bench.reference_bytes_per_step(net, "fp64")
+
+
+def test_roofline_uses_the_kernels_own_bytes_and_full_launch_time():
+    n, na = 10_000_000, 2_000_000
+    net = fake_net(n, na, {"exc": 80 * n, "inh": 20 * n, "e2a": 16 * n, "sic": 16 * n})
+    leg = {"precision": "fp32c", "method": "blocked", "kernel_ms_per_full_launch": 1.0}
+    r = bench.add_roofline(net, leg, "nonexistent_workload")
+    own = n * (16 * 4 + 36) + na * (16 * 4 + 13)
+    assert r["algorithmic_bytes_per_launch"] == own
+    assert abs(r["achieved"] - own / 1e-3 / 1e9) < 1e-6
+    assert abs(r["frac"] - r["achieved"] / 8000.0) < 1e-12
+    assert r["frac"] < 1.0 and r["traffic"] is None
+    plain = bench.add_roofline(net, {"precision": "fp64", "method": "plain", "kernel_ms_per_full_launch": 1.4},
+                               "nonexistent_workload")
+    assert plain["algorithmic_bytes_per_launch"] == bench.reference_bytes_per_step(net, "fp64")

@@ -30,6 +30,7 @@ def run_pair(n, seed, steps, chunk=10, method="plain", oracle_kw=None, **kw):
     okw = {k: kw[k] for k in ("dt", "substeps", "w_n2a", "w_a2n", "stp") if k in kw}
     osim = O.OracleSim(onet, threads=0, **okw)
     done = 0
+    per_neuron = np.zeros(net.n_neurons, dtype=np.int64)
     while done < steps:
         k = min(chunk, steps - done)
         sim.step(k)
@@ -38,8 +39,11 @@ def run_pair(n, seed, steps, chunk=10, method="plain", oracle_kw=None, **kw):
         if not np.array_equal(got, want):
             s, j = np.argwhere(got != want)[0]
             pytest.fail(f"raster differs at step {done + s} neuron {j}: gpu={got[s, j]} oracle={want[s, j]}")
+        per_neuron += want.sum(axis=0)
         done += k
     assert_state_bitwise(sim, osim, f"after {steps} steps")
+    # per-neuron spike counts: the oracle's raster summed over every step
+    assert np.array_equal(sim.spike_counts().astype(np.int64), per_neuron)
     return sim, osim
 
 
