@@ -80,7 +80,7 @@ class _BatchInfo(C.Structure):
 class _ServerConfig(C.Structure):
     _fields_ = [("dt", C.c_double), ("block", C.c_int32), ("hbm_fraction", C.c_double),
                 ("fp64_share", C.c_double), ("build_threads", C.c_int32), ("blocks_per_round", C.c_int32),
-                ("max_members", C.c_int32), ("setup_where", C.c_int32)]
+                ("max_members", C.c_int32), ("setup_where", C.c_int32), ("fail_after_rounds", C.c_int32)]
 
 
 class _Request(C.Structure):

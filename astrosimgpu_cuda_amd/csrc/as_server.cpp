@@ -126,6 +126,8 @@ public:
         if (r.n_steps < 0) throw Error(AS_ERR_INVALID, "n_steps must be >= 0");
         if (r.precision != AS_FP32C && r.precision != AS_FP64) throw Error(AS_ERR_INVALID, "unknown precision");
         std::lock_guard<std::mutex> g(mu_);
+        if (!error_.empty()) throw Error(AS_ERR_HIP, "server stopped after an error: " + error_);
+        if (stop_) throw Error(AS_ERR_INVALID, "server is stopping");
         auto q = std::make_shared<Req>();
         q->in = r;
         q->out.id = static_cast<std::int64_t>(reqs_.size());
@@ -144,13 +146,19 @@ public:
         return reqs_[id]->out;
     }
 
+    // Waits for every request to finish; an error of the driver ends the
+    // wait at once and is returned (every unfinished request was rejected).
     bool wait_all(double timeout_s) {
         std::unique_lock<std::mutex> g(mu_);
-        return cv_done_.wait_for(g, std::chrono::duration<double>(timeout_s), [&] { return finished_ == reqs_.size(); });
+        const bool done = cv_done_.wait_for(g, std::chrono::duration<double>(timeout_s),
+                                            [&] { return finished_ == reqs_.size() || !error_.empty(); });
+        if (!error_.empty()) throw Error(AS_ERR_HIP, "server stopped after an error: " + error_);
+        return done;
     }
 
     as_server_stats stats() {
         std::lock_guard<std::mutex> g(mu_);
+        if (!error_.empty()) throw Error(AS_ERR_HIP, "server stopped after an error: " + error_);
         as_server_stats s{};
         s.submitted = reqs_.size();
         std::vector<double> lat;
@@ -219,6 +227,8 @@ private:
                 ok = false;
             }
             std::lock_guard<std::mutex> g(mu_);
+            // the driver rejected it meanwhile (it stopped after an error)
+            if (q->out.status != AS_REQ_BUILDING) continue;
             q->out.n_neurons = ok ? net->cfg.n_neurons() : 0;
             q->out.n_astro = ok ? net->cfg.n_astro : 0;
             q->out.t_built = now();
@@ -381,6 +391,8 @@ private:
                     }
                     continue;
                 }
+                if (cfg_.fail_after_rounds > 0 && ++rounds_done_ >= cfg_.fail_after_rounds)
+                    throw Error(AS_ERR_HIP, "injected failure after " + std::to_string(rounds_done_) + " rounds");
                 // complete round `prev`; the one just enqueued keeps the device busy
                 const Round& P = rounds_[prev];
                 float round_ms = 0.0f;
@@ -406,16 +418,22 @@ private:
                 }
             }
         } catch (const std::exception& e) {
+            // let the device finish what was enqueued before stopping
+            for (int b = 0; b < 2; ++b) (void)hipStreamSynchronize(stream_[b]);
             std::lock_guard<std::mutex> g(mu_);
             error_ = e.what();
+            if (error_.empty()) error_ = "unknown error";
             stop_ = true;
-            // fail every unfinished request so waiters return
+            // fail every unfinished request so waiters return; builders skip
+            // the requests they were building (their status is final now)
             for (auto& r : reqs_)
                 if (r->out.status != AS_REQ_DONE && r->out.status != AS_REQ_REJECTED) {
                     r->out.status = AS_REQ_REJECTED;
                     ++finished_;
                 }
+            build_q_.clear();
             cv_done_.notify_all();
+            cv_build_.notify_all();
         }
     }
 
@@ -448,6 +466,7 @@ private:
     std::vector<std::thread> builders_;
     std::thread driver_;
     std::uint64_t peak_active_ = 0, peak_cells_ = 0;
+    int rounds_done_ = 0;
     double gpu_busy_s_ = 0.0, cell_steps_ = 0.0, admit_s_ = 0.0, retire_s_ = 0.0;
 };
 
@@ -486,6 +505,7 @@ void as_server_config_default(as_server_config* c) {
     c->blocks_per_round = 16;
     c->max_members = 512;
     c->setup_where = AS_BUILD_GPU;
+    c->fail_after_rounds = 0;
 }
 
 as_status as_server_create(const as_server_config* cfg, as_server** out) {

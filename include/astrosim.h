@@ -235,6 +235,8 @@ typedef struct as_server_config {
                                  between rounds */
     int32_t max_members;      /* member slots per batch */
     int32_t setup_where;      /* as_build_where of the request networks (default AS_BUILD_GPU) */
+    int32_t fail_after_rounds;/* test hook: the driver stops with an error after this many
+                                 rounds (0, the default: never) */
 } as_server_config;
 typedef struct as_request {
     uint32_t n_total;   /* --n */
@@ -270,7 +272,10 @@ as_status as_server_create(const as_server_config* cfg, as_server** out);
 as_status as_server_submit(as_server* s, const as_request* req, int64_t* id);
 as_status as_server_result(as_server* s, int64_t id, as_request_result* out);
 /* Waits until every submitted request is done or rejected (*all_done = 1) or
- * the timeout passes (*all_done = 0). */
+ * the timeout passes (*all_done = 0). If the driver stopped after an error
+ * (a HIP failure), every unfinished request is rejected, the wait ends at
+ * once and the error is returned (AS_ERR_HIP, message in as_last_error);
+ * as_server_submit and as_server_get_stats return it too from then on. */
 as_status as_server_wait_all(as_server* s, double timeout_s, int32_t* all_done);
 as_status as_server_get_stats(as_server* s, as_server_stats* out);
 void as_server_destroy(as_server* s);

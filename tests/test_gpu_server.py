@@ -1,6 +1,8 @@
 """The continuous-batching server: requests arriving while others run are
 admitted as capacity frees up, and each result is bit-identical to running
 the same network alone (FP32c and FP64)."""
+import time
+
 import numpy as np
 import pytest
 
@@ -48,3 +50,21 @@ def test_server_results_equal_solo_runs(blocks, n_hi, hbm, seed):
         assert res["mean_ca"] == pytest.approx(mean_ca, rel=1e-12), (req, res)
         assert res["active_astro"] == active, (req, res)
     assert st["peak_active"] >= 2  # requests overlapped on the device
+
+
+def test_driver_failure_ends_the_wait_with_the_error():
+    # the fail_after_rounds hook makes the driver raise after 3 rounds (no
+    # kernel is involved: the error is thrown on the host between rounds)
+    with A.Server(hbm_fraction=0.01, fp64_share=0.3, build_threads=2, blocks_per_round=2,
+                  fail_after_rounds=3) as srv:
+        ids = [srv.submit(2000 + 100 * i, 50 + i, 3000, "fp32c") for i in range(6)]
+        t0 = time.perf_counter()
+        with pytest.raises(A.AstrosimError, match="injected failure"):
+            srv.wait_all(120.0)
+        assert time.perf_counter() - t0 < 60.0
+        statuses = {srv.result(i)["status"] for i in ids}
+        assert statuses <= {"done", "rejected"} and "rejected" in statuses
+        with pytest.raises(A.AstrosimError, match="injected failure"):
+            srv.submit(2000, 1, 100, "fp32c")
+        with pytest.raises(A.AstrosimError):
+            srv.stats()
