@@ -126,6 +126,8 @@ SIGNATURES = [
     ("as_sim_read_ever_active", C.c_int, [C.c_void_p, C.c_void_p]),
     ("as_sim_read_spike_counts", C.c_int, [C.c_void_p, C.c_void_p]),
     ("as_sim_read_raster", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
+    ("as_sim_read_spikes", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64,
+                                     C.POINTER(C.c_uint64)]),
     ("as_sim_bytes", C.c_uint64, [C.c_void_p]),
     ("as_sim_set_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("as_sim_kernel_time", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
@@ -365,6 +367,24 @@ class Simulation:
         out = np.empty((n_steps, self.n_neurons), dtype=np.uint8)
         _check(load().as_sim_read_raster(self.handle, C.c_int64(first_step), C.c_int32(n_steps), _ptr(out)))
         return out
+
+    def spikes(self, first_step: int, n_steps: int, cap: int | None = None) -> tuple[np.ndarray, np.ndarray]:
+        """(steps, neurons) of the spikes of held steps [first_step, first_step
+        + n_steps), in step then neuron order. With cap=None the buffers grow
+        until every event fits."""
+        fn = load().as_sim_read_spikes
+        want = 4096 if cap is None else cap
+        while True:
+            steps = np.empty(max(1, want), dtype=np.int64)
+            neurons = np.empty(max(1, want), dtype=np.uint32)
+            count = C.c_uint64()
+            _check(fn(self.handle, C.c_int64(first_step), C.c_int32(n_steps), _ptr(steps), _ptr(neurons),
+                      C.c_uint64(want), C.byref(count)))
+            n = count.value
+            if n <= want or cap is not None:
+                k = min(n, want)
+                return steps[:k].copy(), neurons[:k].copy()
+            want = n
 
     def device_bytes(self) -> int:
         return load().as_sim_bytes(self.handle)

@@ -47,7 +47,7 @@ DEVICE = ["-x", "hip", f"--offload-arch={ARCH}",
 HOST = ["-D__HIP_PLATFORM_AMD__", "-mfma", "-I", str(ROCM / "include")]
 
 LIB_SOURCES = ["as_setup.cpp", "as_engine.cpp", "as_capi.cpp", "as_plain.hip", "as_batch.hip", "as_server.cpp",
-               "as_gpubuild.hip"]
+               "as_gpubuild.hip", "as_record.hip"]
 HEADERS = ["as_dmath.h", "as_rng.h", "as_model.h", "as_model32.h", "as_setup.h", "as_engine.h", "as_batch.h",
            "as_gpubuild.h", "as_gpubuild_run.h"]
 

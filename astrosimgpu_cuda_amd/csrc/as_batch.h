@@ -63,6 +63,7 @@ public:
     virtual void read_ever_active(int member, std::uint8_t* out) = 0;
     virtual void read_spike_counts(int member, std::uint32_t* out) = 0;  // spikes per neuron
     virtual void read_raster(int member, std::int64_t first_local, int n, std::uint8_t* out) = 0;
+    virtual SpikeWords spike_words(int member, std::int64_t first_local, int n) = 0;
     virtual std::int64_t member_steps(int member) const = 0;  // local steps enqueued
     // Local steps the member has run once the device reaches global step g.
     virtual std::int64_t member_steps_at(int member, std::int64_t g) const = 0;

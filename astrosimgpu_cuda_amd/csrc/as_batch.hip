@@ -1475,6 +1475,27 @@ public:
         AS_HIP_CHECK(hipStreamSynchronize(stream_));
     }
 
+    SpikeWords spike_words(int member, std::int64_t first, int n) override {
+        const MemberHost& h = host(member);
+        const std::int64_t done = member_steps(member);
+        const std::int64_t cur_block = step_ > 0 ? (step_ - 1) / geo_.L : 0;
+        const std::int64_t g0 = h.m.step0 + first;
+        if (n < 0 || n > kRecordMaxSteps || first < 0 || first + n > done ||
+            (n > 0 && g0 / geo_.L <= cur_block - geo_.H))
+            throw Error(AS_ERR_RANGE, "spike steps not held (the blocked method keeps the last H blocks)");
+        SpikeWords w;
+        w.kind = 1;
+        w.base = D_.spk_w + h.m.nb;
+        w.n_neurons = h.m.nn;
+        w.n_steps = n;
+        for (int i = 0; i < n; ++i) {
+            const std::int64_t s = g0 + i, bl = s / geo_.L;
+            w.off[i] = std::uint64_t(bl % geo_.H) * D_.ncap;
+            w.bit[i] = static_cast<std::uint32_t>(s - bl * geo_.L);
+        }
+        return w;
+    }
+
     void read_raster(int member, std::int64_t first, int n, std::uint8_t* out) override {
         const MemberHost& h = host(member);
         const std::int64_t done = member_steps(member);
@@ -1949,6 +1970,7 @@ public:
     void read_raster(std::int64_t first, int n, std::uint8_t* out) override {
         batch_->read_raster(member_, first, n, out);
     }
+    SpikeWords spike_words(std::int64_t first, int n) override { return batch_->spike_words(member_, first, n); }
     std::uint64_t device_bytes() const override { return batch_->device_bytes(); }
     void set_timing(bool on) override { batch_->set_timing(on); }
     void kernel_time(double* total_ms, std::int64_t* launches) override { batch_->kernel_time(total_ms, launches); }

@@ -258,6 +258,14 @@ as_status as_sim_read_spike_counts(as_sim* sim, uint32_t* out) {
     });
 }
 
+as_status as_sim_read_spikes(as_sim* sim, int64_t first_step, int32_t n_steps, int64_t* steps, uint32_t* neurons,
+                             uint64_t cap, uint64_t* count) {
+    return guarded([&] {
+        require(sim && count, "NULL argument");
+        astrosim::read_spikes(*sim->eng, first_step, n_steps, steps, neurons, cap, count);
+    });
+}
+
 as_status as_sim_read_raster(as_sim* sim, int64_t first_step, int32_t n_steps, uint8_t* out) {
     return guarded([&] {
         require(sim && out, "NULL argument");

@@ -70,6 +70,20 @@ struct Layout {
     }
 };
 
+// Where the spike flags of a window of held steps live on the device, for
+// the recorder (as_record.hip). kind 0: one 64-bit word per 64 neurons and
+// step (bit j & 63 of word j >> 6 at base + off[i]); kind 1: one 32-bit word
+// per neuron and block (bit bit[i] of word j at base + off[i]).
+constexpr int kRecordMaxSteps = 64;
+struct SpikeWords {
+    int kind = 0;
+    const void* base = nullptr;
+    std::uint32_t n_neurons = 0;
+    int n_steps = 0;
+    std::uint64_t off[kRecordMaxSteps] = {};
+    std::uint32_t bit[kRecordMaxSteps] = {};
+};
+
 // One simulation on one HIP stream.
 class Engine {
 public:
@@ -82,6 +96,9 @@ public:
     virtual void read_ever_active(std::uint8_t* out) = 0;
     virtual void read_spike_counts(std::uint32_t* out) = 0;  // spikes per neuron since creation
     virtual void read_raster(std::int64_t first, int n, std::uint8_t* out) = 0;
+    // Spike words of held steps [first, first + n), n <= kRecordMaxSteps
+    // (AS_ERR_RANGE when not held).
+    virtual SpikeWords spike_words(std::int64_t first, int n) = 0;
     virtual std::uint64_t device_bytes() const = 0;
 
     std::int64_t steps() const { return step_; }
@@ -107,6 +124,12 @@ protected:
     hipEvent_t pending_start_ = nullptr;
     std::int64_t timed_launches_ = 0;
 };
+
+// The spikes of steps [first, first + n_steps) as (step, neuron) events in
+// step, then neuron order: the first min(total, cap) into steps/neurons,
+// the total into *count. Synchronises the engine's stream.
+void read_spikes(Engine& eng, std::int64_t first, std::int64_t n_steps, std::int64_t* steps, std::uint32_t* neurons,
+                 std::uint64_t cap, std::uint64_t* count);
 
 std::uint64_t plain_footprint(const Network& net, const RunParams& rp);
 std::unique_ptr<Engine> make_plain_engine(const Network& net, const RunParams& rp, hipStream_t s);

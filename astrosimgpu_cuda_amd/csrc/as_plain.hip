@@ -853,6 +853,19 @@ public:
         }
     }
 
+    SpikeWords spike_words(std::int64_t first, int n) override {
+        const int R = P_.ring_slots;
+        if (n < 0 || n > kRecordMaxSteps || first < 0 || first + n > step_ || first < step_ - R)
+            throw Error(AS_ERR_RANGE, "spike steps not held (the plain method keeps the last R steps)");
+        SpikeWords w;
+        w.kind = 0;
+        w.base = D_.flag_n;
+        w.n_neurons = n_neurons_;
+        w.n_steps = n;
+        for (int i = 0; i < n; ++i) w.off[i] = std::uint64_t((first + i) % R) * lay_.wn;
+        return w;
+    }
+
     std::uint64_t device_bytes() const override { return lay_.total; }
 
 private:

@@ -141,6 +141,16 @@ as_status as_sim_read_spike_counts(as_sim* sim, uint32_t* out);
  * minimum synaptic delay); AS_ERR_RANGE otherwise. Test and analysis aid,
  * the reference has no counterpart. */
 as_status as_sim_read_raster(as_sim* sim, int64_t first_step, int32_t n_steps, uint8_t* out);
+/* Spike events of steps [first_step, first_step + n_steps) -- steps the
+ * simulation still holds, as for as_sim_read_raster (AS_ERR_RANGE otherwise)
+ * -- in step order and, within a step, neuron order: the first
+ * min(total, cap) events go to steps[i] / neurons[i], and *count receives
+ * the total (a caller that gets *count > cap calls again with more room).
+ * The events are compacted on the device, in staging this call allocates and
+ * frees (hipMalloc); synchronises. The reference keeps only a spike total
+ * (src/astrosim_fused.cu:584); this is the recorder's read. */
+as_status as_sim_read_spikes(as_sim* sim, int64_t first_step, int32_t n_steps, int64_t* steps, uint32_t* neurons,
+                             uint64_t cap, uint64_t* count);
 /* Device bytes held by this simulation. */
 uint64_t as_sim_bytes(const as_sim* sim);
 /* Kernel timing: when enabled, every launch of the method's dominant step

@@ -39,7 +39,7 @@ def run_pair(n, seed, steps, chunk=10, method="plain", oracle_kw=None, **kw):
         if not np.array_equal(got, want):
             s, j = np.argwhere(got != want)[0]
             pytest.fail(f"raster differs at step {done + s} neuron {j}: gpu={got[s, j]} oracle={want[s, j]}")
-        per_neuron += want.sum(axis=0)
+        per_neuron += want.sum(axis=0, dtype=np.int64)
         done += k
     assert_state_bitwise(sim, osim, f"after {steps} steps")
     # per-neuron spike counts: the oracle's raster summed over every step
