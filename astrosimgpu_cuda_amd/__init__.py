@@ -128,6 +128,8 @@ SIGNATURES = [
     ("as_sim_read_raster", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
     ("as_sim_read_spikes", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64,
                                      C.POINTER(C.c_uint64)]),
+    ("as_sim_read_spikes_noalloc", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64,
+                                             C.POINTER(C.c_uint64)]),
     ("as_sim_bytes", C.c_uint64, [C.c_void_p]),
     ("as_sim_set_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("as_sim_kernel_time", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
@@ -368,11 +370,12 @@ class Simulation:
         _check(load().as_sim_read_raster(self.handle, C.c_int64(first_step), C.c_int32(n_steps), _ptr(out)))
         return out
 
-    def spikes(self, first_step: int, n_steps: int, cap: int | None = None) -> tuple[np.ndarray, np.ndarray]:
+    def spikes(self, first_step: int, n_steps: int, cap: int | None = None,
+               no_alloc: bool = False) -> tuple[np.ndarray, np.ndarray]:
         """(steps, neurons) of the spikes of held steps [first_step, first_step
         + n_steps), in step then neuron order. With cap=None the buffers grow
-        until every event fits."""
-        fn = load().as_sim_read_spikes
+        until every event fits; no_alloc uses the allocation-free entry point."""
+        fn = load().as_sim_read_spikes_noalloc if no_alloc else load().as_sim_read_spikes
         want = 4096 if cap is None else cap
         while True:
             steps = np.empty(max(1, want), dtype=np.int64)

@@ -262,7 +262,15 @@ as_status as_sim_read_spikes(as_sim* sim, int64_t first_step, int32_t n_steps, i
                              uint64_t cap, uint64_t* count) {
     return guarded([&] {
         require(sim && count, "NULL argument");
-        astrosim::read_spikes(*sim->eng, first_step, n_steps, steps, neurons, cap, count);
+        astrosim::read_spikes(*sim->eng, first_step, n_steps, steps, neurons, cap, count, false);
+    });
+}
+
+as_status as_sim_read_spikes_noalloc(as_sim* sim, int64_t first_step, int32_t n_steps, int64_t* steps,
+                                     uint32_t* neurons, uint64_t cap, uint64_t* count) {
+    return guarded([&] {
+        require(sim && count, "NULL argument");
+        astrosim::read_spikes(*sim->eng, first_step, n_steps, steps, neurons, cap, count, true);
     });
 }
 

@@ -127,9 +127,10 @@ protected:
 
 // The spikes of steps [first, first + n_steps) as (step, neuron) events in
 // step, then neuron order: the first min(total, cap) into steps/neurons,
-// the total into *count. Synchronises the engine's stream.
+// the total into *count. no_alloc: staged in a fixed device array instead
+// of per-call allocations (cap <= 2^20). Synchronises the engine's stream.
 void read_spikes(Engine& eng, std::int64_t first, std::int64_t n_steps, std::int64_t* steps, std::uint32_t* neurons,
-                 std::uint64_t cap, std::uint64_t* count);
+                 std::uint64_t cap, std::uint64_t* count, bool no_alloc);
 
 std::uint64_t plain_footprint(const Network& net, const RunParams& rp);
 std::unique_ptr<Engine> make_plain_engine(const Network& net, const RunParams& rp, hipStream_t s);

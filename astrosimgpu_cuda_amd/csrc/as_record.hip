@@ -12,10 +12,12 @@
 //     steps' counts and writes its neurons in order (a workgroup scan of the
 //     words' popcounts gives each word its place).
 // Both need device memory beyond the caller's outputs: the per-step counts
-// and the events (the outputs are host arrays); as_sim_read_spikes
-// allocates them per call.
+// and the events (the outputs are host arrays). as_sim_read_spikes allocates
+// them per call; as_sim_read_spikes_noalloc stages them in `scratch`, a
+// fixed device array, and allocates nothing.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "as_engine.h"
@@ -24,6 +26,20 @@ namespace astrosim {
 namespace {
 
 constexpr int kThreads = 256;
+
+// Events the allocation-free path stages per window chunk: 2^20 (8 MB).
+constexpr std::uint64_t kNoAllocEvents = std::uint64_t(1) << 20;
+
+struct ScratchLayout {
+    std::uint32_t counts[kRecordMaxSteps];
+    std::uint32_t step_rel[kNoAllocEvents];
+    std::uint32_t neuron[kNoAllocEvents];
+};
+
+// The allocation-free path's staging, one copy per GPU for the life of the
+// process. Host code must not use this name as a device address: its device
+// address comes from hipGetSymbolAddress (scratch_on_current_device).
+static __device__ __align__(256) unsigned char scratch[sizeof(ScratchLayout)];
 
 __device__ __forceinline__ unsigned long long step_word(const SpikeWords& w, int s, std::uint32_t word) {
     if (w.kind == 0) return static_cast<const unsigned long long*>(w.base)[w.off[s] + word];
@@ -95,16 +111,42 @@ __global__ __launch_bounds__(kThreads) void write_kernel(SpikeWords w, const std
     }
 }
 
+// Device address of this GPU's `scratch`, looked up once per device (with
+// that device current) and kept.
+void* scratch_on_current_device() {
+    static std::mutex mu;
+    static std::vector<void*> ptr;
+    int dev = 0;
+    AS_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    if (ptr.size() <= std::size_t(dev)) ptr.resize(std::size_t(dev) + 1, nullptr);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        AS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptr[dev] = p;
+    }
+    return ptr[dev];
+}
+
 }  // namespace
 
 void read_spikes(Engine& eng, std::int64_t first, std::int64_t n_steps, std::int64_t* steps, std::uint32_t* neurons,
-                 std::uint64_t cap, std::uint64_t* count) {
+                 std::uint64_t cap, std::uint64_t* count, bool no_alloc) {
     if (n_steps < 0 || first < 0) throw Error(AS_ERR_RANGE, "negative step window");
+    if (no_alloc && cap > kNoAllocEvents)
+        throw Error(AS_ERR_INVALID, "cap " + std::to_string(cap) + " exceeds the " + std::to_string(kNoAllocEvents) +
+                                        " events the allocation-free path stages per call");
     if (cap > 0 && (!steps || !neurons)) throw Error(AS_ERR_INVALID, "NULL output arrays");
     hipStream_t s = eng.stream();
     std::uint32_t* counts = nullptr;
     std::uint32_t* ev_step = nullptr;
     std::uint32_t* ev_neuron = nullptr;
+    if (no_alloc) {
+        ScratchLayout* sc = static_cast<ScratchLayout*>(scratch_on_current_device());
+        counts = sc->counts;
+        ev_step = sc->step_rel;
+        ev_neuron = sc->neuron;
+    }
     std::vector<void*> owned;
     auto alloc = [&](std::size_t bytes) {
         void* p = nullptr;
@@ -117,7 +159,7 @@ void read_spikes(Engine& eng, std::int64_t first, std::int64_t n_steps, std::int
     };
     std::uint64_t total = 0, written = 0;
     try {
-        counts = static_cast<std::uint32_t*>(alloc(kRecordMaxSteps * sizeof(std::uint32_t)));
+        if (!no_alloc) counts = static_cast<std::uint32_t*>(alloc(kRecordMaxSteps * sizeof(std::uint32_t)));
         for (std::int64_t c0 = 0; c0 < n_steps; c0 += kRecordMaxSteps) {
             const int n = static_cast<int>(std::min<std::int64_t>(kRecordMaxSteps, n_steps - c0));
             const SpikeWords w = eng.spike_words(first + c0, n);
@@ -131,8 +173,10 @@ void read_spikes(Engine& eng, std::int64_t first, std::int64_t n_steps, std::int
             total += chunk;
             const std::uint64_t want = std::min<std::uint64_t>(chunk, cap - written);
             if (want == 0) continue;
-            ev_step = static_cast<std::uint32_t*>(alloc(want * sizeof(std::uint32_t)));
-            ev_neuron = static_cast<std::uint32_t*>(alloc(want * sizeof(std::uint32_t)));
+            if (!no_alloc) {
+                ev_step = static_cast<std::uint32_t*>(alloc(want * sizeof(std::uint32_t)));
+                ev_neuron = static_cast<std::uint32_t*>(alloc(want * sizeof(std::uint32_t)));
+            }
             hipLaunchKernelGGL(write_kernel, dim3(n), dim3(kThreads), 0, s, w, counts, want, ev_step, ev_neuron);
             AS_HIP_CHECK(hipGetLastError());
             // the steps' 32-bit offsets land in the first half of their int64
@@ -150,9 +194,11 @@ void read_spikes(Engine& eng, std::int64_t first, std::int64_t n_steps, std::int
                 std::memcpy(out + i, &v, sizeof v);
             }
             written += want;
-            for (void* p : {static_cast<void*>(ev_step), static_cast<void*>(ev_neuron)}) {
-                (void)hipFree(p);
-                owned.erase(std::find(owned.begin(), owned.end(), p));
+            if (!no_alloc) {
+                for (void* p : {static_cast<void*>(ev_step), static_cast<void*>(ev_neuron)}) {
+                    (void)hipFree(p);
+                    owned.erase(std::find(owned.begin(), owned.end(), p));
+                }
             }
         }
     } catch (...) {

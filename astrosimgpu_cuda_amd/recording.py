@@ -36,7 +36,8 @@ def min_delay_steps(dt: float) -> int:
     return max(1, int(1.0 / dt + 0.5))
 
 
-def record(sim, n_steps: int, out_dir, ca_every: int = 10, dt: float | None = None) -> dict:
+def record(sim, n_steps: int, out_dir, ca_every: int = 10, dt: float | None = None,
+           no_alloc: bool = False) -> dict:
     """Advances ``sim`` by ``n_steps`` steps, writing its spikes and sampled
     calcium to ``out_dir``. Returns the header written to meta.json."""
     dt = sim.cfg.dt if dt is None else dt
@@ -56,7 +57,7 @@ def record(sim, n_steps: int, out_dir, ca_every: int = 10, dt: float | None = No
                 k = min(k, to_sample)
             s0 = sim.steps
             sim.step(k)
-            steps, neurons = sim.spikes(s0, k)
+            steps, neurons = sim.spikes(s0, k, no_alloc=no_alloc)
             files["spike_steps.i64"].write(steps.astype("<i8").tobytes())
             files["spike_neurons.u32"].write(neurons.astype("<u4").tobytes())
             n_spikes += len(steps)

@@ -151,6 +151,15 @@ as_status as_sim_read_raster(as_sim* sim, int64_t first_step, int32_t n_steps, u
  * (src/astrosim_fused.cu:584); this is the recorder's read. */
 as_status as_sim_read_spikes(as_sim* sim, int64_t first_step, int32_t n_steps, int64_t* steps, uint32_t* neurons,
                              uint64_t cap, uint64_t* count);
+/* The same, allocating nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under graph capture,
+ * tight serving or recording loops). The events are staged in a fixed device
+ * array of 2^20 events (8 MB, one per GPU, held until the process ends), so
+ * cap may not exceed 1,048,576 (AS_ERR_INVALID otherwise); n_steps has no
+ * limit. Every call on a GPU shares that GPU's array: calls that overlap on
+ * one GPU (from several threads) race. */
+as_status as_sim_read_spikes_noalloc(as_sim* sim, int64_t first_step, int32_t n_steps, int64_t* steps,
+                                     uint32_t* neurons, uint64_t cap, uint64_t* count);
 /* Device bytes held by this simulation. */
 uint64_t as_sim_bytes(const as_sim* sim);
 /* Kernel timing: when enabled, every launch of the method's dominant step
