@@ -5,11 +5,16 @@
 // below FP32 resolution over thousands of steps, so each is carried as an
 // unevaluated hi+lo pair of floats and every increment is added with an
 // error-free TwoSum and renormalised (about 48 significant bits). The AdEx
-// spike current g_L*DeltaT*exp((V-V_T)/DeltaT) is evaluated in FP64 while
-// |V - V_T| < 3*DeltaT, where the spike onset is decided; far from V_T it is
-// either negligible or already runaway and FP32 suffices. Random draws stay
-// in FP64 (as_rng.h), so Poisson counts and Gaussian values are bit-identical
-// to the FP64 path. Formulas follow src/astrosim_fused.cu:236-290, 508-576.
+// spike current g_L*DeltaT*exp((V-V_T)/DeltaT) is evaluated in FP64 for every
+// V (spike_current32: one fused argument, a shifter reduction and a degree-8
+// polynomial, rounded to FP32 once): the band |V - V_T| < 3*DeltaT, where the
+// spike onset is decided and FP64 is required, holds some lane of almost
+// every wave, so a per-lane choice would run both paths. Poisson counts use
+// the FP64 path's 64-bit draws and FP64 products (bit-identical counts);
+// Gaussian noise uses the same two uniforms per draw with the Box-Muller
+// transform in FP32 (normal32: hardware log2 and cosine), within FP32
+// rounding of the FP64 draw. Formulas follow src/astrosim_fused.cu:236-290,
+// 508-576.
 #pragma once
 
 #include <cmath>
