@@ -791,6 +791,12 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         pc.C_m_div = in_vgpr(pc.C_m_div);
         pc.tau_w_div = in_vgpr(pc.tau_w_div);
     }
+    // FP32: the spike current's two FP64 constants in vector registers too
+    NeuronConst32 pc32 = exc ? Q.ne : Q.ni;
+    if constexpr (sizeof(R) == 4) {
+        pc32.y_scale_d = in_vgpr(pc32.y_scale_d);
+        pc32.y_shift_d = in_vgpr(pc32.y_shift_d);
+    }
     for (std::int64_t t = lo; t < hi; ++t) {
         const unsigned k = static_cast<unsigned>(t - B.bstart), i = static_cast<unsigned>(t - lo);
         const std::int64_t tl = t - M.step0;  // the member's own step
@@ -822,7 +828,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
                                      exc ? P.ex_decay_e : P.ex_decay_i, exc ? P.in_decay_e : P.in_decay_i);
             V = r.V; w = r.w; g_ex = r.g_ex; dg_ex = r.dg_ex; g_in = r.g_in; dg_in = r.dg_in; refr = r.refractory;
         } else {
-            const NeuronConst32& p = exc ? Q.ne : Q.ni;
+            const NeuronConst32& p = pc32;
             const float I_ext = I_sic + noise_val;
             NeuronRegs32 r{V, w, g_ex, dg_ex + exc_input * p.psc_init_ex, g_in, dg_in + inh_input * p.psc_init_in, refr};
             spiked = neuron_substeps32(p, Q, r, I_ext, V_reset, bb, P.substeps, exc ? Q.ex_decay_e : Q.ex_decay_i,

@@ -7,7 +7,9 @@ to FP32 rounding (relative 1e-5 for V, w; 1e-7 for the compensated Ca, IP3,
 h); over a full config-2 run (N_total=1M, seed 7, pre 200 ms, sim 1000 ms)
 the recorded mean firing rate agrees within 1% and the fractions of
 astrocytes above the SIC threshold (final step and any time in the window)
-within 0.01."""
+within 0.01; the state at 100 ms agrees to the bounds written in the test
+(medians at FP32 rounding, 99th percentiles from cells whose spike times
+moved)."""
 import numpy as np
 import pytest
 
@@ -68,12 +70,24 @@ def test_fp32c_config2_statistics_match_fp64():
     rate32 = sp32 / (net.n_neurons * sim_ms * 1e-3)
     print(f"[fp32c] rate fp64 {rate64:.4f} Hz fp32c {rate32:.4f} Hz (rel {rate32 / rate64 - 1:+.4%}); "
           f"active {act64:.4f} vs {act32:.4f}; engaged {eng64:.4f} vs {eng32:.4f}", flush=True)
+    # State error at 100 ms. The median cell still agrees to FP32 rounding;
+    # the upper percentiles are cells whose spike times moved by a step
+    # (V) and the astrocytes they drive (Ca, IP3, h). Bounds: about 5x the
+    # measured values (V |diff| median 1.3e-5 mV, p99 0.45 mV; relative
+    # medians Ca 1.5e-8, IP3 3.7e-8, h 6.9e-11; p99 Ca 1.0e-3, IP3 4.7e-2,
+    # h 4.2e-6).
+    bounds = {"V": ("abs", 1e-4, 2.0), "Ca": ("rel", 1e-7, 5e-3), "IP3": ("rel", 2e-7, 0.2),
+              "h": ("rel", 1e-9, 2e-5)}
     for f in ("V", "Ca", "IP3", "h"):
         d = np.abs(e32[f] - e64[f])
         r = d / np.maximum(np.abs(e64[f]), 1e-3)
         print(f"[fp32c] state at 100 ms {f}: |diff| median {np.median(d):.3e} p99 {np.quantile(d, 0.99):.3e} "
               f"max {d.max():.3e}; rel median {np.median(r):.3e} p99 {np.quantile(r, 0.99):.3e} "
               f"max {r.max():.3e}", flush=True)
+        kind, med_tol, p99_tol = bounds[f]
+        x = d if kind == "abs" else r
+        assert np.median(x) < med_tol, (f, kind, "median", np.median(x))
+        assert np.quantile(x, 0.99) < p99_tol, (f, kind, "p99", np.quantile(x, 0.99))
     assert abs(rate32 / rate64 - 1.0) < 0.01
     assert abs(act32 - act64) < 0.01
     assert abs(eng32 - eng64) < 0.01
