@@ -561,10 +561,9 @@ class Server:
     with the GPU connectivity generator, kept in device memory; smaller ones
     by the host walk; ``setup_where`` as in include/astrosim.h), admitted into
     resident FP32c/FP64 batches as their footprint fits, advanced in rounds
-    of blocks pipelined two deep with both batches overlapping on the device,
-    and retired when done. Give the process more hardware queues than build
-    threads + 6 (GPU_MAX_HW_QUEUES, read when HIP initialises): streams that
-    share a queue wait for each other's work."""
+    of blocks pipelined two deep, and retired when done. The server creates
+    no more HIP streams than the process has hardware queues
+    (GPU_MAX_HW_QUEUES, HIP's default 4), so it needs no setting of it."""
 
     def __init__(self, **kw):
         lib = load()

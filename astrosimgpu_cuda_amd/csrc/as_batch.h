@@ -72,6 +72,11 @@ public:
     // last step, so the read runs beside them. The caller has waited for the
     // work that ran the member's last step.
     virtual void read_final(int member, std::uint64_t* spikes, double* ca) = 0;
+    // Enqueues the copies read_final makes on the batch's block stream, after
+    // the work enqueued so far (which must include the member's last step),
+    // into pinned staging; the member's read_final then waits for these
+    // copies only. A block stream needs no stream of its own for reads.
+    virtual void prefetch_final(int member) = 0;
     virtual std::int64_t member_target(int member) const = 0;
     virtual bool member_active(int member) const = 0;
     virtual int active_members() const = 0;
@@ -87,6 +92,11 @@ public:
 };
 
 std::uint64_t batch_footprint(const BatchCapacity& cap, const RunParams& rp);
-std::unique_ptr<Batch> make_batch(const RunParams& rp, const BatchCapacity& cap, hipStream_t s);
+// s: the stream of the batch's blocks. io: the stream of member uploads and
+// of result reads that were not prefetched (nullptr: two streams of the
+// batch's own); a caller-given io stream may be shared by several batches
+// and must outlive this one.
+std::unique_ptr<Batch> make_batch(const RunParams& rp, const BatchCapacity& cap, hipStream_t s,
+                                  hipStream_t io = nullptr);
 
 }  // namespace astrosim

@@ -1165,7 +1165,7 @@ struct MemberHost {
 template <class R>
 class BatchImpl final : public Batch {
 public:
-    BatchImpl(const RunParams& rp, const BatchCapacity& cap, hipStream_t s)
+    BatchImpl(const RunParams& rp, const BatchCapacity& cap, hipStream_t s, hipStream_t io)
         : P_(rp), Q_(make_params32(rp)), cap_(cap), stream_(s), geo_(geometry(rp)),
           nalloc_(cap.neurons), aalloc_(cap.astros),
           ealloc_{RangeAllocator(cap.edges[0]), RangeAllocator(cap.edges[1]), RangeAllocator(cap.edges[2]),
@@ -1212,10 +1212,17 @@ public:
         D_.members = (const Member*)dptr(lay_.members);
         D_.tiles = (const Tile*)dptr(lay_.tiles);
         spans_ = (TileSpan*)dptr(lay_.spans);
-        AS_HIP_CHECK(hipStreamCreateWithFlags(&up_, hipStreamNonBlocking));
-        AS_HIP_CHECK(hipStreamCreateWithFlags(&rd_, hipStreamNonBlocking));
+        own_io_ = io == nullptr;
+        if (own_io_) {
+            AS_HIP_CHECK(hipStreamCreateWithFlags(&up_, hipStreamNonBlocking));
+            AS_HIP_CHECK(hipStreamCreateWithFlags(&rd_, hipStreamNonBlocking));
+        } else {
+            up_ = rd_ = io;
+        }
         AS_HIP_CHECK(hipEventCreateWithFlags(&enqueued_, hipEventDisableTiming));
         pin_slot_.resize(cap.members);
+        pin_prefetch_.resize(cap.members);
+        prefetched_.assign(cap.members, nullptr);
         AS_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pin_members_), std::size_t(cap.members) * sizeof(Member)));
         AS_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pin_spans_), std::size_t(cap.members) * sizeof(TileSpan)));
         AS_HIP_CHECK(hipEventCreateWithFlags(&tables_copied_, hipEventDisableTiming));
@@ -1233,7 +1240,8 @@ public:
             host_ctx[0] = {D_, S_, P_, Q_};
             host_ctx[1] = host_ctx[0];
             host_ctx[1].D.clock = clock_;
-            AS_HIP_CHECK(hipMemcpy(ctx_eager_, host_ctx, sizeof host_ctx, hipMemcpyHostToDevice));
+            AS_HIP_CHECK(hipMemcpyAsync(ctx_eager_, host_ctx, sizeof host_ctx, hipMemcpyHostToDevice, stream_));
+            AS_HIP_CHECK(hipStreamSynchronize(stream_));
         }
         AS_HIP_CHECK(hipMemsetAsync(D_.lcount, 0, std::size_t(geo_.H) * 2 * 4, stream_));
         AS_HIP_CHECK(hipMemsetAsync(D_.overflow, 0, sizeof(unsigned), stream_));
@@ -1248,10 +1256,14 @@ public:
         if (up_) (void)hipStreamSynchronize(up_);
         if (rd_) (void)hipStreamSynchronize(rd_);
         if (base_) (void)hipFree(base_);
-        if (up_) (void)hipStreamDestroy(up_);
-        if (rd_) (void)hipStreamDestroy(rd_);
+        if (own_io_ && up_) (void)hipStreamDestroy(up_);
+        if (own_io_ && rd_) (void)hipStreamDestroy(rd_);
         for (auto& h : hosts_)
             if (h.up_done) (void)hipEventDestroy(h.up_done);
+        for (std::size_t i = 0; i < pin_prefetch_.size(); ++i) {
+            if (pin_prefetch_[i].p) (void)hipHostFree(pin_prefetch_[i].p);
+            if (prefetched_[i]) (void)hipEventDestroy(prefetched_[i]);
+        }
         for (Pinned& b : pin_slot_)
             if (b.p) (void)hipHostFree(b.p);
         for (Pinned& b : pin_free_) (void)hipHostFree(b.p);
@@ -1340,6 +1352,12 @@ public:
         MemberHost& h = host(member);
         if (!h.active) return;
         h.active = false;
+        const std::size_t slot = std::size_t(member);
+        if (pin_prefetch_[slot].p) {  // prefetched but never read: the staging returns to the pool
+            AS_HIP_CHECK(hipEventSynchronize(prefetched_[slot]));
+            give_pinned(pin_prefetch_[slot]);
+            pin_prefetch_[slot] = Pinned();
+        }
         // emissions of the member's last block are still scattered d/L
         // blocks later (into its own ranges): release after that
         h.free_at_block = (step_ + geo_.L - 1) / geo_.L + geo_.d / geo_.L + 1;
@@ -1536,11 +1554,57 @@ public:
         return std::max<std::int64_t>(0, std::min(g, h.m.step_end) - h.m.step0);
     }
 
-    void read_final(int member, std::uint64_t* spikes, double* ca) override {
-        const MemberHost& h = host(member);
+    // Copies a member's spike counters, Ca and (FP32c) Ca_lo into pinned
+    // staging `base` on stream s.
+    void copy_final(const MemberHost& h, char* base, hipStream_t s) {
         const std::size_t n = h.m.na;
         constexpr std::size_t kCountBytes = kCountersPerMember * sizeof(unsigned long long);
-        const std::size_t need = kCountBytes + 2 * pinned_bytes(n);
+        AS_HIP_CHECK(hipMemcpyAsync(base, D_.counters + std::size_t(h.m.slot) * kCountersPerMember, kCountBytes,
+                                    hipMemcpyDeviceToHost, s));
+        if (n) AS_HIP_CHECK(hipMemcpyAsync(base + kCountBytes, S_.Ca + h.m.ab, n * sizeof(R), hipMemcpyDeviceToHost, s));
+        if (sizeof(R) == 4 && n)
+            AS_HIP_CHECK(hipMemcpyAsync(base + kCountBytes + pinned_bytes(n), S_.Ca_lo + h.m.ab, n * sizeof(R),
+                                        hipMemcpyDeviceToHost, s));
+    }
+
+    // The results from staging filled by copy_final.
+    void decode_final(const MemberHost& h, const char* base, std::uint64_t* spikes, double* ca) const {
+        const std::size_t n = h.m.na;
+        constexpr std::size_t kCountBytes = kCountersPerMember * sizeof(unsigned long long);
+        const auto* c = reinterpret_cast<const unsigned long long*>(base);
+        const R* hv = reinterpret_cast<const R*>(base + kCountBytes);
+        const R* lv = reinterpret_cast<const R*>(base + kCountBytes + pinned_bytes(n));
+        std::uint64_t t = 0;
+        for (int i = 0; i < kCountersPerMember; ++i) t += c[i];
+        *spikes = t;
+        for (std::size_t i = 0; i < n; ++i) ca[i] = double(hv[i]) + (sizeof(R) == 4 ? double(lv[i]) : 0.0);
+    }
+
+    static std::size_t final_bytes(std::size_t n_astro) {
+        return kCountersPerMember * sizeof(unsigned long long) + 2 * pinned_bytes(n_astro);
+    }
+
+    void prefetch_final(int member) override {
+        const MemberHost& h = host(member);
+        const std::size_t slot = std::size_t(member);
+        if (pin_prefetch_[slot].p) return;  // already enqueued
+        pin_prefetch_[slot] = take_pinned(final_bytes(h.m.na));
+        copy_final(h, static_cast<char*>(pin_prefetch_[slot].p), stream_);
+        if (!prefetched_[slot]) AS_HIP_CHECK(hipEventCreateWithFlags(&prefetched_[slot], hipEventDisableTiming));
+        AS_HIP_CHECK(hipEventRecord(prefetched_[slot], stream_));
+    }
+
+    void read_final(int member, std::uint64_t* spikes, double* ca) override {
+        const MemberHost& h = host(member);
+        const std::size_t slot = std::size_t(member);
+        if (pin_prefetch_[slot].p) {
+            AS_HIP_CHECK(hipEventSynchronize(prefetched_[slot]));
+            decode_final(h, static_cast<const char*>(pin_prefetch_[slot].p), spikes, ca);
+            give_pinned(pin_prefetch_[slot]);
+            pin_prefetch_[slot] = Pinned();
+            return;
+        }
+        const std::size_t need = final_bytes(h.m.na);
         if (pin_read_.bytes < need) {
             if (pin_read_.p) AS_HIP_CHECK(hipHostFree(pin_read_.p));
             pin_read_ = Pinned();
@@ -1548,21 +1612,9 @@ public:
             AS_HIP_CHECK(hipHostMalloc(&pin_read_.p, bytes, hipHostMallocDefault));
             pin_read_.bytes = bytes;
         }
-        char* base = static_cast<char*>(pin_read_.p);
-        auto* c = reinterpret_cast<unsigned long long*>(base);
-        R* hv = reinterpret_cast<R*>(base + kCountBytes);
-        R* lv = reinterpret_cast<R*>(base + kCountBytes + pinned_bytes(n));
-        const bool has_lo = sizeof(R) == 4;
-        AS_HIP_CHECK(hipMemcpyAsync(c, D_.counters + std::size_t(h.m.slot) * kCountersPerMember, kCountBytes,
-                                    hipMemcpyDeviceToHost, rd_));
-        if (n) AS_HIP_CHECK(hipMemcpyAsync(hv, S_.Ca + h.m.ab, n * sizeof(R), hipMemcpyDeviceToHost, rd_));
-        if (has_lo && n)
-            AS_HIP_CHECK(hipMemcpyAsync(lv, S_.Ca_lo + h.m.ab, n * sizeof(R), hipMemcpyDeviceToHost, rd_));
+        copy_final(h, static_cast<char*>(pin_read_.p), rd_);
         AS_HIP_CHECK(hipStreamSynchronize(rd_));
-        std::uint64_t t = 0;
-        for (int i = 0; i < kCountersPerMember; ++i) t += c[i];
-        *spikes = t;
-        for (std::size_t i = 0; i < n; ++i) ca[i] = double(hv[i]) + (has_lo ? double(lv[i]) : 0.0);
+        decode_final(h, static_cast<const char*>(pin_read_.p), spikes, ca);
     }
 
     std::int64_t member_target(int member) const override { return host(member).m.step_end - host(member).m.step0; }
@@ -1849,14 +1901,17 @@ private:
             const std::size_t ne = net.edges(T);
             const index_t eb = static_cast<index_t>(h.ebase[T]);
             index_t* src = const_cast<index_t*>(D_.in_src[T]) + eb;
+            // host CSRs from their pinned copy when there is one
+            const index_t* h_src = net.pin_src[T] ? net.pin_src[T] : c[T]->src.data();
+            const index_t* h_rs = net.pin_rs[T] ? net.pin_rs[T] : c[T]->row_start.data();
             if (ne) {
-                AS_HIP_CHECK(hipMemcpyAsync(src, dev ? net.dev[T].src : c[T]->src.data(), ne * sizeof(index_t),
+                AS_HIP_CHECK(hipMemcpyAsync(src, dev ? net.dev[T].src : h_src, ne * sizeof(index_t),
                                             dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
                 if (sbase[T] != 0)
                     hipLaunchKernelGGL(rebase_kernel, dim3(unsigned((ne + 255) / 256)), dim3(256), 0, s, src, ne,
                                        sbase[T]);
             }
-            AS_HIP_CHECK(hipMemcpyAsync(scratch0_, dev ? net.dev[T].row_start : c[T]->row_start.data(),
+            AS_HIP_CHECK(hipMemcpyAsync(scratch0_, dev ? net.dev[T].row_start : h_rs,
                                         (std::size_t(nt) + 1) * sizeof(index_t),
                                         dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
             uint2* in_rs = const_cast<uint2*>(D_.in_rs[T]) + tbase[T];
@@ -1938,6 +1993,9 @@ private:
     hipEvent_t tables_copied_ = nullptr;
     hipStream_t up_ = nullptr;          // member uploads (add_async)
     hipStream_t rd_ = nullptr;          // reads of finished members (read_final)
+    bool own_io_ = true;                // up_ and rd_ are the batch's own (else one shared stream)
+    std::vector<Pinned> pin_prefetch_;  // per slot: prefetch_final's staging (p == nullptr: none)
+    std::vector<hipEvent_t> prefetched_;  // per slot: its copies' completion
     hipEvent_t enqueued_ = nullptr;     // the batch stream's work at an add_async
     std::vector<Pinned> pin_slot_;  // per slot: the converted parameters of its upload, until activation
     std::vector<Pinned> pin_free_;  // pinned staging buffers not in use
@@ -2002,14 +2060,14 @@ BatchCapacity capacity_for(const Network* const* nets, int n) {
 
 std::uint64_t batch_footprint(const BatchCapacity& cap, const RunParams& rp) { return batch_layout(cap, rp).total; }
 
-std::unique_ptr<Batch> make_batch(const RunParams& rp, const BatchCapacity& cap, hipStream_t s) {
+std::unique_ptr<Batch> make_batch(const RunParams& rp, const BatchCapacity& cap, hipStream_t s, hipStream_t io) {
     if (cap.neurons + cap.astros >= (std::uint64_t(1) << 32) ||
         std::max({cap.edges[0], cap.edges[1], cap.edges[2], cap.edges[3]}) >= (std::uint64_t(1) << 32))
         throw Error(AS_ERR_INVALID, "batch capacity exceeds 32-bit ids");
     if (std::max(cap.neurons, cap.astros) > kSrcMask)
         throw Error(AS_ERR_INVALID, "batch capacity exceeds 2^26 cells of one kind (bucket entry source ids)");
-    if (rp.precision == AS_FP64) return std::unique_ptr<Batch>(new BatchImpl<double>(rp, cap, s));
-    return std::unique_ptr<Batch>(new BatchImpl<float>(rp, cap, s));
+    if (rp.precision == AS_FP64) return std::unique_ptr<Batch>(new BatchImpl<double>(rp, cap, s, io));
+    return std::unique_ptr<Batch>(new BatchImpl<float>(rp, cap, s, io));
 }
 
 std::uint64_t blocked_footprint(const Network& net, const RunParams& rp) {

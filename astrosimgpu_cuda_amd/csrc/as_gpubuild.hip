@@ -43,10 +43,13 @@ struct HipFailure {
 
 class HipBackend {
 public:
-    HipBackend() { GB_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking)); }
+    // on the caller's stream, or on one of its own (nullptr)
+    explicit HipBackend(hipStream_t s) : s_(s), own_(s == nullptr) {
+        if (own_) GB_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    }
     ~HipBackend() {
         (void)hipStreamSynchronize(s_);
-        (void)hipStreamDestroy(s_);
+        if (own_) (void)hipStreamDestroy(s_);
     }
     // A failed allocation declines the build (the caller falls back to the
     // host walk) rather than failing it.
@@ -104,12 +107,14 @@ private:
         release(tmp);
     }
     hipStream_t s_ = nullptr;
+    bool own_ = true;
 };
 
 }  // namespace
 
 bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::string* why,
-                       gb::BuildStats* stats, bool keep_on_device) {
+                       gb::BuildStats* stats, bool keep_on_device, hipStream_t stream, hipStream_t free_stream) {
+    if (!free_stream) free_stream = stream;
     std::string reason;
     auto decline = [&](const std::string& r) {
         if (why) *why = r;
@@ -154,16 +159,16 @@ bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::s
         };
         std::vector<void*> kept;
         {
-            HipBackend bk;  // its destructor waits for the build's stream
+            HipBackend bk(stream);  // its destructor waits for the build's stream
             ok = gb::build_connectivity(bk, cfg, seed, c0_lo, pools_fn, ScalingConfig::kPoolSize, out, st, reason,
                                         keep_on_device ? &kept : nullptr);
         }
         if (pools.valid()) pools.wait();
         if (ok && keep_on_device) {
             auto* held = new std::vector<void*>(std::move(kept));
-            out.dev_owner = std::shared_ptr<void>(held, [](void* p) {
+            out.dev_owner = std::shared_ptr<void>(held, [free_stream](void* p) {
                 auto* v = static_cast<std::vector<void*>*>(p);
-                for (void* q : *v) (void)hipFreeAsync(q, nullptr);
+                for (void* q : *v) (void)hipFreeAsync(q, free_stream);
                 delete v;
             });
         }
@@ -186,11 +191,12 @@ bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::s
     return true;
 }
 
-void stage_csrs_on_device(Network& net) {
+void stage_csrs_on_device(Network& net, hipStream_t stream, hipStream_t free_stream) {
     if (net.on_device()) return;
+    if (!free_stream) free_stream = stream;
     HostCSR* c[4] = {&net.inc_exc, &net.inc_inh, &net.inc_sic, &net.inc_e2a};
-    hipStream_t s = nullptr;
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+    hipStream_t s = stream;
+    if (!s && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
         (void)hipGetLastError();
         return;  // left on the host
     }
@@ -220,14 +226,14 @@ void stage_csrs_on_device(Network& net) {
         delete held;
         for (auto& d : net.dev) d = DeviceCSR{};
     } else {
-        net.dev_owner = std::shared_ptr<void>(held, [](void* p) {
+        net.dev_owner = std::shared_ptr<void>(held, [free_stream](void* p) {
             auto* v = static_cast<std::vector<void*>*>(p);
-            for (void* q : *v) (void)hipFreeAsync(q, nullptr);
+            for (void* q : *v) (void)hipFreeAsync(q, free_stream);
             delete v;
         });
         for (HostCSR* h : c) *h = HostCSR();
     }
-    (void)hipStreamDestroy(s);
+    if (!stream) (void)hipStreamDestroy(s);
 }
 
 void materialize_host_csrs(Network& net) {

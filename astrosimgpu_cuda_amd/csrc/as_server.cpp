@@ -9,11 +9,28 @@
 // deep), and retires members that reached their step count, reading their
 // results and freeing their ranges for the next admissions. No request waits
 // for a batch to drain.
+//
+// Streams and hardware queues: work on two HIP streams that share a hardware
+// queue runs in submission order, so a batch's blocks on a queue shared with
+// the network generator would wait for its kernels. The server therefore
+// creates no more streams than the process has queues (GPU_MAX_HW_QUEUES,
+// HIP's default 4): one block stream per batch from 5 queues on, else one
+// block stream for both (the batches' rounds then run one after the other),
+// one upload stream, and the rest as build streams (at most 8), each held by
+// one builder at a time. At 4 queues that is 1 + 1 + 2; two build streams
+// served the measured load faster than two block streams with one.
+// A builder that finds no build stream free builds a network below
+// kGpuWaitMin cells with the host walk instead, and waits for a stream
+// otherwise (the host walk takes about 2 s at 1M cells). Host-built networks
+// need no stream: their CSRs are copied into pinned host memory (a pool the
+// server keeps), and the upload copies them from there asynchronously.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -30,6 +47,76 @@ namespace astrosim {
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+// Networks from this size on wait for a free build stream rather than take
+// the host walk (0.8 s at 400,000 cells against about 0.1 s on the GPU).
+constexpr index_t kGpuWaitMin = 400000;
+
+// Pinned host buffers, reused: a buffer returns to the pool when the network
+// staged in it is dropped (after its upload has completed).
+class PinnedPool {
+public:
+    ~PinnedPool() {
+        for (auto& b : all_) (void)hipHostFree(b.first);
+    }
+    // A buffer of at least `bytes` (its size in *got).
+    void* take(std::size_t bytes, std::size_t* got) {
+        std::lock_guard<std::mutex> g(mu_);
+        std::size_t best = free_.size();
+        for (std::size_t i = 0; i < free_.size(); ++i)
+            if (free_[i].second >= bytes && (best == free_.size() || free_[i].second < free_[best].second)) best = i;
+        if (best < free_.size()) {
+            auto b = free_[best];
+            free_.erase(free_.begin() + std::ptrdiff_t(best));
+            *got = b.second;
+            return b.first;
+        }
+        const std::size_t sz = (std::max<std::size_t>(bytes, 1) + (std::size_t(4) << 20) - 1) & ~((std::size_t(4) << 20) - 1);
+        void* p = nullptr;
+        AS_HIP_CHECK(hipHostMalloc(&p, sz, hipHostMallocDefault));
+        all_.emplace_back(p, sz);
+        *got = sz;
+        return p;
+    }
+    void give(void* p, std::size_t bytes) {
+        std::lock_guard<std::mutex> g(mu_);
+        free_.emplace_back(p, bytes);
+    }
+
+private:
+    std::mutex mu_;
+    std::vector<std::pair<void*, std::size_t>> all_, free_;
+};
+
+// Copies a host-built network's CSRs into one pinned buffer from `pool`.
+void stage_pinned(Network& net, PinnedPool& pool) {
+    const HostCSR* c[4] = {&net.inc_exc, &net.inc_inh, &net.inc_sic, &net.inc_e2a};
+    std::size_t total = 0;
+    for (const HostCSR* h : c) total += (h->row_start.size() + h->src.size()) * sizeof(index_t);
+    std::size_t got = 0;
+    char* p = static_cast<char*>(pool.take(total, &got));
+    std::size_t off = 0;
+    for (int k = 0; k < 4; ++k) {
+        index_t* rs = reinterpret_cast<index_t*>(p + off);
+        std::memcpy(rs, c[k]->row_start.data(), c[k]->row_start.size() * sizeof(index_t));
+        off += c[k]->row_start.size() * sizeof(index_t);
+        index_t* src = reinterpret_cast<index_t*>(p + off);
+        if (!c[k]->src.empty()) std::memcpy(src, c[k]->src.data(), c[k]->src.size() * sizeof(index_t));
+        off += c[k]->src.size() * sizeof(index_t);
+        net.pin_rs[k] = rs;
+        net.pin_src[k] = src;
+    }
+    PinnedPool* pp = &pool;
+    net.pin_owner = std::shared_ptr<void>(p, [pp, got](void* q) { pp->give(q, got); });
+}
+
+// The process's hardware queues per device: GPU_MAX_HW_QUEUES, else HIP's
+// default of 4.
+int hw_queues() {
+    const char* v = std::getenv("GPU_MAX_HW_QUEUES");
+    const int q = v ? std::atoi(v) : 0;
+    return q > 0 ? q : 4;
+}
 
 struct Req {
     as_request in{};
@@ -88,13 +175,27 @@ public:
         const std::uint32_t members = std::max(8, c.max_members);
         cap_[0] = capacity_for_budget(std::uint64_t(budget * (1.0 - share64)), rp_[0], members);
         cap_[1] = capacity_for_budget(std::uint64_t(budget * share64), rp_[1], members);
+        // One stream per hardware queue, block streams first: the batches'
+        // blocks, with the reads of finished members' results enqueued right
+        // after the round that ends them (Batch::prefetch_final); the member
+        // uploads of both batches (they build each member's out-CSR: device
+        // work beside the blocks); the build streams.
+        const int queues = hw_queues();
+        const int n_block = queues >= 5 ? 2 : 1;
+        AS_HIP_CHECK(hipStreamCreateWithFlags(&stream_[0], hipStreamNonBlocking));
+        if (n_block == 2) AS_HIP_CHECK(hipStreamCreateWithFlags(&stream_[1], hipStreamNonBlocking));
+        else stream_[1] = stream_[0];
+        AS_HIP_CHECK(hipStreamCreateWithFlags(&io_, hipStreamNonBlocking));
+        const int n_build = std::min(8, std::max(1, queues - n_block - 1));
+        build_.assign(std::size_t(n_build), nullptr);
+        for (hipStream_t& b : build_) AS_HIP_CHECK(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
+        free_build_ = build_;
         for (int i = 0; i < 2; ++i) {
-            AS_HIP_CHECK(hipStreamCreateWithFlags(&stream_[i], hipStreamNonBlocking));
             for (int k = 0; k < 2; ++k) {
                 AS_HIP_CHECK(hipEventCreate(&round0_[k][i]));
                 AS_HIP_CHECK(hipEventCreate(&round1_[k][i]));
             }
-            batch_[i] = make_batch(rp_[i], cap_[i], stream_[i]);
+            batch_[i] = make_batch(rp_[i], cap_[i], stream_[i], io_);
         }
         setup_where_ = c.setup_where;
         const int nb = std::max(1, c.build_threads);
@@ -107,17 +208,28 @@ public:
             std::lock_guard<std::mutex> g(mu_);
             stop_ = true;
         }
+        stopping_ = true;
         cv_build_.notify_all();
         cv_ready_.notify_all();
+        cv_stream_.notify_all();
         for (auto& t : builders_) t.join();
         if (driver_.joinable()) driver_.join();
+        // networks not yet resident free their device CSRs on io_
+        for (auto& q : reqs_) q->net.reset();
         for (int i = 0; i < 2; ++i) {
             batch_[i].reset();
             for (int k = 0; k < 2; ++k) {
                 if (round0_[k][i]) (void)hipEventDestroy(round0_[k][i]);
                 if (round1_[k][i]) (void)hipEventDestroy(round1_[k][i]);
             }
-            if (stream_[i]) (void)hipStreamDestroy(stream_[i]);
+        }
+        std::vector<hipStream_t> own = build_;
+        own.insert(own.end(), {stream_[0], io_});
+        if (stream_[1] != stream_[0]) own.push_back(stream_[1]);
+        for (hipStream_t s : own) {
+            if (!s) continue;
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
         }
     }
 
@@ -201,6 +313,26 @@ public:
 private:
     double now() const { return std::chrono::duration<double>(Clock::now() - t0_).count(); }
 
+    // A free build stream, or nullptr when none is free and `wait` is false
+    // (also when the server stops).
+    hipStream_t take_build_stream(bool wait) {
+        std::unique_lock<std::mutex> g(stream_mu_);
+        if (wait) cv_stream_.wait(g, [&] { return stopping_.load() || !free_build_.empty(); });
+        if (free_build_.empty()) return nullptr;
+        hipStream_t s = free_build_.back();
+        free_build_.pop_back();
+        return s;
+    }
+
+    void give_build_stream(hipStream_t s) {
+        if (!s) return;
+        {
+            std::lock_guard<std::mutex> g(stream_mu_);
+            free_build_.push_back(s);
+        }
+        cv_stream_.notify_one();
+    }
+
     void build_loop() {
         for (;;) {
             std::shared_ptr<Req> q;
@@ -213,19 +345,24 @@ private:
             }
             auto net = std::make_unique<Network>();
             bool ok = true;
+            hipStream_t bs = nullptr;
             try {
-                // a GPU-built network stays in device memory until its admission copies it
-                const bool try_gpu = setup_where_ == AS_BUILD_GPU_ONLY ||
-                                     (setup_where_ == AS_BUILD_GPU && q->in.n_total >= kGpuBuildAutoMin);
-                if (!try_gpu || !build_network_gpu(q->in.n_total, q->in.seed, *net, nullptr, nullptr, true)) {
-                    if (setup_where_ == AS_BUILD_GPU_ONLY) throw std::runtime_error("GPU build declined");
+                // a GPU-built network stays in device memory until its admission
+                // copies it; its CSRs are freed on io_ (the driver drops it)
+                const bool only_gpu = setup_where_ == AS_BUILD_GPU_ONLY;
+                const bool try_gpu = only_gpu || (setup_where_ == AS_BUILD_GPU && q->in.n_total >= kGpuBuildAutoMin);
+                if (try_gpu) bs = take_build_stream(only_gpu || q->in.n_total >= kGpuWaitMin);
+                if (!bs || !build_network_gpu(q->in.n_total, q->in.seed, *net, nullptr, nullptr, true, bs, io_)) {
+                    if (only_gpu) throw std::runtime_error(bs ? "GPU build declined" : "server stopping");
+                    give_build_stream(bs);
+                    bs = nullptr;
                     build_network(q->in.n_total, q->in.seed, *net);
-                    // the builder uploads it, not the driver at admission
-                    stage_csrs_on_device(*net);
+                    stage_pinned(*net, pinned_);
                 }
             } catch (...) {
                 ok = false;
             }
+            give_build_stream(bs);
             std::lock_guard<std::mutex> g(mu_);
             // the driver rejected it meanwhile (it stopped after an error)
             if (q->out.status != AS_REQ_BUILDING) continue;
@@ -341,8 +478,8 @@ private:
 
     // Rounds are pipelined two deep: round k+1 is enqueued before the driver
     // waits for round k, so the device runs k+1 while the host admits,
-    // retires (the results of members that finished by the end of k are read
-    // on each batch's read stream) and rebuilds the tile tables for k+2.
+    // retires (the results of members that finished by the end of k were
+    // copied right after k) and rebuilds the tile tables for k+2.
     void drive_loop() {
         const int blocks = std::max(1, cfg_.blocks_per_round);
         int cur = 0, prev = 0;
@@ -372,6 +509,11 @@ private:
                     batch_[b]->enqueue_steps(std::int64_t(blocks) * batch_[b]->block_len());  // async
                     AS_HIP_CHECK(hipEventRecord(round1_[cur][b], stream_[b]));
                     R.end_step[b] = batch_[b]->global_step();
+                    // the results of members this round finishes are read
+                    // right after it on the same stream
+                    for (auto& q : running_[b])
+                        if (batch_[b]->member_steps_at(q->member, R.end_step[b]) >= q->in.n_steps)
+                            batch_[b]->prefetch_final(q->member);
                 }
                 const bool now_ran = R.ran[0] || R.ran[1];
                 // Admission's host work runs while the rounds execute; its
@@ -395,16 +537,23 @@ private:
                     throw Error(AS_ERR_HIP, "injected failure after " + std::to_string(rounds_done_) + " rounds");
                 // complete round `prev`; the one just enqueued keeps the device busy
                 const Round& P = rounds_[prev];
-                float round_ms = 0.0f;
+                int first = -1;
                 for (int b = 0; b < 2; ++b) {
                     if (!P.ran[b]) continue;
                     AS_HIP_CHECK(hipEventSynchronize(round1_[prev][b]));
-                    float ms = 0.0f;
-                    AS_HIP_CHECK(hipEventElapsedTime(&ms, round0_[prev][b], round1_[prev][b]));
-                    round_ms = std::max(round_ms, ms);
                     done_step_[b] = P.end_step[b];
+                    if (first < 0) first = b;
                 }
-                gpu_busy_s_ += 1e-3 * double(round_ms);  // the longer of the two batches' spans
+                // the round's span: from the first batch's start to the
+                // later end of the two (on two streams they overlap)
+                float round_ms = 0.0f;
+                for (int b = 0; b < 2; ++b) {
+                    if (!P.ran[b]) continue;
+                    float ms = 0.0f;
+                    AS_HIP_CHECK(hipEventElapsedTime(&ms, round0_[prev][first], round1_[prev][b]));
+                    round_ms = std::max(round_ms, ms);
+                }
+                gpu_busy_s_ += 1e-3 * double(round_ms);
                 peak_active_ = std::max<std::uint64_t>(peak_active_, P.active);
                 peak_cells_ = std::max<std::uint64_t>(peak_cells_, P.cells);
                 const auto r0 = Clock::now();
@@ -441,7 +590,14 @@ private:
     Clock::time_point t0_;
     RunParams rp_[2];
     BatchCapacity cap_[2];
-    hipStream_t stream_[2] = {nullptr, nullptr};
+    hipStream_t stream_[2] = {nullptr, nullptr};  // each batch's blocks and result reads (may be one stream)
+    hipStream_t io_ = nullptr;                     // member uploads of both batches; frees of built CSRs
+    std::vector<hipStream_t> build_;               // the builders' device work, one builder per stream
+    std::vector<hipStream_t> free_build_;          // build streams no builder holds
+    std::mutex stream_mu_;
+    std::condition_variable cv_stream_;
+    PinnedPool pinned_;  // host-built networks' CSRs until their upload
+    std::atomic<bool> stopping_{false};
     // A round: up to `blocks_per_round` blocks of each batch with members.
     struct Round {
         bool ran[2] = {false, false};

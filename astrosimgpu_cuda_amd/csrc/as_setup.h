@@ -9,6 +9,8 @@
 #include <utility>
 #include <vector>
 
+#include <hip/hip_runtime_api.h>
+
 #include "as_model.h"
 
 namespace astrosim {
@@ -97,6 +99,12 @@ struct Network {
     // batch on the same GPU consumes such a network.
     DeviceCSR dev[4];
     std::shared_ptr<void> dev_owner;
+    // A copy of the host CSRs in pinned host memory, held by pin_owner (a
+    // server stages host-built networks so): uploads copy from it
+    // asynchronously instead of from the vectors above.
+    const index_t* pin_rs[4] = {nullptr, nullptr, nullptr, nullptr};
+    const index_t* pin_src[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::shared_ptr<void> pin_owner;
     bool on_device() const { return dev_owner != nullptr; }
     bool host_csrs() const { return !on_device() || !inc_exc.row_start.empty(); }
     // Edges of CSR `which` (0 exc, 1 inh, 2 sic, 3 e2a), wherever it is held.
@@ -124,14 +132,20 @@ void build_network(index_t n_total, std::uint64_t seed, Network& net);
 // with the threshold at 30,000 measured slower than at 100,000.
 constexpr index_t kGpuBuildAutoMin = 100000;
 
+// `stream`: the HIP stream of the build's device work (nullptr: a stream of
+// its own). `free_stream`: the stream the kept CSRs are freed on when the
+// network goes (nullptr: `stream`; both nullptr: the null stream). A
+// caller-given stream must outlive the network.
 namespace gb { struct BuildStats; }
 bool build_network_gpu(index_t n_total, std::uint64_t seed, Network& net, std::string* why,
-                       gb::BuildStats* stats = nullptr, bool keep_on_device = false);
+                       gb::BuildStats* stats = nullptr, bool keep_on_device = false, hipStream_t stream = nullptr,
+                       hipStream_t free_stream = nullptr);
 
 // Moves a host-built network's CSRs into device memory of the current GPU
 // (as build_network_gpu's keep_on_device leaves them; the host CSRs are then
 // emptied). Leaves the network as it was when device memory is short.
-void stage_csrs_on_device(Network& net);
+// `stream` and `free_stream` as for build_network_gpu.
+void stage_csrs_on_device(Network& net, hipStream_t stream = nullptr, hipStream_t free_stream = nullptr);
 
 // Fills the host CSRs of a network whose connectivity build_network_gpu
 // kept on the device (a copy; the device CSRs stay). No-op otherwise.

@@ -55,13 +55,6 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# Hardware queues per process, read when HIP initialises (nothing has yet).
-# The serving leg runs the two batch streams with their upload and read
-# streams beside the network-build streams; with HIP's default of 4 queues
-# those streams share queues, and every batch synchronisation then also
-# waits for unrelated build kernels.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "24")
-
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 SIC_TH = 0.19669
 

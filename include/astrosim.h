@@ -236,12 +236,14 @@ void as_batch_destroy(as_batch* b);
  * default with the GPU generator from N_total = 100000 on, see
  * as_network_build_on), admitted into resident FP32c / FP64 batches
  * (capacities sized from a fraction of free HBM) as soon as their footprint
- * fits, advanced in rounds of blocks with both batches' streams
- * overlapping on the device, and retired when they reach their step count.
- * Results equal a solo run. The process needs more hardware queues than
- * build_threads + 6 (GPU_MAX_HW_QUEUES, read when HIP initialises; each
- * batch has a block stream, an upload stream and a read stream): streams
- * that share a queue wait for each other's work. */
+ * fits, advanced in rounds of blocks, and retired when they reach their
+ * step count. Results equal a solo run. The server creates no more HIP
+ * streams than the process has hardware queues (GPU_MAX_HW_QUEUES, HIP's
+ * default 4; streams that share a queue wait for each other's work): a
+ * block stream per batch from 5 queues on (else one for both), one upload
+ * stream, and the rest for network builds, one builder per stream; a
+ * builder finding none free builds networks below 400,000 cells on the
+ * host. */
 typedef struct as_server as_server;
 typedef struct as_server_config {
     double dt;                /* ms */
