@@ -74,7 +74,7 @@ class _BatchCap(C.Structure):
 class _BatchInfo(C.Structure):
     _fields_ = [("global_step", C.c_int64), ("block_steps", C.c_int32), ("active_members", C.c_int32),
                 ("block_threads", C.c_int32), ("device_bytes", C.c_uint64), ("free_neurons", C.c_uint64),
-                ("bucket_overflows", C.c_uint64)]
+                ("bucket_overflows", C.c_uint64), ("chunk_steps", C.c_int32)]
 
 
 class _ServerConfig(C.Structure):
@@ -139,6 +139,8 @@ SIGNATURES = [
     ("as_batch_create", C.c_int, [C.POINTER(_BatchCap), C.POINTER(_Config), C.POINTER(C.c_void_p)]),
     ("as_batch_fits", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]),
     ("as_batch_add", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]),
+    ("as_batch_add_many", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int32)]),
     ("as_batch_remove", C.c_int, [C.c_void_p, C.c_int32]),
     ("as_batch_step", C.c_int, [C.c_void_p, C.c_int64]),
     ("as_batch_run_graph", C.c_int, [C.c_void_p, C.c_int64]),
@@ -467,6 +469,23 @@ class Batch:
         _check(load().as_batch_add(self.handle, net.handle, C.c_int64(n_steps), C.byref(m)))
         self._sizes[m.value] = (net.n_neurons, net.n_astro)
         return m.value
+
+    def add_many(self, nets, n_steps) -> list[int]:
+        """add() for several networks, uploaded back to back and waited for
+        once; ``n_steps`` is one count for all or one per network."""
+        nets = list(nets)
+        steps = [int(n_steps)] * len(nets) if isinstance(n_steps, int) else [int(s) for s in n_steps]
+        if len(steps) != len(nets):
+            raise ValueError("one step count per network")
+        hs = (C.c_void_p * len(nets))(*[n.handle for n in nets])
+        st = (C.c_int64 * len(nets))(*steps)
+        ms = (C.c_int32 * len(nets))()
+        err = load().as_batch_add_many(self.handle, hs, len(nets), st, ms)
+        for m, n in zip(ms, nets):
+            if m >= 0:
+                self._sizes[m] = (n.n_neurons, n.n_astro)
+        _check(err)
+        return list(ms)
 
     def remove(self, member: int) -> None:
         _check(load().as_batch_remove(self.handle, member))

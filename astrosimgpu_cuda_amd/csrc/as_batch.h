@@ -33,6 +33,11 @@ public:
     // Admits `net` to run `n_steps` steps starting at the current global
     // step, which must be a block boundary. Returns the member slot.
     virtual int add(const Network& net, std::int64_t n_steps) = 0;
+    // add for n networks at once: their uploads are enqueued back to back
+    // and waited for once. members[i] receives network i's slot. If one
+    // does not fit, the ones before it are admitted, the rest get -1 and
+    // the error is thrown.
+    virtual void add_many(const Network* const* nets, int n, const std::int64_t* n_steps, int* members) = 0;
     // Reserves `net`'s ranges and slot and uploads it on the batch's upload
     // stream, after the work enqueued so far and beside later steps; the
     // member joins at the first block boundary after the upload completes
@@ -55,6 +60,8 @@ public:
     // keeps the fastest (results do not depend on it). Returns the choice.
     virtual int autotune(const int* candidates, int n, int blocks_each) = 0;
     virtual int block_threads() const = 0;
+    // Steps one integrate launch advances (the LDS chunk in use).
+    virtual int chunk_steps() const = 0;
 
     virtual std::uint64_t spike_count(int member) = 0;
     virtual void reset_ever_active(int member) = 0;

@@ -1291,6 +1291,24 @@ public:
         return slot;
     }
 
+    void add_many(const Network* const* nets, int n, const std::int64_t* n_steps, int* members) override {
+        if (step_ % geo_.L != 0) throw Error(AS_ERR_INVALID, "members join at block boundaries");
+        for (int i = 0; i < n; ++i) members[i] = -1;
+        auto join = [&](int upto) {
+            AS_HIP_CHECK(hipStreamSynchronize(up_));
+            for (int i = 0; i < upto; ++i) activate(members[i]);
+            if (upto > 0) rebuild_tiles();
+        };
+        int i = 0;
+        try {
+            for (; i < n; ++i) members[i] = add_async(*nets[i], n_steps[i]);
+        } catch (...) {
+            join(i);
+            throw;
+        }
+        join(n);
+    }
+
     int add_async(const Network& net, std::int64_t n_steps) override {
         if (n_steps < 0) throw Error(AS_ERR_INVALID, "n_steps must be >= 0");
         release_due();
@@ -1403,34 +1421,50 @@ public:
         release_due();
     }
 
+    // Times `blocks_each` real blocks per (workgroup size, LDS chunk) pair
+    // and keeps the fastest: each candidate workgroup size with the longest
+    // chunk its LDS budget allows and with about half of it (half the
+    // staging per lane, so more resident waves, against a second launch
+    // per block). Results do not depend on either.
     int autotune(const int* cands, int n, int blocks_each) override {
         if (n <= 0) return threads_;
         if (step_ % geo_.L != 0) throw Error(AS_ERR_INVALID, "autotuning starts at a block boundary");
         double best_ms = 1e300;
-        int best = threads_;
+        int best = threads_, best_cap = chunk_cap_;
         for (int i = 0; i < n; ++i) {
             const int c = cands[i];
             if (c < 64 || c > 1024 || c % 64 != 0) continue;
             set_threads(c);
-            hipEvent_t e0, e1;
-            AS_HIP_CHECK(hipEventCreate(&e0));
-            AS_HIP_CHECK(hipEventCreate(&e1));
-            AS_HIP_CHECK(hipEventRecord(e0, stream_));
-            enqueue_steps(std::int64_t(blocks_each) * geo_.L);
-            AS_HIP_CHECK(hipEventRecord(e1, stream_));
-            AS_HIP_CHECK(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            AS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = c;
+            set_chunk_cap(0);
+            const int full = chunk_len();
+            const int caps[2] = {0, full >= 4 ? (full + 1) / 2 : 0};
+            for (int k = 0; k < 2; ++k) {
+                if (k == 1 && caps[1] == 0) break;
+                set_chunk_cap(caps[k]);
+                hipEvent_t e0, e1;
+                AS_HIP_CHECK(hipEventCreate(&e0));
+                AS_HIP_CHECK(hipEventCreate(&e1));
+                AS_HIP_CHECK(hipEventRecord(e0, stream_));
+                enqueue_steps(std::int64_t(blocks_each) * geo_.L);
+                AS_HIP_CHECK(hipEventRecord(e1, stream_));
+                AS_HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                AS_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = c;
+                    best_cap = caps[k];
+                }
             }
         }
         set_threads(best);
+        set_chunk_cap(best_cap);
         return best;
     }
+
+    int chunk_steps() const override { return chunk_len(); }
 
     int block_threads() const override { return threads_; }
 
@@ -1708,7 +1742,7 @@ private:
     // lane (3 resident waves per SIMD share 160 KB) and 64 KB per workgroup.
     int chunk_len() const {
         const int per_lane = std::min(208, 65536 / threads_);
-        int c = geo_.L;
+        int c = chunk_cap_ > 0 ? std::min(chunk_cap_, geo_.L) : geo_.L;
         while (c > 1 && lds_words(c) * int(sizeof(R)) > per_lane) --c;
         return c;
     }
@@ -1742,6 +1776,13 @@ private:
         if (t == threads_) return;
         threads_ = t;
         rebuild_tiles();
+    }
+    // Caps the LDS chunk (0: the longest the budget allows); captured
+    // graphs hold the old launch shapes.
+    void set_chunk_cap(int c) {
+        if (c == chunk_cap_) return;
+        chunk_cap_ = c;
+        drop_graphs();
     }
 
     // The member table and the tile table of the active members. The host
@@ -2002,6 +2043,7 @@ private:
     Pinned pin_read_;               // read_final's destination
     std::int64_t step_ = 0;
     std::map<std::int64_t, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
+    int chunk_cap_ = 0;  // most steps per integrate launch (0: as many as the LDS budget allows)
     bool timing_ = false;
     hipEvent_t pending_ = nullptr;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;

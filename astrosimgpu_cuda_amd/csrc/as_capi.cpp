@@ -374,6 +374,27 @@ as_status as_batch_add(as_batch* b, const as_network* net, int64_t n_steps, int3
     });
 }
 
+as_status as_batch_add_many(as_batch* b, const as_network* const* nets, int32_t n, const int64_t* n_steps,
+                            int32_t* members) {
+    return guarded([&] {
+        require(b && (n == 0 || (nets && n_steps && members)), "NULL argument");
+        require(n >= 0, "negative count");
+        std::vector<const astrosim::Network*> v(static_cast<std::size_t>(n));
+        for (int32_t i = 0; i < n; ++i) {
+            require(nets[i] != nullptr, "NULL network");
+            v[std::size_t(i)] = &nets[i]->net;
+        }
+        std::vector<int> m(static_cast<std::size_t>(n), -1);
+        try {
+            b->b->add_many(v.data(), n, n_steps, m.data());
+        } catch (...) {
+            for (int32_t i = 0; i < n; ++i) members[i] = m[std::size_t(i)];
+            throw;
+        }
+        for (int32_t i = 0; i < n; ++i) members[i] = m[std::size_t(i)];
+    });
+}
+
 as_status as_batch_remove(as_batch* b, int32_t member) {
     return guarded([&] {
         require(b != nullptr, "batch is NULL");
@@ -424,6 +445,7 @@ as_status as_batch_get_info(as_batch* b, as_batch_info* info) {
         info->device_bytes = b->b->device_bytes();
         info->free_neurons = b->b->free_neurons();
         info->bucket_overflows = b->b->member_overflows();
+        info->chunk_steps = b->b->chunk_steps();
     });
 }
 

@@ -43,15 +43,16 @@ def run_batched(nets: list[Network], steps: int, precision: str = "fp32c", graph
     per-network spike counts."""
     t_create = time.perf_counter()
     batch = Batch(nets, precision=precision)
-    members = [batch.add(n, steps) for n in nets]
+    members = batch.add_many(nets, steps)
     batch.sync()
     t0 = time.perf_counter()
     L = batch.info()["block_steps"]
     done = 0
     chosen = batch.info()["block_threads"]
     if autotune:
-        chosen = batch.autotune(autotune, tune_blocks)  # runs len(autotune)*tune_blocks real blocks
-        done = len(autotune) * tune_blocks * L
+        # runs tune_blocks real blocks per (workgroup size, LDS chunk) pair
+        chosen = batch.autotune(autotune, tune_blocks)
+        done = batch.info()["global_step"]
     while done + graph_blocks * L <= steps:
         batch.run_graph(graph_blocks)
         done += graph_blocks * L
@@ -63,7 +64,8 @@ def run_batched(nets: list[Network], steps: int, precision: str = "fp32c", graph
     info = batch.info()
     batch.close()
     return {"wall_s": wall, "setup_s": t0 - t_create, "sims": len(nets), "sims_per_s": len(nets) / wall,
-            "steps": steps, "block_threads": chosen, "graph_blocks": graph_blocks, "device_bytes": info["device_bytes"],
+            "steps": steps, "block_threads": chosen, "chunk_steps": info["chunk_steps"], "block_steps": L,
+            "graph_blocks": graph_blocks, "device_bytes": info["device_bytes"],
             "bucket_overflows": info["bucket_overflows"], "spikes": spikes}
 
 

@@ -47,6 +47,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -315,7 +316,7 @@ def ensemble_leg(pre_ms: float = 100.0, sim_ms: float = 500.0, n_sample: int = 8
     batched = E.run_batched(nets, steps, precision="fp32c")
     incl = len(nets) / (batched["wall_s"] + batched["setup_s"])
     log(f"ensemble: batched {batched['sims_per_s']:.2f} sims/s ({batched['wall_s']:.2f} s, "
-        f"tile {batched['block_threads']}); with admission {incl:.2f} sims/s")
+        f"tile {batched['block_threads']}, chunk {batched['chunk_steps']}); with admission {incl:.2f} sims/s")
     order = sorted(range(len(nets)), key=lambda i: sizes[i])
     pick = [order[int((k + 0.5) * len(order) / n_sample)] for k in range(n_sample)]
     sample = [nets[i] for i in pick]
@@ -329,7 +330,8 @@ def ensemble_leg(pre_ms: float = 100.0, sim_ms: float = 500.0, n_sample: int = 8
         "steps_per_sim": steps, "precision": "fp32c",
         "batched_sims_per_s": batched["sims_per_s"], "batched_wall_s": batched["wall_s"],
         "batched_setup_s": batched["setup_s"], "batched_sims_per_s_with_admission": incl,
-        "tile_threads": batched["block_threads"], "graph_blocks": batched["graph_blocks"],
+        "tile_threads": batched["block_threads"], "chunk_steps": batched["chunk_steps"],
+        "block_steps": batched["block_steps"], "graph_blocks": batched["graph_blocks"],
         "network_build_s": build_s, "network_setup": "gpu",
         "bucket_overflows": batched["bucket_overflows"], "device_bytes": batched["device_bytes"],
         "one_at_a_time_plain_fp64_sims_per_s": plain["sims_per_s"],
@@ -440,6 +442,47 @@ def add_roofline(net, leg: dict, workload: str) -> dict:
     return r
 
 
+def graph_gaps(trace_csv: str) -> int:
+    """Launch gaps of the graph replays in a `rocprofv3 --kernel-trace`
+    kernel_trace.csv: the replays are the span from the first to the last
+    tick_kernel (a kernel only captured graphs launch, one per block);
+    within it, the summed kernel durations against the span are the device
+    time a persistent kernel could at most take back. Prints one JSON line."""
+    import csv
+    rows = []
+    with open(trace_csv, newline="") as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+    ticks = [i for i, r in enumerate(rows) if "tick_kernel" in r[2]]
+    if len(ticks) < 2:
+        print(json.dumps({"error": "no graph replays (tick_kernel) in the trace"}))
+        return 1
+    sel = rows[ticks[0]:ticks[-1] + 1]
+    span = sel[-1][1] - sel[0][0]
+    busy, end = 0, sel[0][0]
+    for a, b, _ in sel:  # union of kernel intervals (kernels of one stream do not overlap)
+        a = max(a, end)
+        if b > a:
+            busy += b - a
+        end = max(end, b)
+    by = {}
+    for a, b, n in sel:
+        m = re.search(r"(\w+_kernel|__amd_rocclr_\w+)", n)
+        k = m.group(1) if m else n[:40]
+        by.setdefault(k, [0, 0])
+        by[k][0] += 1
+        by[k][1] += b - a
+    blocks = len(ticks)
+    print(json.dumps({
+        "blocks": blocks, "span_ms": span / 1e6, "kernel_busy_ms": busy / 1e6,
+        "gap_ms": (span - busy) / 1e6, "gap_fraction": (span - busy) / span,
+        "span_us_per_block": span / 1e3 / blocks, "gap_us_per_block": (span - busy) / 1e3 / blocks,
+        "kernels": {k: {"calls": c, "avg_us": t / 1e3 / c} for k, (c, t) in by.items()},
+    }))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -462,9 +505,13 @@ def main() -> int:
                     help="arrival window of the serving load (0 skips it)")
     ap.add_argument("--serving-blocks", type=int, default=16,
                     help="1 ms blocks the server advances between admission checks")
+    ap.add_argument("--graph-gaps", default="", metavar="CSV",
+                    help="only analyse the graph replays of a rocprofv3 kernel_trace.csv (no GPU use)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the known-size copy that HBM counter readings are scaled by")
     args = ap.parse_args()
+    if args.graph_gaps:
+        return graph_gaps(args.graph_gaps)
     if args.copy_calibration > 0:
         return copy_calibration(args.copy_calibration)
     if args.gpus != 1:

@@ -193,6 +193,7 @@ typedef struct as_batch_info {
     uint64_t device_bytes;
     uint64_t free_neurons;    /* largest free neuron range */
     uint64_t bucket_overflows;/* targets that fell back to a full in-row walk */
+    int32_t chunk_steps;      /* steps one integrate launch advances (LDS chunk) */
 } as_batch_info;
 
 /* Exact capacity for the given networks side by side. */
@@ -205,14 +206,24 @@ as_status as_batch_fits(as_batch* b, const as_network* net, int32_t* fits);
 /* Admits a network to run n_steps steps from the current global step (a
  * block boundary). Uploads it and builds its out-CSR on the batch's stream. */
 as_status as_batch_add(as_batch* b, const as_network* net, int64_t n_steps, int32_t* member);
+/* as_batch_add for n networks: the uploads run back to back and are waited
+ * for once. If one does not fit, the networks before it are admitted, the
+ * rest get member -1, and the error is returned. No counterpart in the
+ * reference (it runs one network per process). */
+as_status as_batch_add_many(as_batch* b, const as_network* const* nets, int32_t n, const int64_t* n_steps,
+                            int32_t* members);
 /* Retires a member; its ranges are reused once in-flight deliveries drain. */
 as_status as_batch_remove(as_batch* b, int32_t member);
 /* Advances all active members k steps (asynchronous; partial blocks allowed). */
 as_status as_batch_step(as_batch* b, int64_t k);
 /* Advances whole blocks by replaying a captured hipGraph (asynchronous). */
 as_status as_batch_run_graph(as_batch* b, int64_t blocks);
-/* Times each candidate workgroup size over `blocks_each` blocks of the live
- * run (the results do not depend on it) and keeps the fastest. */
+/* Times each candidate workgroup size, each with the longest LDS chunk its
+ * budget allows and with about half of it, over `blocks_each` blocks of the
+ * live run (the results depend on neither), and keeps the fastest pair;
+ * *chosen is the workgroup size (as_batch_info.chunk_steps the chunk). The
+ * block length is the delay in steps (the longest the delay allows: every
+ * emission is delivered by the next block). */
 as_status as_batch_autotune(as_batch* b, const int32_t* candidates, int32_t n, int32_t blocks_each,
                             int32_t* chosen);
 as_status as_batch_sync(as_batch* b);

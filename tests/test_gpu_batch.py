@@ -113,13 +113,33 @@ def test_chunked_graph_replays_match(nets, precision):
     e.close()
 
 
-def test_autotune_keeps_results(nets):
-    print("[batch] autotune", flush=True)
-    b = A.Batch(nets[:3])
-    ms = [b.add(n, 400) for n in nets[:3]]
-    chosen = b.autotune((64, 128, 256, 512), blocks_each=3)   # runs 4 x 3 blocks of the real run
+def test_add_many_matches_alone(nets):
+    # bulk admission: uploads back to back, one wait; members with their own step counts
+    print("[batch] add_many fp32c", flush=True)
+    steps = [300, 200, 300]
+    refs = [alone(n, k, "fp32c") for n, k in zip(nets[:3], steps)]
+    b = A.Batch(nets[:3], precision="fp32c")
+    ms = b.add_many(nets[:3], steps)
+    assert sorted(ms) == sorted(set(ms)) and all(m >= 0 for m in ms)
+    b.step(300)
+    for m, ref, k in zip(ms, refs, steps):
+        assert b.member_steps(m) == (k, k)
+        assert_member_equals(b, m, ref, f"add_many member {m}")
+    b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+def test_autotune_keeps_results(nets, precision):
+    # the tuner runs real blocks at each (workgroup size, LDS chunk) pair
+    print(f"[batch] autotune {precision}", flush=True)
+    b = A.Batch(nets[:3], precision=precision)
+    ms = [b.add(n, 600) for n in nets[:3]]
+    chosen = b.autotune((64, 128, 256, 512), blocks_each=3)
+    info = b.info()
     assert chosen in (64, 128, 256, 512)
-    b.step(400 - 120)
+    assert 1 <= info["chunk_steps"] <= info["block_steps"]
+    assert 0 < info["global_step"] <= 8 * 3 * info["block_steps"]
+    b.step(600 - info["global_step"])
     for m, n in zip(ms, nets[:3]):
-        assert_member_equals(b, m, alone(n, 400), f"autotuned ({chosen})")
+        assert_member_equals(b, m, alone(n, 600, precision), f"autotuned ({chosen}, chunk {info['chunk_steps']})")
     b.close()
