@@ -6,13 +6,16 @@
 // bit for bit over whole input ranges (subnormal and overflowing results,
 // signed zeros, quotients next to rounding midpoints). It also evaluates the
 // Gaussian draws: the FP64 one on both sides, and the FP32 one FP32 state
-// uses (device only), for a bound on their difference. And it runs the
+// uses (device only), for a bound on their difference; and the FP32 state's
+// AdEx spike current (device) against the same current in long double
+// (host), for its error in ulps. And it runs the
 // parallel connectivity generator of as_gpubuild_run.h over a host backend
 // (loops for kernels, std::stable_sort for the radix sort), so that its
 // method is checked against the sequential builder without a GPU.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <cstring>
 #include <numeric>
@@ -28,7 +31,28 @@
 
 namespace {
 
-enum Fn { kExp = 0, kLog = 1, kCos = 2, kDivide = 3, kDivideIeee = 4, kNormal = 5, kNormal32 = 6 };
+enum Fn { kExp = 0, kLog = 1, kCos = 2, kDivide = 3, kDivideIeee = 4, kNormal = 5, kNormal32 = 6, kSpike32 = 7,
+          kSpikeExact = 8 };
+
+// The default configuration's neuron constants of population y (0
+// excitatory, else inhibitory).
+const astrosim::NeuronConst& neuron_const(const astrosim::RunParams& rp, double y) { return y == 0.0 ? rp.ne : rp.ni; }
+
+astrosim::RunParams default_params() {
+    as_config c;
+    as_config_default(&c);
+    c.precision = AS_FP32C;
+    c.method = AS_METHOD_BLOCKED;
+    return astrosim::resolve_params(c);
+}
+
+__global__ void spike32_kernel(astrosim::NeuronConst32 ne, astrosim::NeuronConst32 ni, const double* __restrict__ x,
+                               const double* __restrict__ y, double* __restrict__ out, std::size_t n) {
+    const std::size_t i = blockIdx.x * std::size_t(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const astrosim::NeuronConst32& p = y[i] == 0.0 ? ne : ni;
+    out[i] = static_cast<double>(astrosim::spike_current32(p, static_cast<float>(x[i])));
+}
 
 __host__ __device__ inline double eval(int fn, double x, double y) {
     switch (fn) {
@@ -53,8 +77,21 @@ __global__ void eval_kernel(int fn, const double* __restrict__ x, const double* 
 // out[i] = f(x[i], y[i]) on the host. fn: 0 exp(x), 1 log(x), 2 cos(x),
 // 3 x / y by the fixed-divisor method (y > 0), 4 x / y, 5 the FP64 Gaussian
 // draw of the stream whose counter-0 finaliser input has the bits of x,
-// 6 the FP32 one (device only). Returns 0, or -1 for an fn this side lacks.
+// 6 the FP32 one (device only), 7 the FP32 state's spike current
+// g_L*Delta_T/C_m*exp((V - V_T)/Delta_T) at V = float(x) for population y
+// (device only), 8 the same in long double (host only). Returns 0, or -1
+// for an fn this side lacks.
 extern "C" int as_selftest_eval_host(int fn, const double* x, const double* y, double* out, std::size_t n) {
+    if (fn == kSpikeExact) {
+        const astrosim::RunParams rp = default_params();
+        for (std::size_t i = 0; i < n; ++i) {
+            const astrosim::NeuronConst& p = neuron_const(rp, y[i]);
+            const long double V = static_cast<float>(x[i]);
+            out[i] = static_cast<double>((long double)p.g_L * p.Delta_T / p.C_m *
+                                         expl((V - (long double)p.V_th) / (long double)p.Delta_T));
+        }
+        return 0;
+    }
     if (fn < kExp || fn > kNormal) return -1;
     for (std::size_t i = 0; i < n; ++i) out[i] = eval(fn, x[i], y[i]);
     return 0;
@@ -63,7 +100,7 @@ extern "C" int as_selftest_eval_host(int fn, const double* x, const double* y, d
 // The same on the current device (host arrays in and out). Returns 0, -1 for
 // an unknown fn, or the HIP error code.
 extern "C" int as_selftest_eval_device(int fn, const double* x, const double* y, double* out, std::size_t n) {
-    if (fn < kExp || fn > kNormal32) return -1;
+    if (fn < kExp || fn > kSpike32) return -1;
     if (n == 0) return 0;
     double *dx = nullptr, *dy = nullptr, *dout = nullptr;
     const std::size_t bytes = n * sizeof(double);
@@ -75,7 +112,12 @@ extern "C" int as_selftest_eval_device(int fn, const double* x, const double* y,
     if (e == hipSuccess) {
         const unsigned threads = 256;
         const unsigned blocks = static_cast<unsigned>((n + threads - 1) / threads);
-        hipLaunchKernelGGL(eval_kernel, dim3(blocks), dim3(threads), 0, 0, fn, dx, dy, dout, n);
+        if (fn == kSpike32) {
+            const astrosim::Params32 q = astrosim::make_params32(default_params());
+            hipLaunchKernelGGL(spike32_kernel, dim3(blocks), dim3(threads), 0, 0, q.ne, q.ni, dx, dy, dout, n);
+        } else {
+            hipLaunchKernelGGL(eval_kernel, dim3(blocks), dim3(threads), 0, 0, fn, dx, dy, dout, n);
+        }
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost);

@@ -11,7 +11,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parents[1]
 LIB = ROOT / "astrosimgpu_cuda_amd" / "lib" / "libastrosim_selftest.so"
 
-EXP, LOG, COS, DIVIDE, DIVIDE_IEEE, NORMAL, NORMAL32 = 0, 1, 2, 3, 4, 5, 6
+EXP, LOG, COS, DIVIDE, DIVIDE_IEEE, NORMAL, NORMAL32, SPIKE32, SPIKE_EXACT = 0, 1, 2, 3, 4, 5, 6, 7, 8
 
 # The fixed divisors the FP64 step uses with the default biology
 # (C_m, tau_w and Delta_T of both populations, ratio_ER_cyt, the STP
