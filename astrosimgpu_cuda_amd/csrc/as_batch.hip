@@ -117,11 +117,26 @@ struct BlkLaunch {
     int sub;           // graph replays: first step of the launch within its block
 };
 
+// A value every lane of the wave holds, moved to scalar registers. Loads
+// through pointers the kernel may also write (the graph clock, the tile and
+// member tables) are vector loads, so their results, and everything computed
+// from them (steps, stream bases, 64-bit divisions), would otherwise occupy
+// vector registers and vector-instruction issue.
+__device__ __forceinline__ std::uint32_t uniform(std::uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ std::uint64_t uniform(std::uint64_t x) {
+    const std::uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<std::uint32_t>(x));
+    const std::uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<std::uint32_t>(x >> 32));
+    return (std::uint64_t(hi) << 32) | lo;
+}
+__device__ __forceinline__ std::int64_t uniform(std::int64_t x) {
+    return static_cast<std::int64_t>(uniform(static_cast<std::uint64_t>(x)));
+}
+
 // Graph replays read the block index from device memory instead of the
 // launch arguments: the launch is a whole block.
 __device__ __forceinline__ void resolve_launch(BlkLaunch& B, const std::int64_t* clock) {
     if (!clock) return;
-    const std::int64_t b = *clock;
+    const std::int64_t b = uniform(*clock);
     B.bstart = b * B.L;
     B.t0 = B.bstart + B.sub;
     B.n = B.L - B.sub < B.chunk ? B.L - B.sub : B.chunk;
@@ -529,6 +544,16 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
                            std::int64_t hi, R* col, unsigned stride) {
     const std::uint32_t oa = ga * std::uint32_t(sizeof(R)), oa4 = ga * 4u;
     const unsigned C = static_cast<unsigned>(B.chunk);
+    // state first: the loads' latency passes during the prologue
+    R ca = at(S.Ca, oa), ip3 = at(S.IP3, oa), hh = at(S.h, oa);
+    R ca_lo = R(0), ip3_lo = R(0), hh_lo = R(0);
+    if constexpr (sizeof(R) == 4) {
+        ca_lo = at(S.Ca_lo, oa);
+        ip3_lo = at(S.IP3_lo, oa);
+        hh_lo = at(S.h_lo, oa);
+    }
+    const R Ca_tot = at(S.Ca_tot, oa), IP3_0 = at(S.IP3_0, oa), tau = at(S.tau_IP3, oa), dIP3 = at(S.delta_IP3, oa);
+    std::uint32_t wout = B.first ? 0u : at(D.sic_w + std::size_t(B.wslot) * D.acap, oa4);
     {
         const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
         uint4* ent = D.bent[T_E2A] + std::size_t(ga) * kBucketCap;
@@ -568,17 +593,8 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
         }
     }
     std::uint32_t pr = ring_row(lo, B.Rv);
-    R ca = at(S.Ca, oa), ip3 = at(S.IP3, oa), hh = at(S.h, oa);
-    R ca_lo = R(0), ip3_lo = R(0), hh_lo = R(0);
-    if constexpr (sizeof(R) == 4) {
-        ca_lo = at(S.Ca_lo, oa);
-        ip3_lo = at(S.IP3_lo, oa);
-        hh_lo = at(S.h_lo, oa);
-    }
-    const R Ca_tot = at(S.Ca_tot, oa), IP3_0 = at(S.IP3_0, oa), tau = at(S.tau_IP3, oa), dIP3 = at(S.delta_IP3, oa);
     FixedDivisor tau_div{};  // FP64: the cell's tau_IP3 divisor, made once per launch
     if constexpr (sizeof(R) == 8) tau_div = make_fixed_divisor(tau);
-    std::uint32_t wout = B.first ? 0u : at(D.sic_w + std::size_t(B.wslot) * D.acap, oa4);
     bool ever = false;
     for (std::int64_t t = lo; t < hi; ++t) {
         const unsigned k = static_cast<unsigned>(t - B.bstart), j = static_cast<unsigned>(t - lo);
@@ -632,6 +648,15 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     const double nstd = exc ? P.noise_exc : P.noise_inh;
     const std::int64_t tl0 = lo - M.step0;  // the member's own step at lo
     const std::uint64_t nidx0 = static_cast<std::uint64_t>(tl0 / P.steps_per_noise);
+    // The cell's state is loaded first: the loads' latency then passes during
+    // the prologue (nothing the prologue writes aliases the state).
+    R V = at(S.V, on), w = at(S.w, on), g_ex = at(S.g_ex, on), dg_ex = at(S.dg_ex, on), g_in = at(S.g_in, on),
+      dg_in = at(S.dg_in, on);
+    int refr = at(D.refr, o4);
+    R I_sic = at(S.I_sic, on);
+    const R V_reset = at(S.V_reset, on), bb = at(S.b, on);
+    std::uint32_t wout = B.first ? 0u : at(D.spk_w + std::size_t(B.wslot) * D.ncap, o4);
+    const unsigned cs = at(D.bcnt[T_SIC], o4);
     {
         const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
         uint4* ee = D.bent[T_EXC] + std::size_t(gj) * kBucketCap;
@@ -729,7 +754,6 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         gather_chunk<R>(col + C * stride, stride, k0, k1, ei, ci, nwords, nring, [&] { return D.in_rs[T_INH][gj]; },
                         D.in_src[T_INH]);
         if constexpr (kSicInLds) {
-            const unsigned cs = at(D.bcnt[T_SIC], o4);
             sort_bucket(es(), cs);
             gather_chunk<R>(col + 2 * C * stride, stride, k0, k1, es(), cs, awords, sring,
                             [&] { return D.in_rs[T_SIC][gj]; }, D.in_src[T_SIC]);
@@ -762,18 +786,12 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     InputStream<R, 1> is;
     std::uint32_t vr = ring_row(lo - B.d, B.Rv);
     if constexpr (!kSicInLds) {
-        input_init(is, es(), at(D.bcnt[T_SIC], o4), awords);
+        input_init(is, es(), cs, awords);
         input_prefetch(is, static_cast<unsigned>(lo - B.bstart), S.sic_v + std::size_t(vr) * D.acap);
     }
     std::uint32_t pr = ring_row(lo, B.Rv);
-    R V = at(S.V, on), w = at(S.w, on), g_ex = at(S.g_ex, on), dg_ex = at(S.dg_ex, on), g_in = at(S.g_in, on),
-      dg_in = at(S.dg_in, on);
-    int refr = at(D.refr, o4);
-    R I_sic = at(S.I_sic, on);
-    const R V_reset = at(S.V_reset, on), bb = at(S.b, on);
     R noise_val = R(0);
     unsigned nspikes = 0;
-    std::uint32_t wout = B.first ? 0u : at(D.spk_w + std::size_t(B.wslot) * D.ncap, o4);
     // The SIC-update phase (an update at tl >= d_a2n with (tl - d_a2n) %
     // sic_interval == 0) and the noise index with its remainder, advanced
     // per step.
@@ -920,8 +938,19 @@ __global__ __launch_bounds__(MAXT, kIntegrateWaves<R>) void integrate_kernel(con
     const RunParams& P = ctx->P;
     const Params32& Q = ctx->Q;
     resolve_launch(B, D.clock);
-    const Tile tile = D.tiles[blockIdx.x];
-    const Member M = D.members[tile.member];
+    // the tile and its member are the same for the whole workgroup
+    Tile tile = D.tiles[blockIdx.x];
+    tile = Tile{uniform(tile.member), uniform(tile.first), uniform(tile.kind), uniform(tile.end)};
+    Member M = D.members[tile.member];
+    M.seed = uniform(M.seed);
+    M.nb = uniform(M.nb);
+    M.nn = uniform(M.nn);
+    M.ne = uniform(M.ne);
+    M.ab = uniform(M.ab);
+    M.na = uniform(M.na);
+    M.step0 = uniform(M.step0);
+    M.step_end = uniform(M.step_end);
+    M.slot = uniform(M.slot);
     // the part of this launch's steps inside the member's run
     const std::int64_t lo = B.t0 > M.step0 ? B.t0 : M.step0;
     const std::int64_t end = B.t0 + B.n;

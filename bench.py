@@ -494,7 +494,10 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--legs", default="fp64:plain,fp64:blocked,fp32c:blocked")
     ap.add_argument("--cpu-steps", type=int, default=40)
-    ap.add_argument("--block", type=int, default=128, help="threads per workgroup (the CLI's --block)")
+    ap.add_argument("--block", type=int, default=128,
+                    help="threads per workgroup of the plain method (the CLI's --block)")
+    ap.add_argument("--tile", type=int, default=64,
+                    help="threads per workgroup (tile) of the blocked method: 64 measured fastest at N=10M")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
@@ -531,7 +534,7 @@ def main() -> int:
 
     legs = [tuple(x.split(":")) for x in args.legs.split(",")]
     results = [run_leg(A, net, p, m, args.plain_steps if (m == "plain" and args.plain_steps > 0) else args.steps,
-                       args.warmup, args.block, deg)
+                       args.warmup, args.tile if m == "blocked" else args.block, deg)
                for p, m in legs]
     workload = f"n{args.n}_seed{args.seed}"
     for leg in results:
