@@ -43,6 +43,9 @@ namespace astrosim {
 namespace {
 
 constexpr unsigned kBucketCap = 8;
+// The chunk length the integrate kernel is also compiled for: the whole
+// 1 ms block at dt = 0.1 ms.
+constexpr int kDefaultChunk = 10;
 constexpr int kCountersPerMember = 8;
 enum { T_EXC = 0, T_INH = 1, T_SIC = 2, T_E2A = 3 };
 
@@ -115,6 +118,7 @@ struct BlkLaunch {
     int first, last;   // the launch starts / completes its block
     int chunk;         // most steps one integrate launch advances (LDS staging)
     int sub;           // graph replays: first step of the launch within its block
+    int lane_words;    // LDS words (of the state type) of each lane's staging column
 };
 
 // A value every lane of the wave holds, moved to scalar registers. Loads
@@ -289,16 +293,40 @@ __device__ __forceinline__ std::uint32_t step_mask(unsigned c0, unsigned c1) {
     return hi & ~((1u << c0) - 1u);
 }
 
+// Each lane stages its inputs in its own contiguous run of LDS words (an odd
+// number of 32-bit words apart, so a wave's accesses to one slot fall in
+// distinct banks). With the launch's chunk length KC known at compile time
+// (0: taken from the launch), loops over a launch's steps unroll and every
+// slot is an immediate offset from the lane's base.
+template <int KC>
+__device__ __forceinline__ unsigned chunk_of(const BlkLaunch& B) {
+    return KC > 0 ? unsigned(KC) : static_cast<unsigned>(B.chunk);
+}
+
+// f(i) for i in [0, n), n <= the chunk length (uniform).
+template <int KC, class F>
+__device__ __forceinline__ void for_steps(unsigned n, F f) {
+    if constexpr (KC > 0) {
+#pragma unroll
+        for (unsigned i = 0; i < unsigned(KC); ++i) {
+            if (i >= n) break;
+            f(i);
+        }
+    } else {
+        for (unsigned i = 0; i < n; ++i) f(i);
+    }
+}
+
 // Input sums of one target and connection type for block steps [c0, c1),
-// into the lane's LDS column col[(k - c0) * stride], in ascending in-CSR
-// slot order: the bucket is sorted; an attached entry adds its value at its
-// step, an unattached one the source's ring values at its emitted steps, and
-// an overflowed bucket walks the whole in-row. ring(k): the source
+// into the lane's LDS slots col[k - c0], in ascending in-CSR slot order: the
+// bucket is sorted; an attached entry adds its value at its step, an
+// unattached one the source's ring values at its emitted steps, and an
+// overflowed bucket walks the whole in-row. ring(k): the source
 // population's value-ring row of block step k.
-template <class R, class RingF, class RowF>
-__device__ void gather_chunk(R* col, unsigned stride, unsigned c0, unsigned c1, const uint4* ent, unsigned cnt,
+template <int KC, class R, class RingF, class RowF>
+__device__ void gather_chunk(R* col, unsigned c0, unsigned c1, const uint4* ent, unsigned cnt,
                              const std::uint32_t* words, RingF ring, RowF row, const index_t* src) {
-    for (unsigned k = c0; k < c1; ++k) col[(k - c0) * stride] = R(0);
+    for_steps<KC>(c1 - c0, [&](unsigned i) { col[i] = R(0); });
     if (cnt == 0u) return;
     const std::uint32_t mask = step_mask(c0, c1);
     if (cnt <= kBucketCap) {
@@ -306,12 +334,12 @@ __device__ void gather_chunk(R* col, unsigned stride, unsigned c0, unsigned c1, 
             const uint4 e = ent[r];
             if (e.y & kAttached) {
                 const unsigned k = attached_step(e);
-                if ((mask >> k) & 1u) col[(k - c0) * stride] += attached_value<R>(e);
+                if ((mask >> k) & 1u) col[k - c0] += attached_value<R>(e);
             } else {
                 const index_t q = e.y & kSrcMask;
                 for (std::uint32_t w = words[q] & mask; w != 0u; w &= w - 1u) {
                     const unsigned k = __ffs(static_cast<int>(w)) - 1;
-                    col[(k - c0) * stride] += ring(k)[q];
+                    col[k - c0] += ring(k)[q];
                 }
             }
         }
@@ -321,7 +349,7 @@ __device__ void gather_chunk(R* col, unsigned stride, unsigned c0, unsigned c1, 
             const index_t q = src[e];
             for (std::uint32_t w = words[q] & mask; w != 0u; w &= w - 1u) {
                 const unsigned k = __ffs(static_cast<int>(w)) - 1;
-                col[(k - c0) * stride] += ring(k)[q];
+                col[k - c0] += ring(k)[q];
             }
         }
     }
@@ -528,8 +556,8 @@ __device__ __forceinline__ FixedDivisor in_vgpr(const FixedDivisor& d) {
 // (src/astrosim_fused.cu:374-427). Returns whether it joins the emitter list.
 // Both cell functions advance one cell over global steps [lo, hi), at most
 // B.chunk of them, in two phases. The prologue computes everything that does
-// not depend on the cell's own state into the lane's LDS column col[i *
-// stride]: the delayed input sums (in slot order), the Poisson drive folded
+// not depend on the cell's own state into the lane's LDS slots col[i]: the
+// delayed input sums (in slot order), the Poisson drive folded
 // in with the reference's operation order, and the noise draws. The step
 // loop then reads those and runs the arithmetic that does; keeping the
 // prologue's random-number and gather code out of that loop keeps its
@@ -538,12 +566,13 @@ __device__ __forceinline__ FixedDivisor in_vgpr(const FixedDivisor& d) {
 // One astrocyte (global id ga, local id a) (src/astrosim_fused.cu:374-427).
 // Returns whether it joins the emitter list. col: [0, C) the IP3 input of
 // each step, [C, 2C) its noise term.
-template <class R>
+template <int KC, class R>
 __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
                            const Member& M, const BlkLaunch& B, index_t ga, index_t a, std::int64_t lo,
-                           std::int64_t hi, R* col, unsigned stride) {
+                           std::int64_t hi, R* col) {
     const std::uint32_t oa = ga * std::uint32_t(sizeof(R)), oa4 = ga * 4u;
-    const unsigned C = static_cast<unsigned>(B.chunk);
+    const unsigned C = chunk_of<KC>(B);
+    const unsigned n = static_cast<unsigned>(hi - lo);
     // state first: the loads' latency passes during the prologue
     R ca = at(S.Ca, oa), ip3 = at(S.IP3, oa), hh = at(S.h, oa);
     R ca_lo = R(0), ip3_lo = R(0), hh_lo = R(0);
@@ -560,16 +589,15 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
         const unsigned cnt = at(D.bcnt[T_E2A], oa4);
         sort_bucket(ent, cnt);
         auto nring = [&](unsigned k) { return S.spk_v + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.ncap; };
-        gather_chunk<R>(col, stride, static_cast<unsigned>(lo - B.bstart), static_cast<unsigned>(hi - B.bstart), ent,
-                        cnt, nwords, nring, [&] { return D.in_rs[T_E2A][ga]; }, D.in_src[T_E2A]);
+        gather_chunk<KC, R>(col, static_cast<unsigned>(lo - B.bstart), static_cast<unsigned>(hi - B.bstart), ent,
+                            cnt, nwords, nring, [&] { return D.in_rs[T_E2A][ga]; }, D.in_src[T_E2A]);
         // random streams tl*kStepStride + a of the member's step tl = t - step0
         // (src/astrosim_fused.cu:393-405): their bases advance by a constant
         // per step; FP32 state draws the noise in FP32 from the same uniforms
         const std::uint64_t stream0 = std::uint64_t(lo - M.step0) * kStepStride + a;
         std::uint64_t zp = rng_base(M.seed ^ kKeyAstroPoisson, stream0);
         std::uint64_t zn = rng_base(M.seed ^ kKeyAstroNoise, stream0);
-        for (std::int64_t t = lo; t < hi; ++t) {
-            const unsigned j = static_cast<unsigned>(t - lo);
+        for (unsigned j = 0; j < n; ++j) {
             int ev = 0;
             if (P.poiss_astro) ev = rng_poisson_z(zp, P.lam_astro, P.lim_astro);
             R noise = R(0);
@@ -579,7 +607,7 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
             }
             zp += kStepStride * kRngStreamMul;
             zn += kStepStride * kRngStreamMul;
-            R& in = col[j * stride];
+            R& in = col[j];
             if constexpr (sizeof(R) == 8) {
                 double ip3_input = P.w_n2a * in;
                 if (ev > 0) ip3_input += P.w_poiss_astro * static_cast<double>(ev);
@@ -589,7 +617,7 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
                 if (ev > 0) ip3_input += Q.w_poiss_astro * static_cast<float>(ev);
                 in = ip3_input;
             }
-            col[(C + j) * stride] = noise;
+            col[C + j] = noise;
         }
     }
     std::uint32_t pr = ring_row(lo, B.Rv);
@@ -598,7 +626,7 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
     bool ever = false;
     for (std::int64_t t = lo; t < hi; ++t) {
         const unsigned k = static_cast<unsigned>(t - B.bstart), j = static_cast<unsigned>(t - lo);
-        const R ip3_input = col[j * stride], noise = col[(C + j) * stride];
+        const R ip3_input = col[j], noise = col[C + j];
         R factor;
         if constexpr (sizeof(R) == 8) {
             astro_advance(P.ac, Ca_tot, IP3_0, tau_div, dIP3, ip3_input, noise, P.h_step, P.h_sixth, P.substeps, ca,
@@ -634,12 +662,12 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
 // Returns the number of spikes. col: [0, C) the excitatory input of each
 // step, [C, 2C) the inhibitory input, FP32c only: [2C, 3C) the SIC current
 // of each step, then the noise values of the launch's noise indices.
-template <class R, bool exc>
+template <int KC, class R, bool exc>
 __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
                                 const Member& M, const BlkLaunch& B, index_t gj, index_t j, std::int64_t lo,
-                                std::int64_t hi, R* col, unsigned stride, bool* emit) {
+                                std::int64_t hi, R* col, bool* emit) {
     const std::uint32_t on = gj * std::uint32_t(sizeof(R)), o4 = gj * 4u;
-    const unsigned C = static_cast<unsigned>(B.chunk);
+    const unsigned C = chunk_of<KC>(B);
     constexpr bool kSicInLds = sizeof(R) == 4;
     const unsigned noise_base = (kSicInLds ? 3u : 2u) * C;
     const std::uint32_t* awords = D.sic_w + std::size_t(B.rslot) * D.acap;
@@ -666,8 +694,8 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         sort_bucket(ei, ci);
         auto nring = [&](unsigned k) { return S.spk_v + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.ncap; };
         const unsigned k0 = static_cast<unsigned>(lo - B.bstart), k1 = static_cast<unsigned>(hi - B.bstart);
-        gather_chunk<R>(col, stride, k0, k1, ee, ce, nwords, nring, [&] { return D.in_rs[T_EXC][gj]; },
-                        D.in_src[T_EXC]);
+        gather_chunk<KC, R>(col, k0, k1, ee, ce, nwords, nring, [&] { return D.in_rs[T_EXC][gj]; },
+                            D.in_src[T_EXC]);
         // The Poisson drive of each step (stream tl*kStepStride + j), folded
         // into its excitatory input as w_e*sum, then + w_poiss*events for
         // steps with events (src/astrosim_fused.cu:472-479).
@@ -677,17 +705,17 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         const std::uint64_t zstep = kStepStride * kRngStreamMul;  // stream base advance per step
         const std::uint64_t zp0 = rng_base(M.seed ^ kKeyNeuronPoisson, std::uint64_t(tl0) * kStepStride + j);
         auto add_events = [&](unsigned i, int ev) {
-            R& ein = col[i * stride];
+            R& ein = col[i];
             if constexpr (sizeof(R) == 8)
                 ein = ein + (exc ? P.w_poiss_exc : P.w_poiss_inh) * static_cast<double>(ev);
             else
                 ein = ein + (exc ? Q.w_poiss_exc : Q.w_poiss_inh) * static_cast<float>(ev);
         };
-        for (unsigned i = 0; i < n; ++i) {
-            R& ein = col[i * stride];
+        for_steps<KC>(n, [&](unsigned i) {
+            R& ein = col[i];
             if constexpr (sizeof(R) == 8) ein = P.w_e * ein;
             else ein = Q.w_e_abs * ein;
-        }
+        });
         if (poiss && lam > 30.0) {  // the normal approximation: one draw per step
             std::uint64_t z = zp0;
             for (unsigned i = 0; i < n; ++i, z += zstep) {
@@ -705,30 +733,31 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
             const std::uint64_t lim53 = static_cast<std::uint64_t>(lim * 9007199254740992.0);
             auto stash = [&](unsigned i, std::uint64_t m0) {
                 if constexpr (sizeof(R) == 8) {
-                    col[(C + i) * stride] = static_cast<double>(m0);  // exact: m0 < 2^53
+                    col[C + i] = static_cast<double>(m0);  // exact: m0 < 2^53
                 } else {  // the two halves' bits, in the inhibitory and SIC regions
-                    col[(C + i) * stride] = __uint_as_float(static_cast<std::uint32_t>(m0));
-                    col[(2 * C + i) * stride] = __uint_as_float(static_cast<std::uint32_t>(m0 >> 32));
+                    col[C + i] = __uint_as_float(static_cast<std::uint32_t>(m0));
+                    col[2 * C + i] = __uint_as_float(static_cast<std::uint32_t>(m0 >> 32));
                 }
             };
             auto unstash = [&](unsigned i) -> double {
                 if constexpr (sizeof(R) == 8) {
-                    return col[(C + i) * stride] * (1.0 / 9007199254740992.0);
+                    return col[C + i] * (1.0 / 9007199254740992.0);
                 } else {
-                    const std::uint64_t m0 = std::uint64_t(__float_as_uint(col[(C + i) * stride])) |
-                                             (std::uint64_t(__float_as_uint(col[(2 * C + i) * stride])) << 32);
+                    const std::uint64_t m0 = std::uint64_t(__float_as_uint(col[C + i])) |
+                                             (std::uint64_t(__float_as_uint(col[2 * C + i])) << 32);
                     return static_cast<double>(m0) * (1.0 / 9007199254740992.0);
                 }
             };
             std::uint32_t pending = 0u;
             std::uint64_t z = zp0;
-            for (unsigned i = 0; i < n; ++i, z += zstep) {
+            for_steps<KC>(n, [&](unsigned i) {
                 const std::uint64_t m0 = rng_mix(z) >> 11;
                 if (m0 > lim53) {
                     pending |= 1u << i;
                     stash(i, m0);
                 }
-            }
+                z += zstep;
+            });
             unsigned cur = 32u;  // the step being continued; 32: none
             double product = 0.0;
             int count = 0;
@@ -751,23 +780,23 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         }
         // inhibitory source ids are local indices used as neuron ids
         // (src/astrosim_fused.cu:459-462); the scatter has already resolved them
-        gather_chunk<R>(col + C * stride, stride, k0, k1, ei, ci, nwords, nring, [&] { return D.in_rs[T_INH][gj]; },
-                        D.in_src[T_INH]);
+        gather_chunk<KC, R>(col + C, k0, k1, ei, ci, nwords, nring, [&] { return D.in_rs[T_INH][gj]; },
+                            D.in_src[T_INH]);
         if constexpr (kSicInLds) {
             sort_bucket(es(), cs);
-            gather_chunk<R>(col + 2 * C * stride, stride, k0, k1, es(), cs, awords, sring,
-                            [&] { return D.in_rs[T_SIC][gj]; }, D.in_src[T_SIC]);
+            gather_chunk<KC, R>(col + 2 * C, k0, k1, es(), cs, awords, sring, [&] { return D.in_rs[T_SIC][gj]; },
+                                D.in_src[T_SIC]);
         }
-        for (unsigned i = 0; i < n; ++i) {
-            R& iin = col[(C + i) * stride];
+        for_steps<KC>(n, [&](unsigned i) {
+            R& iin = col[C + i];
             if constexpr (sizeof(R) == 8) {
                 iin = (-P.w_i) * iin;
             } else {
                 iin = Q.w_i_abs * iin;
-                R& sin_ = col[(2 * C + i) * stride];
+                R& sin_ = col[2 * C + i];
                 sin_ = Q.w_a2n * sin_;
             }
-        }
+        });
         // the noise values of the launch's noise indices tl / steps_per_noise
         // (stream idx*kNoiseStride + j, src/astrosim_fused.cu:496-500); FP32
         // state draws them in FP32 from the same uniforms
@@ -778,7 +807,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
                 R v;
                 if constexpr (sizeof(R) == 8) v = nstd * rng_normal_z(zn);
                 else v = static_cast<float>(nstd) * normal32(zn);
-                col[(noise_base + static_cast<unsigned>(idx - nidx0)) * stride] = v;
+                col[noise_base + static_cast<unsigned>(idx - nidx0)] = v;
             }
         }
     }
@@ -818,12 +847,12 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     for (std::int64_t t = lo; t < hi; ++t) {
         const unsigned k = static_cast<unsigned>(t - B.bstart), i = static_cast<unsigned>(t - lo);
         const std::int64_t tl = t - M.step0;  // the member's own step
-        const R exc_input = col[i * stride];
-        const R inh_input = col[(C + i) * stride];
+        const R exc_input = col[i];
+        const R inh_input = col[C + i];
         const bool sic_step = sic_wait == 0;
         sic_wait = sic_step ? P.sic_interval - 1 : sic_wait - 1;
         if constexpr (kSicInLds) {
-            if (sic_step) I_sic = col[(2 * C + i) * stride];
+            if (sic_step) I_sic = col[2 * C + i];
         } else {
             if (sic_step) {
                 const R sic_sum = input_sum(is, k, es, awords, S.sic_v + std::size_t(vr) * D.acap,
@@ -835,7 +864,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
             vr = vn;
         }
         if (nstd > 0.0 && (t == lo || nrem == 0))  // the noise stream changes every steps_per_noise steps
-            noise_val = col[(noise_base + nslot) * stride];
+            noise_val = col[noise_base + nslot];
         bool spiked;
         if constexpr (sizeof(R) == 8) {
             const NeuronConst& p = pc;
@@ -928,11 +957,13 @@ struct IntegrateCtx {
 template <class R>
 constexpr int kIntegrateWaves = sizeof(R) == 8 ? AS_INTEGRATE_WAVES_F64 : AS_INTEGRATE_WAVES_F32;
 
-template <int MAXT, class R>
+// KC: the launch's chunk length when it is compiled in (the default block of
+// 10 steps), 0 otherwise.
+template <int MAXT, class R, int KC>
 __global__ __launch_bounds__(MAXT, kIntegrateWaves<R>) void integrate_kernel(const IntegrateCtx<R>* __restrict__ ctx, BlkLaunch B) {
-    // each lane's column of staged input sums: [2 types][B.chunk][blockDim.x]
+    // each lane's staging slots: B.lane_words consecutive words of R
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    R* col = reinterpret_cast<R*>(lds_raw) + threadIdx.x;
+    R* col = reinterpret_cast<R*>(lds_raw) + threadIdx.x * unsigned(B.lane_words);
     const BatchDev& D = ctx->D;
     const BlkState<R>& S = ctx->S;
     const RunParams& P = ctx->P;
@@ -958,7 +989,7 @@ __global__ __launch_bounds__(MAXT, kIntegrateWaves<R>) void integrate_kernel(con
     const index_t local = tile.first + threadIdx.x;
     if (tile.kind == 0) {
         const bool run = local < tile.end && lo < hi;
-        const bool emit = run ? astro_cell<R>(D, S, P, Q, M, B, M.ab + local, local, lo, hi, col, blockDim.x) : false;
+        const bool emit = run ? astro_cell<KC, R>(D, S, P, Q, M, B, M.ab + local, local, lo, hi, col) : false;
         wave_append(emit, M.ab + local, D.alist + std::size_t(B.wslot) * D.acap, &D.lcount[B.wslot * 2 + 1]);
         return;
     }
@@ -968,9 +999,9 @@ __global__ __launch_bounds__(MAXT, kIntegrateWaves<R>) void integrate_kernel(con
     if (run) {
         // one instantiation per population keeps only its constants live
         if (tile.kind == 1)
-            nspikes = neuron_cell<R, true>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, col, blockDim.x, &emit);
+            nspikes = neuron_cell<KC, R, true>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, col, &emit);
         else
-            nspikes = neuron_cell<R, false>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, col, blockDim.x, &emit);
+            nspikes = neuron_cell<KC, R, false>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, col, &emit);
     }
     wave_append(emit, M.nb + local, D.nlist + std::size_t(B.wslot) * D.ncap, &D.lcount[B.wslot * 2 + 0]);
     // diagnostic spike total of the member: one atomic per wave with spikes
@@ -1758,6 +1789,7 @@ private:
         B.first = 1;
         B.last = 1;
         B.chunk = chunk_len();
+        B.lane_words = lane_words(B.chunk);
         return B;
     }
 
@@ -1767,23 +1799,37 @@ private:
     int lds_words(int c) const {
         return (sizeof(R) == 4 ? 3 : 2) * c + (c - 1) / P_.steps_per_noise + 2;
     }
+    // A lane's slots in LDS, rounded up to an odd number of 32-bit words
+    // (FP64: of 64-bit words) so that a wave's accesses to one slot hit
+    // distinct banks.
+    int lane_words(int c) const { return lds_words(c) | 1; }
     // Most steps per integrate launch: the staged words fit 208 bytes per
     // lane (3 resident waves per SIMD share 160 KB) and 64 KB per workgroup.
     int chunk_len() const {
         const int per_lane = std::min(208, 65536 / threads_);
         int c = chunk_cap_ > 0 ? std::min(chunk_cap_, geo_.L) : geo_.L;
-        while (c > 1 && lds_words(c) * int(sizeof(R)) > per_lane) --c;
+        while (c > 1 && lane_words(c) * int(sizeof(R)) > per_lane) --c;
         return c;
     }
 
     void launch_integrate(const BatchDev& D, const BlkLaunch& B) {
         if (n_tiles_ == 0) return;
         const IntegrateCtx<R>* ctx = D.clock ? ctx_graph_ : ctx_eager_;
-        const std::size_t lds = std::size_t(lds_words(B.chunk)) * std::size_t(threads_) * sizeof(R);
-        if (threads_ <= 256)
-            hipLaunchKernelGGL((integrate_kernel<256, R>), dim3(n_tiles_), dim3(threads_), lds, stream_, ctx, B);
-        else
-            hipLaunchKernelGGL((integrate_kernel<1024, R>), dim3(n_tiles_), dim3(threads_), lds, stream_, ctx, B);
+        const std::size_t lds = std::size_t(B.lane_words) * std::size_t(threads_) * sizeof(R);
+        if (threads_ <= 256) {
+            if (B.chunk == kDefaultChunk)
+                hipLaunchKernelGGL((integrate_kernel<256, R, kDefaultChunk>), dim3(n_tiles_), dim3(threads_), lds,
+                                   stream_, ctx, B);
+            else
+                hipLaunchKernelGGL((integrate_kernel<256, R, 0>), dim3(n_tiles_), dim3(threads_), lds, stream_, ctx, B);
+        } else {
+            if (B.chunk == kDefaultChunk)
+                hipLaunchKernelGGL((integrate_kernel<1024, R, kDefaultChunk>), dim3(n_tiles_), dim3(threads_), lds,
+                                   stream_, ctx, B);
+            else
+                hipLaunchKernelGGL((integrate_kernel<1024, R, 0>), dim3(n_tiles_), dim3(threads_), lds, stream_, ctx,
+                                   B);
+        }
     }
 
     void timing_begin() {
