@@ -91,7 +91,7 @@ class _RequestResult(C.Structure):
     _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("n_neurons", C.c_uint32), ("n_astro", C.c_uint32),
                 ("t_submit", C.c_double), ("t_built", C.c_double), ("t_admit", C.c_double),
                 ("t_done", C.c_double), ("spikes", C.c_uint64), ("mean_ca", C.c_double),
-                ("active_astro", C.c_uint32)]
+                ("active_astro", C.c_uint32), ("t_build_start", C.c_double), ("built_on_gpu", C.c_int32)]
 
 
 class _ServerStats(C.Structure):
@@ -99,7 +99,8 @@ class _ServerStats(C.Structure):
                 ("p50_latency_s", C.c_double), ("p99_latency_s", C.c_double), ("mean_latency_s", C.c_double),
                 ("sims_per_s", C.c_double), ("device_bytes", C.c_uint64), ("peak_active", C.c_uint64),
                 ("peak_resident_cells", C.c_uint64), ("capacity_cells", C.c_uint64), ("gpu_busy_s", C.c_double),
-                ("cell_steps", C.c_double), ("admit_s", C.c_double), ("retire_s", C.c_double)]
+                ("cell_steps", C.c_double), ("admit_s", C.c_double), ("retire_s", C.c_double),
+                ("dedicated_queues", C.c_int32), ("build_streams", C.c_int32)]
 
 
 _lib = None

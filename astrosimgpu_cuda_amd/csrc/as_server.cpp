@@ -11,14 +11,19 @@
 // for a batch to drain.
 //
 // Streams and hardware queues: work on two HIP streams that share a hardware
-// queue runs in submission order, so a batch's blocks on a queue shared with
-// the network generator would wait for its kernels. The server therefore
-// creates no more streams than the process has queues (GPU_MAX_HW_QUEUES,
-// HIP's default 4): one block stream per batch from 5 queues on, else one
-// block stream for both (the batches' rounds then run one after the other),
-// one upload stream, and the rest as build streams (at most 8), each held by
-// one builder at a time. At 4 queues that is 1 + 1 + 2; two build streams
-// served the measured load faster than two block streams with one.
+// queue runs in submission order, so a build on a queue shared with a
+// batch's blocks would wait for each round. Plain HIP streams share the
+// process's GPU_MAX_HW_QUEUES queues (default 4) with every other stream the
+// process has created, the least-used queue first, so which streams meet
+// depends on what ran before the server. The server therefore creates its
+// streams with a CU mask covering every CU: HIP gives such a stream a
+// hardware queue of its own. Then each batch has a block stream, one upload
+// stream serves both batches, and up to 8 build streams are each held by one
+// builder at a time. If a masked stream cannot be created, the server falls
+// back to plain streams and creates no more of them than the process has
+// queues: one block stream per batch from 5 queues on, else one for both
+// (the batches' rounds then run one after the other), one upload stream, and
+// the rest as build streams (at 4 queues: 1 + 1 + 2).
 // A builder that finds no build stream free builds a network below
 // kGpuWaitMin cells with the host walk instead, and waits for a stream
 // otherwise (the host walk takes about 2 s at 1M cells). Host-built networks
@@ -110,6 +115,27 @@ void stage_pinned(Network& net, PinnedPool& pool) {
     net.pin_owner = std::shared_ptr<void>(p, [pp, got](void* q) { pp->give(q, got); });
 }
 
+// A stream on a hardware queue of its own (a full CU mask), or nullptr when
+// HIP refuses one.
+hipStream_t dedicated_stream() {
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+        prop.multiProcessorCount <= 0) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    const int cus = prop.multiProcessorCount;
+    std::vector<std::uint32_t> mask(std::size_t((cus + 31) / 32), 0xFFFFFFFFu);
+    if (cus % 32 != 0) mask.back() = (1u << (cus % 32)) - 1u;
+    hipStream_t s = nullptr;
+    if (hipExtStreamCreateWithCUMask(&s, static_cast<std::uint32_t>(mask.size()), mask.data()) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return s;
+}
+
 // The process's hardware queues per device: GPU_MAX_HW_QUEUES, else HIP's
 // default of 4.
 int hw_queues() {
@@ -180,15 +206,31 @@ public:
         // after the round that ends them (Batch::prefetch_final); the member
         // uploads of both batches (they build each member's out-CSR: device
         // work beside the blocks); the build streams.
-        const int queues = hw_queues();
-        const int n_block = queues >= 5 ? 2 : 1;
-        AS_HIP_CHECK(hipStreamCreateWithFlags(&stream_[0], hipStreamNonBlocking));
-        if (n_block == 2) AS_HIP_CHECK(hipStreamCreateWithFlags(&stream_[1], hipStreamNonBlocking));
-        else stream_[1] = stream_[0];
-        AS_HIP_CHECK(hipStreamCreateWithFlags(&io_, hipStreamNonBlocking));
-        const int n_build = std::min(8, std::max(1, queues - n_block - 1));
-        build_.assign(std::size_t(n_build), nullptr);
-        for (hipStream_t& b : build_) AS_HIP_CHECK(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
+        const int n_ded = 3 + std::min(8, std::max(1, c.build_threads));
+        std::vector<hipStream_t> ded;
+        for (int i = 0; i < n_ded; ++i) {
+            hipStream_t st = dedicated_stream();
+            if (!st) break;
+            ded.push_back(st);
+        }
+        if (int(ded.size()) == n_ded) {
+            dedicated_queues_ = true;
+            stream_[0] = ded[0];
+            stream_[1] = ded[1];
+            io_ = ded[2];
+            build_.assign(ded.begin() + 3, ded.end());
+        } else {
+            for (hipStream_t st : ded) (void)hipStreamDestroy(st);
+            const int queues = hw_queues();
+            const int n_block = queues >= 5 ? 2 : 1;
+            AS_HIP_CHECK(hipStreamCreateWithFlags(&stream_[0], hipStreamNonBlocking));
+            if (n_block == 2) AS_HIP_CHECK(hipStreamCreateWithFlags(&stream_[1], hipStreamNonBlocking));
+            else stream_[1] = stream_[0];
+            AS_HIP_CHECK(hipStreamCreateWithFlags(&io_, hipStreamNonBlocking));
+            const int n_build = std::min(8, std::max(1, queues - n_block - 1));
+            build_.assign(std::size_t(n_build), nullptr);
+            for (hipStream_t& b : build_) AS_HIP_CHECK(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
+        }
         free_build_ = build_;
         for (int i = 0; i < 2; ++i) {
             for (int k = 0; k < 2; ++k) {
@@ -307,6 +349,8 @@ public:
         s.admit_s = admit_s_;
         s.retire_s = retire_s_;
         s.cell_steps = cell_steps_;
+        s.dedicated_queues = dedicated_queues_ ? 1 : 0;
+        s.build_streams = static_cast<std::int32_t>(build_.size());
         return s;
     }
 
@@ -342,6 +386,7 @@ private:
                 if (stop_) return;
                 q = build_q_.front();
                 build_q_.pop_front();
+                q->out.t_build_start = now();
             }
             auto net = std::make_unique<Network>();
             bool ok = true;
@@ -368,6 +413,7 @@ private:
             if (q->out.status != AS_REQ_BUILDING) continue;
             q->out.n_neurons = ok ? net->cfg.n_neurons() : 0;
             q->out.n_astro = ok ? net->cfg.n_astro : 0;
+            q->out.built_on_gpu = ok && net->on_device() ? 1 : 0;
             q->out.t_built = now();
             if (!ok) {
                 q->out.status = AS_REQ_REJECTED;
@@ -594,6 +640,7 @@ private:
     hipStream_t io_ = nullptr;                     // member uploads of both batches; frees of built CSRs
     std::vector<hipStream_t> build_;               // the builders' device work, one builder per stream
     std::vector<hipStream_t> free_build_;          // build streams no builder holds
+    bool dedicated_queues_ = false;                // the streams have hardware queues of their own
     std::mutex stream_mu_;
     std::condition_variable cv_stream_;
     PinnedPool pinned_;  // host-built networks' CSRs until their upload

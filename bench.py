@@ -364,6 +364,7 @@ def serving_leg(duration_s: float, blocks_per_round: int = 16) -> dict:
     return {
         "arrival_rate_hz": 40.0, "arrival_window_s": duration_s, "requests": len(load),
         "blocks_per_round": blocks_per_round, "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+        "dedicated_queues": bool(st["dedicated_queues"]), "build_streams": st["build_streams"],
         "completed": st["completed"], "rejected": st["rejected"], "all_done": st["all_done"],
         "p50_latency_s": st["p50_latency_s"], "p99_latency_s": st["p99_latency_s"],
         "mean_latency_s": st["mean_latency_s"], "sims_per_s": st["sims_per_s"],

@@ -285,6 +285,8 @@ typedef struct as_request_result {
     uint64_t spikes;              /* spikes over the whole run */
     double mean_ca;               /* final mean calcium */
     uint32_t active_astro;        /* astrocytes above the SIC threshold at the end */
+    double t_build_start;         /* when a builder took the request (seconds since the server started) */
+    int32_t built_on_gpu;         /* 1: the GPU connectivity generator built the network; 0: the host walk */
 } as_request_result;
 typedef struct as_server_stats {
     uint64_t submitted, completed, rejected;
@@ -297,6 +299,8 @@ typedef struct as_server_stats {
                                      longer batch's span) */
     double cell_steps;            /* cells x steps of the completed requests */
     double admit_s, retire_s;     /* driver time admitting and retiring requests between rounds */
+    int32_t dedicated_queues;     /* 1: every server stream has a hardware queue of its own */
+    int32_t build_streams;        /* streams the network generator's builds run on */
 } as_server_stats;
 
 void as_server_config_default(as_server_config* cfg);
