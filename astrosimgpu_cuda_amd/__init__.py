@@ -581,9 +581,9 @@ class Server:
     with the GPU connectivity generator, kept in device memory; smaller ones
     by the host walk; ``setup_where`` as in include/astrosim.h), admitted into
     resident FP32c/FP64 batches as their footprint fits, advanced in rounds
-    of blocks pipelined two deep, and retired when done. The server creates
-    no more HIP streams than the process has hardware queues
-    (GPU_MAX_HW_QUEUES, HIP's default 4), so it needs no setting of it."""
+    of blocks pipelined two deep, and retired when done. Each server stream
+    has a hardware queue of its own (created with a CU mask covering every
+    CU), so it needs no GPU_MAX_HW_QUEUES setting."""
 
     def __init__(self, **kw):
         lib = load()
