@@ -252,13 +252,27 @@ __device__ __forceinline__ void load_sorted(uint4 (&e)[N], const uint4* ent, uns
     sort_entries<N>(e);
 }
 
-// Sorts a bucket of at most kBucketCap entries in place.
+// Sorts a bucket of at most kBucketCap entries in place, by selection in
+// memory: more than four entries are rare, and a register sorting network for
+// eight would hold 32 registers at the kernel's register peak (fewer resident
+// waves). In-CSR slots are distinct, so the order is the same either way.
 __device__ __forceinline__ void sort_in_place(uint4* ent, unsigned cnt) {
-    uint4 e[kBucketCap];
-    load_sorted<kBucketCap>(e, ent, cnt);
-#pragma unroll
-    for (int q = 0; q < int(kBucketCap); ++q)
-        if (unsigned(q) < cnt) ent[q] = e[q];
+    for (unsigned r = 0; r + 1u < cnt; ++r) {
+        unsigned m = r;
+        std::uint32_t km = ent[r].x;
+        for (unsigned q = r + 1u; q < cnt; ++q) {
+            const std::uint32_t k = ent[q].x;
+            if (k < km) {
+                km = k;
+                m = q;
+            }
+        }
+        if (m != r) {
+            const uint4 a = ent[r], b = ent[m];
+            ent[r] = b;
+            ent[m] = a;
+        }
+    }
 }
 
 template <class R>
@@ -660,19 +674,17 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
 
 // One neuron (global id gj, local id j) (src/astrosim_fused.cu:433-587).
 // Returns the number of spikes. col: [0, C) the excitatory input of each
-// step, [C, 2C) the inhibitory input, FP32c only: [2C, 3C) the SIC current
-// of each step, then the noise values of the launch's noise indices.
+// step, [C, 2C) the inhibitory input, then the noise values of the launch's
+// noise indices. SIC inputs stay in registers (values loaded a step ahead).
 template <int KC, class R, bool exc>
 __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
                                 const Member& M, const BlkLaunch& B, index_t gj, index_t j, std::int64_t lo,
                                 std::int64_t hi, R* col, bool* emit) {
     const std::uint32_t on = gj * std::uint32_t(sizeof(R)), o4 = gj * 4u;
     const unsigned C = chunk_of<KC>(B);
-    constexpr bool kSicInLds = sizeof(R) == 4;
-    const unsigned noise_base = (kSicInLds ? 3u : 2u) * C;
+    const unsigned noise_base = 2u * C;
     const std::uint32_t* awords = D.sic_w + std::size_t(B.rslot) * D.acap;
     auto es = [&] { return D.bent[T_SIC] + std::size_t(gj) * kBucketCap; };
-    auto sring = [&](unsigned k) { return S.sic_v + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.acap; };
     const double nstd = exc ? P.noise_exc : P.noise_inh;
     const std::int64_t tl0 = lo - M.step0;  // the member's own step at lo
     const std::uint64_t nidx0 = static_cast<std::uint64_t>(tl0 / P.steps_per_noise);
@@ -694,9 +706,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         sort_bucket(ei, ci);
         auto nring = [&](unsigned k) { return S.spk_v + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.ncap; };
         const unsigned k0 = static_cast<unsigned>(lo - B.bstart), k1 = static_cast<unsigned>(hi - B.bstart);
-        gather_chunk<KC, R>(col, k0, k1, ee, ce, nwords, nring, [&] { return D.in_rs[T_EXC][gj]; },
-                            D.in_src[T_EXC]);
-        // The Poisson drive of each step (stream tl*kStepStride + j), folded
+        // The Poisson drive of each step (stream tl*kStepStride + j) is folded
         // into its excitatory input as w_e*sum, then + w_poiss*events for
         // steps with events (src/astrosim_fused.cu:472-479).
         const unsigned n = static_cast<unsigned>(hi - lo);
@@ -704,50 +714,25 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         const double lam = exc ? P.lam_exc : P.lam_inh, lim = exc ? P.lim_exc : P.lim_inh;
         const std::uint64_t zstep = kStepStride * kRngStreamMul;  // stream base advance per step
         const std::uint64_t zp0 = rng_base(M.seed ^ kKeyNeuronPoisson, std::uint64_t(tl0) * kStepStride + j);
-        auto add_events = [&](unsigned i, int ev) {
-            R& ein = col[i];
-            if constexpr (sizeof(R) == 8)
-                ein = ein + (exc ? P.w_poiss_exc : P.w_poiss_inh) * static_cast<double>(ev);
-            else
-                ein = ein + (exc ? Q.w_poiss_exc : Q.w_poiss_inh) * static_cast<float>(ev);
-        };
-        for_steps<KC>(n, [&](unsigned i) {
-            R& ein = col[i];
-            if constexpr (sizeof(R) == 8) ein = P.w_e * ein;
-            else ein = Q.w_e_abs * ein;
-        });
-        if (poiss && lam > 30.0) {  // the normal approximation: one draw per step
-            std::uint64_t z = zp0;
-            for (unsigned i = 0; i < n; ++i, z += zstep) {
-                const int ev = rng_poisson_z(z, lam, lim);
-                if (ev > 0) add_events(i, ev);
+        // The event counts of every step with events, handed to record(i, ev).
+        // Knuth's method runs in two passes. First every step's first uniform
+        // u0 = m0 * 2^-53: most steps stop there with no event; the m0 of the
+        // others is kept in LDS slots of the step (stash/unstash). Then one
+        // loop continues those steps' products one draw per iteration, so a
+        // wave runs the maximum over its lanes of the launch's draws rather
+        // than the sum over steps of each step's maximum. Same draws, same
+        // products.
+        auto poisson = [&](auto stash, auto unstash, auto record) {
+            if (poiss && lam > 30.0) {  // the normal approximation: one draw per step
+                std::uint64_t z = zp0;
+                for (unsigned i = 0; i < n; ++i, z += zstep) {
+                    const int ev = rng_poisson_z(z, lam, lim);
+                    if (ev > 0) record(i, ev);
+                }
+                return;
             }
-        } else if (poiss) {
-            // Knuth's method in two passes. First every step's first uniform
-            // u0 = m0 * 2^-53: most steps stop there with no event; the m0
-            // of the others is kept in the (not yet gathered) inhibitory
-            // region of the column. Then one loop continues those steps'
-            // products one draw per iteration, so a wave runs the maximum
-            // over its lanes of the launch's draws rather than the sum over
-            // steps of each step's maximum. Same draws, same products.
+            if (!poiss) return;
             const std::uint64_t lim53 = static_cast<std::uint64_t>(lim * 9007199254740992.0);
-            auto stash = [&](unsigned i, std::uint64_t m0) {
-                if constexpr (sizeof(R) == 8) {
-                    col[C + i] = static_cast<double>(m0);  // exact: m0 < 2^53
-                } else {  // the two halves' bits, in the inhibitory and SIC regions
-                    col[C + i] = __uint_as_float(static_cast<std::uint32_t>(m0));
-                    col[2 * C + i] = __uint_as_float(static_cast<std::uint32_t>(m0 >> 32));
-                }
-            };
-            auto unstash = [&](unsigned i) -> double {
-                if constexpr (sizeof(R) == 8) {
-                    return col[C + i] * (1.0 / 9007199254740992.0);
-                } else {
-                    const std::uint64_t m0 = std::uint64_t(__float_as_uint(col[C + i])) |
-                                             (std::uint64_t(__float_as_uint(col[2 * C + i])) << 32);
-                    return static_cast<double>(m0) * (1.0 / 9007199254740992.0);
-                }
-            };
             std::uint32_t pending = 0u;
             std::uint64_t z = zp0;
             for_steps<KC>(n, [&](unsigned i) {
@@ -765,7 +750,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
                 if (cur == 32u) {
                     cur = static_cast<unsigned>(__builtin_ctz(pending));
                     pending &= pending - 1u;
-                    product = unstash(cur);
+                    product = unstash(cur) * (1.0 / 9007199254740992.0);
                     count = 0;
                     z = zp0 + std::uint64_t(cur) * zstep;  // counter 0 of step cur
                 }
@@ -773,29 +758,60 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
                 z += kRngCounterMul;
                 product *= rng_to_unit(rng_mix(z));
                 if (!(product > lim)) {
-                    add_events(cur, count);
+                    record(cur, count);
                     cur = 32u;
                 }
             }
+        };
+        if constexpr (sizeof(R) == 8) {
+            // FP64: excitatory sums first, the m0 of pending steps in their
+            // (not yet gathered) inhibitory slots, events added at once
+            gather_chunk<KC, R>(col, k0, k1, ee, ce, nwords, nring, [&] { return D.in_rs[T_EXC][gj]; },
+                                D.in_src[T_EXC]);
+            for_steps<KC>(n, [&](unsigned i) { col[i] = P.w_e * col[i]; });
+            poisson([&](unsigned i, std::uint64_t m0) { col[C + i] = static_cast<double>(m0); },  // exact: < 2^53
+                    [&](unsigned i) { return col[C + i]; },
+                    [&](unsigned i, int ev) {
+                        col[i] = col[i] + (exc ? P.w_poiss_exc : P.w_poiss_inh) * static_cast<double>(ev);
+                    });
+        } else {
+            // FP32c: the Poisson passes first, while both input regions are
+            // free: a pending step's m0 in its excitatory (low 32 bits) and
+            // inhibitory (high bits) slots, then its event count in its
+            // inhibitory slot (bit i of evmask: step i has events) until the
+            // excitatory sums are gathered. This keeps a lane's LDS to two
+            // words per step, which lets more waves share a CU.
+            std::uint32_t evmask = 0u;
+            poisson(
+                [&](unsigned i, std::uint64_t m0) {
+                    col[i] = __uint_as_float(static_cast<std::uint32_t>(m0));
+                    col[C + i] = __uint_as_float(static_cast<std::uint32_t>(m0 >> 32));
+                },
+                [&](unsigned i) {
+                    return static_cast<double>(std::uint64_t(__float_as_uint(col[i])) |
+                                               (std::uint64_t(__float_as_uint(col[C + i])) << 32));
+                },
+                [&](unsigned i, int ev) {
+                    col[C + i] = __int_as_float(ev);  // the step's slots are free again
+                    evmask |= 1u << i;
+                });
+            gather_chunk<KC, R>(col, k0, k1, ee, ce, nwords, nring, [&] { return D.in_rs[T_EXC][gj]; },
+                                D.in_src[T_EXC]);
+            for_steps<KC>(n, [&](unsigned i) {
+                float ein = Q.w_e_abs * col[i];
+                if ((evmask >> i) & 1u)
+                    ein = ein + (exc ? Q.w_poiss_exc : Q.w_poiss_inh) * static_cast<float>(__float_as_int(col[C + i]));
+                col[i] = ein;
+            });
         }
         // inhibitory source ids are local indices used as neuron ids
         // (src/astrosim_fused.cu:459-462); the scatter has already resolved them
         gather_chunk<KC, R>(col + C, k0, k1, ei, ci, nwords, nring, [&] { return D.in_rs[T_INH][gj]; },
                             D.in_src[T_INH]);
-        if constexpr (kSicInLds) {
-            sort_bucket(es(), cs);
-            gather_chunk<KC, R>(col + 2 * C, k0, k1, es(), cs, awords, sring, [&] { return D.in_rs[T_SIC][gj]; },
-                                D.in_src[T_SIC]);
-        }
         for_steps<KC>(n, [&](unsigned i) {
             R& iin = col[C + i];
-            if constexpr (sizeof(R) == 8) {
-                iin = (-P.w_i) * iin;
-            } else {
-                iin = Q.w_i_abs * iin;
-                R& sin_ = col[2 * C + i];
-                sin_ = Q.w_a2n * sin_;
-            }
+            if constexpr (sizeof(R) == 8) iin = (-P.w_i) * iin;
+            else iin = Q.w_i_abs * iin;
         });
         // the noise values of the launch's noise indices tl / steps_per_noise
         // (stream idx*kNoiseStride + j, src/astrosim_fused.cu:496-500); FP32
@@ -811,13 +827,11 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
             }
         }
     }
-    // FP64: SIC inputs from (source, word) pairs, values loaded a step ahead
+    // SIC inputs from (source, word) pairs, values loaded a step ahead
     InputStream<R, 1> is;
     std::uint32_t vr = ring_row(lo - B.d, B.Rv);
-    if constexpr (!kSicInLds) {
-        input_init(is, es(), cs, awords);
-        input_prefetch(is, static_cast<unsigned>(lo - B.bstart), S.sic_v + std::size_t(vr) * D.acap);
-    }
+    input_init(is, es(), cs, awords);
+    input_prefetch(is, static_cast<unsigned>(lo - B.bstart), S.sic_v + std::size_t(vr) * D.acap);
     std::uint32_t pr = ring_row(lo, B.Rv);
     R noise_val = R(0);
     unsigned nspikes = 0;
@@ -851,14 +865,13 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         const R inh_input = col[C + i];
         const bool sic_step = sic_wait == 0;
         sic_wait = sic_step ? P.sic_interval - 1 : sic_wait - 1;
-        if constexpr (kSicInLds) {
-            if (sic_step) I_sic = col[2 * C + i];
-        } else {
-            if (sic_step) {
-                const R sic_sum = input_sum(is, k, es, awords, S.sic_v + std::size_t(vr) * D.acap,
-                                            [&] { return D.in_rs[T_SIC][gj]; }, D.in_src[T_SIC]);
-                I_sic = P.w_a2n * sic_sum;
-            }
+        if (sic_step) {
+            const R sic_sum = input_sum(is, k, es, awords, S.sic_v + std::size_t(vr) * D.acap,
+                                        [&] { return D.in_rs[T_SIC][gj]; }, D.in_src[T_SIC]);
+            if constexpr (sizeof(R) == 8) I_sic = P.w_a2n * sic_sum;
+            else I_sic = Q.w_a2n * sic_sum;
+        }
+        {
             const std::uint32_t vn = ring_next(vr, B.Rv);
             if (t + 1 < hi) input_prefetch(is, k + 1u, S.sic_v + std::size_t(vn) * D.acap);
             vr = vn;
@@ -1794,10 +1807,10 @@ private:
     }
 
     // LDS words (of R) per lane that an integrate launch of c steps stages:
-    // excitatory and inhibitory inputs (FP32c also SIC currents) per step and
-    // the noise values of up to (c-1)/steps_per_noise + 2 noise indices.
+    // excitatory and inhibitory inputs per step and the noise values of up to
+    // (c-1)/steps_per_noise + 2 noise indices.
     int lds_words(int c) const {
-        return (sizeof(R) == 4 ? 3 : 2) * c + (c - 1) / P_.steps_per_noise + 2;
+        return 2 * c + (c - 1) / P_.steps_per_noise + 2;
     }
     // A lane's slots in LDS, rounded up to an odd number of 32-bit words
     // (FP64: of 64-bit words) so that a wave's accesses to one slot hit
