@@ -95,9 +95,12 @@ def build_library(incremental: bool = False, jobs: int = 8) -> None:
     cli_obj = _compile("as_cli.cpp", incremental)
     _run([HIPCC, "-o", str(CLI), str(cli_obj), "-L", str(LIB.parent), "-lastrosim",
           "-Wl,-rpath,$ORIGIN/../lib"])
-    # the vector-instruction throughput probe (a measurement tool, not linked to the library)
+    # the vector-instruction throughput and HBM streaming probes (measurement tools, not linked
+    # to the library)
     probe_obj = _compile("as_valu_probe.hip", incremental)
     _run([HIPCC, "-o", str(CLI.parent / "valu_probe"), str(probe_obj)])
+    bw_obj = _compile("as_bw_probe.hip", incremental)
+    _run([HIPCC, "-o", str(CLI.parent / "bw_probe"), str(bw_obj)])
 
 
 def build_oracle() -> None:
