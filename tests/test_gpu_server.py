@@ -41,9 +41,16 @@ def test_server_results_equal_solo_runs(blocks, n_hi, hbm, seed):
     st = out["stats"]
     print(f"[server] stats {st}", flush=True)
     assert st["all_done"] and st["completed"] == len(load) and st["rejected"] == 0
+    # every stream on a hardware queue of its own: two block streams, one
+    # upload stream and one build stream per builder (4)
+    assert st["dedicated_queues"] == 1 and st["build_streams"] == 4
+    n_gpu = sum(r["built_on_gpu"] for r in out["results"])
+    print(f"[server] {n_gpu} of {len(load)} networks generated on the GPU", flush=True)
+    if n_hi >= 100000:
+        assert n_gpu >= 1
     for req, res in zip(load, out["results"]):
         assert res["status"] == "done"
-        assert res["t_submit"] <= res["t_built"] <= res["t_admit"] <= res["t_done"]
+        assert res["t_submit"] <= res["t_build_start"] <= res["t_built"] <= res["t_admit"] <= res["t_done"]
         spikes, mean_ca, active = solo(req["n_total"], req["seed"], req["n_steps"], req["precision"])
         assert res["spikes"] == spikes, (req, res)
         # the calcium values are bit-identical; only the summation of the mean differs
