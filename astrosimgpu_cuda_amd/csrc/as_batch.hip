@@ -223,24 +223,17 @@ __device__ __forceinline__ void cas(uint4& a, uint4& b) {
     b = hi;
 }
 
-// Sorting networks (Batcher) for 1, 2, 4 and 8 entries.
+// Sorting networks (Batcher) for 1, 2 and 4 entries.
 template <int N>
 __device__ __forceinline__ void sort_entries(uint4 (&e)[N]) {
-    static_assert(N == 1 || N == 2 || N == 4 || N == 8, "network size");
+    static_assert(N == 1 || N == 2 || N == 4, "network size");
     if constexpr (N == 1) {
     } else if constexpr (N == 2) {
         cas(e[0], e[1]);
-    } else if constexpr (N == 4) {
+    } else {
         cas(e[0], e[1]); cas(e[2], e[3]);
         cas(e[0], e[2]); cas(e[1], e[3]);
         cas(e[1], e[2]);
-    } else {
-        cas(e[0], e[1]); cas(e[2], e[3]); cas(e[4], e[5]); cas(e[6], e[7]);
-        cas(e[0], e[2]); cas(e[1], e[3]); cas(e[4], e[6]); cas(e[5], e[7]);
-        cas(e[1], e[2]); cas(e[5], e[6]);
-        cas(e[0], e[4]); cas(e[1], e[5]); cas(e[2], e[6]); cas(e[3], e[7]);
-        cas(e[2], e[4]); cas(e[3], e[5]);
-        cas(e[1], e[2]); cas(e[3], e[4]); cas(e[5], e[6]);
     }
 }
 
