@@ -78,16 +78,14 @@ struct BatchDev {
     const index_t* in_src[4];
     const uint2* out_rs[4];  // (begin, end) per source: neurons (exc, inh, e2a), astrocytes (sic)
     const uint2* out_ent[4];  // (global target, global in-CSR slot)
-    unsigned* bcnt[4];
-    uint4* bent[4];  // [target][kBucketCap] bucket entries (see entry layout below)
+    // H bucket sets per type: set b % H holds the inputs block b reads
+    unsigned* bcnt[4];  // [H][targets]
+    uint4* bent[4];     // [H][targets][kBucketCap] bucket entries (see entry layout below)
     std::uint8_t* ever;
     std::int32_t* refr;
     std::uint32_t* spk_n;  // spikes per neuron since admission
     std::uint32_t* spk_w;  // [H][ncap]
     std::uint32_t* sic_w;  // [H][acap]
-    std::uint32_t* nlist;  // [H][ncap]
-    std::uint32_t* alist;  // [H][acap]
-    std::uint32_t* lcount; // [H][2]
     unsigned long long* counters;  // [members][kCountersPerMember]
     unsigned* overflow;
     const Member* members;
@@ -115,6 +113,7 @@ struct BlkLaunch {
     int n;             // steps in this launch
     int d, L, H, Rv;   // delay, block length, history blocks, value-ring depth
     std::uint32_t wslot, rslot;  // history slots of block b and block b - d/L
+    std::uint32_t bw;            // the bucket set of block b + d/L (b's emissions go there)
     int first, last;   // the launch starts / completes its block
     int chunk;         // most steps one integrate launch advances (LDS staging)
     int sub;           // graph replays: first step of the launch within its block
@@ -148,6 +147,7 @@ __device__ __forceinline__ void resolve_launch(BlkLaunch& B, const std::int64_t*
     B.last = B.sub + B.n == B.L;
     B.wslot = static_cast<std::uint32_t>(b % B.H);
     B.rslot = static_cast<std::uint32_t>(((b - B.d / B.L) % B.H + B.H) % B.H);
+    B.bw = static_cast<std::uint32_t>((b + B.d / B.L) % B.H);
 }
 
 __device__ __forceinline__ unsigned lane_id() { return threadIdx.x & 63u; }
@@ -175,16 +175,12 @@ __device__ __forceinline__ std::uint32_t ring_next(std::uint32_t r, int Rv) {
     return r + 1u == std::uint32_t(Rv) ? 0u : r + 1u;
 }
 
-// Appends `id` (when `want`) to a list with one atomic per wave.
-__device__ __forceinline__ void wave_append(bool want, std::uint32_t id, std::uint32_t* list,
-                                            std::uint32_t* count) {
-    const unsigned long long m = __ballot(want);
-    if (m == 0ull) return;
-    const unsigned leader = __ffsll((long long)m) - 1;
-    std::uint32_t base = 0;
-    if (lane_id() == leader) base = atomicAdd(count, (std::uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    if (want) list[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = id;
+// Bucket set s of connection type T (targets: neurons; astrocytes for T_E2A).
+__device__ __forceinline__ unsigned* bucket_counts(const BatchDev& D, int T, std::uint32_t s) {
+    return D.bcnt[T] + std::size_t(s) * (T == T_E2A ? D.acap : D.ncap);
+}
+__device__ __forceinline__ uint4* bucket_entries(const BatchDev& D, int T, std::uint32_t s) {
+    return D.bent[T] + std::size_t(s) * (T == T_E2A ? D.acap : D.ncap) * kBucketCap;
 }
 
 // ---------------------------------------------------------------------------
@@ -444,63 +440,65 @@ __device__ __forceinline__ R input_sum(const InputStream<R, NR>& s, unsigned k, 
 }
 
 // ---------------------------------------------------------------------------
-// SCATTER: emitters of block b - d/L append to their targets' buckets.
-// One wave per emitter, lanes over its out-edges.
+// SCATTER: at the end of its block, each wave appends the entries of its
+// emitters to their targets' buckets of block b + d/L (set B.bw), inside the
+// integrate launch that completes the block: no cell reads that set before
+// block b + d/L, and a separate launch per block would wait for the whole
+// integrate launch and then for its own tail.
 // ---------------------------------------------------------------------------
 constexpr unsigned kScatterUnroll = 4;
 
+// `word`: the lane's emission word of the block (0: no emission; a neuron's
+// spike steps, an astrocyte's SIC-active steps). The wave's emitters are taken
+// one at a time; the wave's lanes walk the emitter's out-rows.
 template <class R>
-__global__ __launch_bounds__(256) void scatter_kernel(BatchDev D, BlkLaunch B, const R* __restrict__ spk_v) {
-    resolve_launch(B, D.clock);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        D.lcount[B.wslot * 2 + 0] = 0;
-        D.lcount[B.wslot * 2 + 1] = 0;
-    }
-    const std::uint32_t nN = D.lcount[B.rslot * 2 + 0];
-    const std::uint32_t nA = D.lcount[B.rslot * 2 + 1];
-    const unsigned waves = gridDim.x * (blockDim.x / 64);
-    // wave-uniform (readfirstlane tells the compiler so): one emitter per wave
-    const unsigned wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
-    for (std::uint32_t i = wave; i < nN + nA; i += waves) {
-        const bool is_neuron = i < nN;
-        const std::uint32_t src = is_neuron ? D.nlist[std::size_t(B.rslot) * D.ncap + i]
-                                            : D.alist[std::size_t(B.rslot) * D.acap + (i - nN)];
-        // A neuron that spiked at one step k of its block hands (k, value)
-        // to its targets with the entry.
-        std::uint32_t tag = src, vlo = 0u, vhi = 0u;
-        if (is_neuron) {
-            const std::uint32_t w = D.spk_w[std::size_t(B.rslot) * D.ncap + src];
-            if (__popc(w) == 1) {
-                const unsigned k = __ffs(static_cast<int>(w)) - 1;
-                const R v = spk_v[std::size_t(ring_row(B.bstart - B.d + k, B.Rv)) * D.ncap + src];
-                tag |= kAttached | (k << kStepShift);
-                if constexpr (sizeof(R) == 8) {
-                    vlo = static_cast<std::uint32_t>(__double2loint(v));
-                    vhi = static_cast<std::uint32_t>(__double2hiint(v));
-                } else {
-                    vlo = __float_as_uint(v);
-                }
-            }
+__device__ void scatter_emitters(const BatchDev& D, const R* __restrict__ spk_v, const BlkLaunch& B,
+                                 bool is_neuron, std::uint32_t src, std::uint32_t word) {
+    std::uint64_t em = __ballot(word != 0u);
+    if (em == 0ull) return;
+    // A neuron that spiked at one step k of its block hands (k, value) to
+    // its targets with the entry.
+    std::uint32_t tag = src, vlo = 0u, vhi = 0u;
+    if (is_neuron && __popc(word) == 1) {
+        const unsigned k = __ffs(static_cast<int>(word)) - 1;
+        const R v = spk_v[std::size_t(ring_row(B.bstart + k, B.Rv)) * D.ncap + src];
+        tag |= kAttached | (k << kStepShift);
+        if constexpr (sizeof(R) == 8) {
+            vlo = static_cast<std::uint32_t>(__double2loint(v));
+            vhi = static_cast<std::uint32_t>(__double2hiint(v));
+        } else {
+            vlo = __float_as_uint(v);
         }
-        // A neuron walks its exc and n2a out-edges and the inhibitory rows
-        // whose stored LOCAL id equals its own local id (src/astrosim_fused.cu:
-        // 459-462); an astrocyte walks its a2n out-edges.
-        // The emitter's out-rows (a neuron's exc, inh and n2a rows; an
-        // astrocyte's a2n row) are walked as one sequence of edge slots, each
-        // lane appending kScatterUnroll slots per iteration: their loads, then
-        // their returning atomics, then their stores. An append is a chain of
-        // dependent memory operations; this keeps several chains in flight per
-        // lane and walks all rows in one round instead of one round per row.
-        // Bucket order does not matter: buckets are sorted by in-CSR slot when
-        // they are read.
-        const uint2* ent0 = is_neuron ? D.out_ent[T_EXC] : D.out_ent[T_SIC];
-        unsigned* cnt0 = is_neuron ? D.bcnt[T_EXC] : D.bcnt[T_SIC];
-        uint4* bkt0 = is_neuron ? D.bent[T_EXC] : D.bent[T_SIC];
-        const uint2 r0 = (is_neuron ? D.out_rs[T_EXC] : D.out_rs[T_SIC])[src];
+    }
+    // A neuron walks its exc and n2a out-edges and the inhibitory rows whose
+    // stored LOCAL id equals its own local id (src/astrosim_fused.cu:459-462);
+    // an astrocyte walks its a2n out-edges. The emitter's out-rows are walked
+    // as one sequence of edge slots, each lane appending kScatterUnroll slots
+    // per iteration: their loads, then their returning atomics, then their
+    // stores. An append is a chain of dependent memory operations; this keeps
+    // several chains in flight per lane and walks all rows in one round
+    // instead of one round per row. Bucket order does not matter: buckets are
+    // sorted by in-CSR slot when they are read.
+    const int T0 = is_neuron ? T_EXC : T_SIC;
+    const uint2* ent0 = D.out_ent[T0];
+    unsigned* cnt0 = bucket_counts(D, T0, B.bw);
+    uint4* bkt0 = bucket_entries(D, T0, B.bw);
+    unsigned* cnt1 = bucket_counts(D, T_INH, B.bw);
+    uint4* bkt1 = bucket_entries(D, T_INH, B.bw);
+    unsigned* cnt2 = bucket_counts(D, T_E2A, B.bw);
+    uint4* bkt2 = bucket_entries(D, T_E2A, B.bw);
+    while (em != 0ull) {
+        const unsigned l = static_cast<unsigned>(__builtin_ctzll(em));
+        em &= em - 1ull;
+        const std::uint32_t es = __builtin_amdgcn_readlane(src, l);
+        const std::uint32_t et = __builtin_amdgcn_readlane(tag, l);
+        const std::uint32_t elo = __builtin_amdgcn_readlane(vlo, l);
+        const std::uint32_t ehi = __builtin_amdgcn_readlane(vhi, l);
+        const uint2 r0 = D.out_rs[T0][es];
         uint2 r1 = make_uint2(0u, 0u), r2 = make_uint2(0u, 0u);
         if (is_neuron) {
-            r1 = D.out_rs[T_INH][src];
-            r2 = D.out_rs[T_E2A][src];
+            r1 = D.out_rs[T_INH][es];
+            r2 = D.out_rs[T_E2A][es];
         }
         const unsigned n0 = r0.y - r0.x, n01 = n0 + (r1.y - r1.x), n = n01 + (r2.y - r2.x);
         for (unsigned s0 = lane_id(); s0 < n; s0 += 64u * kScatterUnroll) {
@@ -519,15 +517,15 @@ __global__ __launch_bounds__(256) void scatter_kernel(BatchDev D, BlkLaunch B, c
 #pragma unroll
             for (unsigned u = 0; u < kScatterUnroll; ++u) {
                 if (s0 + 64u * u >= n) continue;
-                unsigned* cnt = ty[u] == 0 ? cnt0 : (ty[u] == 1 ? D.bcnt[T_INH] : D.bcnt[T_E2A]);
+                unsigned* cnt = ty[u] == 0 ? cnt0 : (ty[u] == 1 ? cnt1 : cnt2);
                 slot[u] = atomicAdd(&cnt[te[u].x], 1u);
             }
 #pragma unroll
             for (unsigned u = 0; u < kScatterUnroll; ++u) {
                 if (s0 + 64u * u >= n) continue;
-                uint4* bkt = ty[u] == 0 ? bkt0 : (ty[u] == 1 ? D.bent[T_INH] : D.bent[T_E2A]);
+                uint4* bkt = ty[u] == 0 ? bkt0 : (ty[u] == 1 ? bkt1 : bkt2);
                 if (slot[u] < kBucketCap)
-                    bkt[std::size_t(te[u].x) * kBucketCap + slot[u]] = make_uint4(te[u].y, tag, vlo, vhi);
+                    bkt[std::size_t(te[u].x) * kBucketCap + slot[u]] = make_uint4(te[u].y, et, elo, ehi);
                 else if (slot[u] == kBucketCap)
                     atomicAdd(D.overflow, 1u);
             }
@@ -559,8 +557,6 @@ __device__ __forceinline__ FixedDivisor in_vgpr(const FixedDivisor& d) {
     return FixedDivisor{in_vgpr(d.b), in_vgpr(d.yd), in_vgpr(d.yld), in_vgpr(d.ny)};
 }
 
-// One astrocyte (global id ga, local id a) over global steps [lo, hi)
-// (src/astrosim_fused.cu:374-427). Returns whether it joins the emitter list.
 // Both cell functions advance one cell over global steps [lo, hi), at most
 // B.chunk of them, in two phases. The prologue computes everything that does
 // not depend on the cell's own state into the lane's LDS slots col[i]: the
@@ -571,10 +567,11 @@ __device__ __forceinline__ FixedDivisor in_vgpr(const FixedDivisor& d) {
 // constants and temporaries out of the loop's registers.
 
 // One astrocyte (global id ga, local id a) (src/astrosim_fused.cu:374-427).
-// Returns whether it joins the emitter list. col: [0, C) the IP3 input of
-// each step, [C, 2C) its noise term.
+// Returns its emission word when the launch completes its block (bit k: SIC
+// factor > 0 at step k), else 0. col: [0, C) the IP3 input of each step,
+// [C, 2C) its noise term.
 template <int KC, class R>
-__device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
+__device__ std::uint32_t astro_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
                            const Member& M, const BlkLaunch& B, index_t ga, index_t a, std::int64_t lo,
                            std::int64_t hi, R* col) {
     const std::uint32_t oa = ga * std::uint32_t(sizeof(R)), oa4 = ga * 4u;
@@ -592,8 +589,8 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
     std::uint32_t wout = B.first ? 0u : at(D.sic_w + std::size_t(B.wslot) * D.acap, oa4);
     {
         const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
-        uint4* ent = D.bent[T_E2A] + std::size_t(ga) * kBucketCap;
-        const unsigned cnt = at(D.bcnt[T_E2A], oa4);
+        uint4* ent = bucket_entries(D, T_E2A, B.wslot) + std::size_t(ga) * kBucketCap;
+        const unsigned cnt = at(bucket_counts(D, T_E2A, B.wslot), oa4);
         sort_bucket(ent, cnt);
         auto nring = [&](unsigned k) { return S.spk_v + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.ncap; };
         gather_chunk<KC, R>(col, static_cast<unsigned>(lo - B.bstart), static_cast<unsigned>(hi - B.bstart), ent,
@@ -661,23 +658,25 @@ __device__ bool astro_cell(const BatchDev& D, const BlkState<R>& S, const RunPar
     }
     if (ever) at(D.ever, ga) = 1;
     at(D.sic_w + std::size_t(B.wslot) * D.acap, oa4) = wout;
-    if (B.last) at(D.bcnt[T_E2A], oa4) = 0;
-    return B.last && wout != 0u;
+    if (B.last) at(bucket_counts(D, T_E2A, B.wslot), oa4) = 0;
+    return B.last ? wout : 0u;
 }
 
 // One neuron (global id gj, local id j) (src/astrosim_fused.cu:433-587).
-// Returns the number of spikes. col: [0, C) the excitatory input of each
+// Returns the number of spikes; *emit: its spike word when the launch
+// completes its block, else 0. col: [0, C) the excitatory input of each
 // step, [C, 2C) the inhibitory input, then the noise values of the launch's
 // noise indices. SIC inputs stay in registers (values loaded a step ahead).
 template <int KC, class R, bool exc>
 __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
                                 const Member& M, const BlkLaunch& B, index_t gj, index_t j, std::int64_t lo,
-                                std::int64_t hi, R* col, bool* emit) {
+                                std::int64_t hi, R* col, std::uint32_t* emit) {
     const std::uint32_t on = gj * std::uint32_t(sizeof(R)), o4 = gj * 4u;
     const unsigned C = chunk_of<KC>(B);
     const unsigned noise_base = 2u * C;
     const std::uint32_t* awords = D.sic_w + std::size_t(B.rslot) * D.acap;
-    auto es = [&] { return D.bent[T_SIC] + std::size_t(gj) * kBucketCap; };
+    uint4* const es_set = bucket_entries(D, T_SIC, B.wslot);
+    auto es = [&] { return es_set + std::size_t(gj) * kBucketCap; };
     const double nstd = exc ? P.noise_exc : P.noise_inh;
     const std::int64_t tl0 = lo - M.step0;  // the member's own step at lo
     const std::uint64_t nidx0 = static_cast<std::uint64_t>(tl0 / P.steps_per_noise);
@@ -689,12 +688,12 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     R I_sic = at(S.I_sic, on);
     const R V_reset = at(S.V_reset, on), bb = at(S.b, on);
     std::uint32_t wout = B.first ? 0u : at(D.spk_w + std::size_t(B.wslot) * D.ncap, o4);
-    const unsigned cs = at(D.bcnt[T_SIC], o4);
+    const unsigned cs = at(bucket_counts(D, T_SIC, B.wslot), o4);
     {
         const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
-        uint4* ee = D.bent[T_EXC] + std::size_t(gj) * kBucketCap;
-        uint4* ei = D.bent[T_INH] + std::size_t(gj) * kBucketCap;
-        const unsigned ce = at(D.bcnt[T_EXC], o4), ci = at(D.bcnt[T_INH], o4);
+        uint4* ee = bucket_entries(D, T_EXC, B.wslot) + std::size_t(gj) * kBucketCap;
+        uint4* ei = bucket_entries(D, T_INH, B.wslot) + std::size_t(gj) * kBucketCap;
+        const unsigned ce = at(bucket_counts(D, T_EXC, B.wslot), o4), ci = at(bucket_counts(D, T_INH, B.wslot), o4);
         sort_bucket(ee, ce);
         sort_bucket(ei, ci);
         auto nring = [&](unsigned k) { return S.spk_v + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.ncap; };
@@ -932,11 +931,11 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     at(S.I_sic, on) = I_sic;
     at(D.spk_w + std::size_t(B.wslot) * D.ncap, o4) = wout;
     if (B.last) {
-        at(D.bcnt[T_EXC], o4) = 0;
-        at(D.bcnt[T_INH], o4) = 0;
-        at(D.bcnt[T_SIC], o4) = 0;
+        at(bucket_counts(D, T_EXC, B.wslot), o4) = 0;
+        at(bucket_counts(D, T_INH, B.wslot), o4) = 0;
+        at(bucket_counts(D, T_SIC, B.wslot), o4) = 0;
     }
-    *emit = B.last && wout != 0u;
+    *emit = B.last ? wout : 0u;
     return nspikes;
 }
 
@@ -995,12 +994,12 @@ __global__ __launch_bounds__(MAXT, kIntegrateWaves<R>) void integrate_kernel(con
     const index_t local = tile.first + threadIdx.x;
     if (tile.kind == 0) {
         const bool run = local < tile.end && lo < hi;
-        const bool emit = run ? astro_cell<KC, R>(D, S, P, Q, M, B, M.ab + local, local, lo, hi, col) : false;
-        wave_append(emit, M.ab + local, D.alist + std::size_t(B.wslot) * D.acap, &D.lcount[B.wslot * 2 + 1]);
+        const std::uint32_t emit = run ? astro_cell<KC, R>(D, S, P, Q, M, B, M.ab + local, local, lo, hi, col) : 0u;
+        if (B.last) scatter_emitters<R>(D, S.spk_v, B, false, M.ab + local, emit);
         return;
     }
     const bool run = local < tile.end && lo < hi;
-    bool emit = false;
+    std::uint32_t emit = 0u;
     unsigned nspikes = 0;
     if (run) {
         // one instantiation per population keeps only its constants live
@@ -1009,7 +1008,7 @@ __global__ __launch_bounds__(MAXT, kIntegrateWaves<R>) void integrate_kernel(con
         else
             nspikes = neuron_cell<KC, R, false>(D, S, P, Q, M, B, M.nb + local, local, lo, hi, col, &emit);
     }
-    wave_append(emit, M.nb + local, D.nlist + std::size_t(B.wslot) * D.ncap, &D.lcount[B.wslot * 2 + 0]);
+    if (B.last) scatter_emitters<R>(D, S.spk_v, B, true, M.nb + local, emit);
     // diagnostic spike total of the member: one atomic per wave with spikes
     unsigned long long c = nspikes;
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
@@ -1158,7 +1157,7 @@ private:
 struct BatchLayout {
     std::size_t in_rs[4], in_src[4], out_rs[4], out_ent[4], bcnt[4], bent[4];
     std::size_t astro[10], ever, neuron[12], refr, spk_n;
-    std::size_t spk_w, spk_v, sic_w, sic_v, nlist, alist, lcount, counters, overflow;
+    std::size_t spk_w, spk_v, sic_w, sic_v, counters, overflow;
     std::size_t members, tiles, clock, ctx, scratch0, scratch1, scan_tmp;
     std::size_t scan_tmp_bytes, tiles_cap, spans;
     std::size_t total;
@@ -1185,8 +1184,8 @@ BatchLayout batch_layout(const BatchCapacity& c, const RunParams& rp) {
         o.in_src[i] = L.add(ne * sizeof(index_t));
         o.out_rs[i] = L.add(ns * sizeof(uint2));
         o.out_ent[i] = L.add(ne * sizeof(uint2));
-        o.bcnt[i] = L.add(nt * sizeof(unsigned));
-        o.bent[i] = L.add(nt * kBucketCap * sizeof(uint4));
+        o.bcnt[i] = L.add(std::size_t(g.H) * nt * sizeof(unsigned));
+        o.bent[i] = L.add(std::size_t(g.H) * nt * kBucketCap * sizeof(uint4));
     }
     const int n_astro_arrays = rsize == 4 ? 10 : 7;
     for (int i = 0; i < n_astro_arrays; ++i) o.astro[i] = L.add(na * rsize);
@@ -1198,9 +1197,6 @@ BatchLayout batch_layout(const BatchCapacity& c, const RunParams& rp) {
     o.spk_v = L.add(std::size_t(g.Rv) * nn * rsize);
     o.sic_w = L.add(std::size_t(g.H) * na * 4);
     o.sic_v = L.add(std::size_t(g.Rv) * na * rsize);
-    o.nlist = L.add(std::size_t(g.H) * nn * 4);
-    o.alist = L.add(std::size_t(g.H) * na * 4);
-    o.lcount = L.add(std::size_t(g.H) * 2 * 4);
     o.counters = L.add(std::size_t(c.members) * kCountersPerMember * sizeof(unsigned long long));
     o.overflow = L.add(sizeof(unsigned));
     o.members = L.add(std::size_t(c.members) * sizeof(Member));
@@ -1270,9 +1266,6 @@ public:
         D_.spk_n = (std::uint32_t*)dptr(lay_.spk_n);
         D_.spk_w = (std::uint32_t*)dptr(lay_.spk_w);
         D_.sic_w = (std::uint32_t*)dptr(lay_.sic_w);
-        D_.nlist = (std::uint32_t*)dptr(lay_.nlist);
-        D_.alist = (std::uint32_t*)dptr(lay_.alist);
-        D_.lcount = (std::uint32_t*)dptr(lay_.lcount);
         D_.counters = (unsigned long long*)dptr(lay_.counters);
         D_.overflow = (unsigned*)dptr(lay_.overflow);
         D_.members = (const Member*)dptr(lay_.members);
@@ -1309,7 +1302,6 @@ public:
             AS_HIP_CHECK(hipMemcpyAsync(ctx_eager_, host_ctx, sizeof host_ctx, hipMemcpyHostToDevice, stream_));
             AS_HIP_CHECK(hipStreamSynchronize(stream_));
         }
-        AS_HIP_CHECK(hipMemsetAsync(D_.lcount, 0, std::size_t(geo_.H) * 2 * 4, stream_));
         AS_HIP_CHECK(hipMemsetAsync(D_.overflow, 0, sizeof(unsigned), stream_));
         AS_HIP_CHECK(hipMemsetAsync(D_.counters, 0, std::size_t(cap.members) * kCountersPerMember * 8, stream_));
         AS_HIP_CHECK(hipStreamSynchronize(stream_));
@@ -1460,7 +1452,6 @@ public:
             B.last = step_ + B.n == B.bstart + geo_.L;
             BatchDev D = D_;
             D.clock = nullptr;
-            if (B.first) hipLaunchKernelGGL(scatter_kernel<R>, dim3(kScatterBlocks), dim3(256), 0, stream_, D, B, S_.spk_v);
             timing_begin();
             launch_integrate(D, B);
             timing_end();
@@ -1755,8 +1746,6 @@ public:
     }
 
 private:
-    static constexpr unsigned kScatterBlocks = 1024;
-
     const MemberHost& host(int member) const {
         if (member < 0 || member >= int(hosts_.size()) || !hosts_[member].used)
             throw Error(AS_ERR_INVALID, "no such batch member");
@@ -1792,6 +1781,7 @@ private:
         B.Rv = geo_.Rv;
         B.wslot = static_cast<std::uint32_t>(bidx % geo_.H);
         B.rslot = static_cast<std::uint32_t>(((bidx - geo_.d / geo_.L) % geo_.H + geo_.H) % geo_.H);
+        B.bw = static_cast<std::uint32_t>((bidx + geo_.d / geo_.L) % geo_.H);
         B.first = 1;
         B.last = 1;
         B.chunk = chunk_len();
@@ -1941,7 +1931,6 @@ private:
         const BlkLaunch B = launch_for(0);  // the kernels take the block index from the clock
         AS_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         for (std::int64_t i = 0; i < blocks; ++i) {
-            hipLaunchKernelGGL(scatter_kernel<R>, dim3(kScatterBlocks), dim3(256), 0, stream_, D, B, S_.spk_v);
             for (int sub = 0; sub < geo_.L; sub += B.chunk) {
                 BlkLaunch Bs = B;
                 Bs.sub = sub;
@@ -2054,7 +2043,11 @@ private:
                                    D_.in_src[T], tbase[T], nt, sbase[T], ns, eb, scratch1_,
                                    const_cast<uint2*>(D_.out_ent[T]));
             AS_HIP_CHECK(hipGetLastError());
-            AS_HIP_CHECK(hipMemsetAsync(D_.bcnt[T] + tbase[T], 0, std::size_t(nt) * sizeof(unsigned), s));
+            // the member's bucket counts in every set
+            const std::size_t set_stride = T == T_E2A ? D_.acap : D_.ncap;
+            for (int set = 0; set < geo_.H; ++set)
+                AS_HIP_CHECK(hipMemsetAsync(D_.bcnt[T] + std::size_t(set) * set_stride + tbase[T], 0,
+                                            std::size_t(nt) * sizeof(unsigned), s));
             // the scratch arrays are reused by the next type in stream order
         }
         // parameters: FP32c stores 1/tau_IP3

@@ -1,5 +1,5 @@
 """Batched ensembles: many independent networks of uneven size advanced
-together in one batch (one scatter and one integrate launch per 1 ms block,
+together in one batch (one integrate launch per 1 ms block,
 replayed from a captured hipGraph), with the workgroup/tile size chosen by the
 batch autotuner. The reference runs one network per process; running the same
 networks one at a time is the comparison.

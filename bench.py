@@ -256,7 +256,7 @@ def activity_leg(A, n: int = 1_000_000, seed: int = 7, warm: int = 2000, steps: 
     """The two FP64 methods on one network at normal activity and at high,
     SIC-driven activity (n2a and a2n weights raised: w_n2a 100, w_a2n 5000).
     The plain method's cost is nearly independent of activity; the blocked
-    method's scatter grows with the emitters and its buckets overflow into
+    method's appends grow with the emitters and its buckets overflow into
     whole-row walks. Times `steps` steps after `warm`."""
     from astrosimgpu_cuda_amd import Batch
     net = A.Network(n, seed, setup="gpu")
@@ -437,7 +437,8 @@ def add_roofline(net, leg: dict, workload: str) -> dict:
          "reference_layout_bytes_per_step": reference_bytes_per_step(net, prec)}
     if method == "blocked":
         r["note"] = ("bound by vector-instruction issue (valu_busy_frac), not by HBM: the kernel reads each "
-                     "cell's state once per 10-step block and its inputs from buckets filled by the scatter")
+                     "cell's state once per 10-step block and its inputs from buckets filled by the previous "
+                     "block's emitters at the end of their launch")
     else:
         r["note"] = "the plain method streams every in-edge every step: the reference layout's bytes"
     return r

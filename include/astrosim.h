@@ -176,7 +176,7 @@ void as_sim_destroy(as_sim* sim);
  * No counterpart in the reference, which runs one network per process. A
  * batch holds an HBM arena of fixed capacity; member networks occupy ranges
  * of it, join and leave at block boundaries (1 ms at dt 0.1), and advance
- * together in one scatter and one integrate launch per block. A member's
+ * together in one integrate launch per block. A member's
  * results are bit-identical to running it alone (its random streams use its
  * own seed, local cell ids and local steps). Members are numbered by slot. */
 typedef struct as_batch as_batch;
