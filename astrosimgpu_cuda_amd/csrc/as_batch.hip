@@ -120,6 +120,58 @@ struct BlkLaunch {
     int lane_words;    // LDS words (of the state type) of each lane's staging column
 };
 
+// Device memory (the global address space). The arrays' pointers reach the
+// kernels inside structures, where the compiler cannot tell device memory
+// from LDS: accesses through them would be flat instructions, which take a
+// 64-bit address per access and share the scalar loads' wait counter.
+#define AS_GLOBAL __attribute__((address_space(1)))
+
+// Element at a 32-bit byte offset from a wave-uniform base pointer. One
+// offset register per cell then addresses every state array of its type
+// (scalar base + vector offset), instead of a 64-bit address per array.
+template <class T>
+__device__ __forceinline__ AS_GLOBAL T& at(T* base, std::uint32_t byte_off) {
+    return *reinterpret_cast<AS_GLOBAL T*>((AS_GLOBAL char*)(base) + byte_off);
+}
+template <class T>
+__device__ __forceinline__ const AS_GLOBAL T& at(const T* base, std::uint32_t byte_off) {
+    return *reinterpret_cast<const AS_GLOBAL T*>((const AS_GLOBAL char*)(base) + byte_off);
+}
+// HIP's vector types are classes: their copies from device memory go
+// through these builtin vectors.
+using u32x2 = unsigned __attribute__((ext_vector_type(2)));
+using u32x4 = unsigned __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint2 ld(const AS_GLOBAL uint2* p) {
+    const u32x2 v = *reinterpret_cast<const AS_GLOBAL u32x2*>(p);
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ uint4 ld(const AS_GLOBAL uint4* p) {
+    const u32x4 v = *reinterpret_cast<const AS_GLOBAL u32x4*>(p);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st(AS_GLOBAL uint4* p, const uint4& e) {
+    u32x4 v;
+    v.x = e.x;
+    v.y = e.y;
+    v.z = e.z;
+    v.w = e.w;
+    *reinterpret_cast<AS_GLOBAL u32x4*>(p) = v;
+}
+// Relaxed device-scope atomic add (atomicAdd's) on device memory.
+template <class T>
+__device__ __forceinline__ T atomic_add(AS_GLOBAL T* p, T v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The same pointer, typed as device memory.
+template <class T>
+__device__ __forceinline__ AS_GLOBAL T* gp(T* p) {
+    return (AS_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ const AS_GLOBAL T* gp(const T* p) {
+    return (const AS_GLOBAL T*)p;
+}
+
 // A value every lane of the wave holds, moved to scalar registers. Loads
 // through pointers the kernel may also write (the graph clock, the tile and
 // member tables) are vector loads, so their results, and everything computed
@@ -139,7 +191,7 @@ __device__ __forceinline__ std::int64_t uniform(std::int64_t x) {
 // launch arguments: the launch is a whole block.
 __device__ __forceinline__ void resolve_launch(BlkLaunch& B, const std::int64_t* clock) {
     if (!clock) return;
-    const std::int64_t b = uniform(*clock);
+    const std::int64_t b = uniform(*gp(clock));
     B.bstart = b * B.L;
     B.t0 = B.bstart + B.sub;
     B.n = B.L - B.sub < B.chunk ? B.L - B.sub : B.chunk;
@@ -152,17 +204,6 @@ __device__ __forceinline__ void resolve_launch(BlkLaunch& B, const std::int64_t*
 
 __device__ __forceinline__ unsigned lane_id() { return threadIdx.x & 63u; }
 
-// Element at a 32-bit byte offset from a wave-uniform base pointer. One
-// offset register per cell then addresses every state array of its type
-// (scalar base + vector offset), instead of a 64-bit address per array.
-template <class T>
-__device__ __forceinline__ T& at(T* base, std::uint32_t byte_off) {
-    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off);
-}
-template <class T>
-__device__ __forceinline__ const T& at(const T* base, std::uint32_t byte_off) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
-}
 
 // Row of step t in a ring of depth Rv (t may be negative: the value row of
 // t - d before step d is never dereferenced, but stays in range).
@@ -176,11 +217,11 @@ __device__ __forceinline__ std::uint32_t ring_next(std::uint32_t r, int Rv) {
 }
 
 // Bucket set s of connection type T (targets: neurons; astrocytes for T_E2A).
-__device__ __forceinline__ unsigned* bucket_counts(const BatchDev& D, int T, std::uint32_t s) {
-    return D.bcnt[T] + std::size_t(s) * (T == T_E2A ? D.acap : D.ncap);
+__device__ __forceinline__ AS_GLOBAL unsigned* bucket_counts(const BatchDev& D, int T, std::uint32_t s) {
+    return gp(D.bcnt[T]) + std::size_t(s) * (T == T_E2A ? D.acap : D.ncap);
 }
-__device__ __forceinline__ uint4* bucket_entries(const BatchDev& D, int T, std::uint32_t s) {
-    return D.bent[T] + std::size_t(s) * (T == T_E2A ? D.acap : D.ncap) * kBucketCap;
+__device__ __forceinline__ AS_GLOBAL uint4* bucket_entries(const BatchDev& D, int T, std::uint32_t s) {
+    return gp(D.bent[T]) + std::size_t(s) * (T == T_E2A ? D.acap : D.ncap) * kBucketCap;
 }
 
 // ---------------------------------------------------------------------------
@@ -235,9 +276,9 @@ __device__ __forceinline__ void sort_entries(uint4 (&e)[N]) {
 
 // Loads the first N entries of a bucket (empty slots: kNoSlot) and sorts them.
 template <int N>
-__device__ __forceinline__ void load_sorted(uint4 (&e)[N], const uint4* ent, unsigned cnt) {
+__device__ __forceinline__ void load_sorted(uint4 (&e)[N], const AS_GLOBAL uint4* ent, unsigned cnt) {
 #pragma unroll
-    for (int q = 0; q < N; ++q) e[q] = unsigned(q) < cnt ? ent[q] : make_uint4(kNoSlot, 0u, 0u, 0u);
+    for (int q = 0; q < N; ++q) e[q] = unsigned(q) < cnt ? ld(ent + q) : make_uint4(kNoSlot, 0u, 0u, 0u);
     sort_entries<N>(e);
 }
 
@@ -245,21 +286,21 @@ __device__ __forceinline__ void load_sorted(uint4 (&e)[N], const uint4* ent, uns
 // memory: more than four entries are rare, and a register sorting network for
 // eight would hold 32 registers at the kernel's register peak (fewer resident
 // waves). In-CSR slots are distinct, so the order is the same either way.
-__device__ __forceinline__ void sort_in_place(uint4* ent, unsigned cnt) {
+__device__ __forceinline__ void sort_in_place(AS_GLOBAL uint4* ent, unsigned cnt) {
     for (unsigned r = 0; r + 1u < cnt; ++r) {
         unsigned m = r;
-        std::uint32_t km = ent[r].x;
+        std::uint32_t km = ld(ent + r).x;
         for (unsigned q = r + 1u; q < cnt; ++q) {
-            const std::uint32_t k = ent[q].x;
+            const std::uint32_t k = ld(ent + q).x;
             if (k < km) {
                 km = k;
                 m = q;
             }
         }
         if (m != r) {
-            const uint4 a = ent[r], b = ent[m];
-            ent[r] = b;
-            ent[m] = a;
+            const uint4 a = ld(ent + r), b = ld(ent + m);
+            st(ent + r, b);
+            st(ent + m, a);
         }
     }
 }
@@ -277,14 +318,14 @@ __device__ __forceinline__ unsigned attached_step(const uint4& e) { return (e.y 
 
 // Sorts a bucket of up to kBucketCap entries in place (nothing to do for
 // fewer than two).
-__device__ __forceinline__ void sort_bucket(uint4* ent, unsigned cnt) {
+__device__ __forceinline__ void sort_bucket(AS_GLOBAL uint4* ent, unsigned cnt) {
     if (cnt < 2u || cnt > kBucketCap) return;
     if (cnt <= 4u) {
         uint4 e[4];
         load_sorted<4>(e, ent, cnt);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if (unsigned(q) < cnt) ent[q] = e[q];
+            if (unsigned(q) < cnt) st(ent + q, e[q]);
     } else {
         sort_in_place(ent, cnt);
     }
@@ -327,14 +368,15 @@ __device__ __forceinline__ void for_steps(unsigned n, F f) {
 // overflowed bucket walks the whole in-row. ring(k): the source
 // population's value-ring row of block step k.
 template <int KC, class R, class RingF, class RowF>
-__device__ void gather_chunk(R* col, unsigned c0, unsigned c1, const uint4* ent, unsigned cnt,
-                             const std::uint32_t* words, RingF ring, RowF row, const index_t* src) {
+__device__ void gather_chunk(R* col, unsigned c0, unsigned c1, const AS_GLOBAL uint4* ent, unsigned cnt,
+                             const AS_GLOBAL std::uint32_t* words, RingF ring, RowF row,
+                             const AS_GLOBAL index_t* src) {
     for_steps<KC>(c1 - c0, [&](unsigned i) { col[i] = R(0); });
     if (cnt == 0u) return;
     const std::uint32_t mask = step_mask(c0, c1);
     if (cnt <= kBucketCap) {
         for (unsigned r = 0; r < cnt; ++r) {
-            const uint4 e = ent[r];
+            const uint4 e = ld(ent + r);
             if (e.y & kAttached) {
                 const unsigned k = attached_step(e);
                 if ((mask >> k) & 1u) col[k - c0] += attached_value<R>(e);
@@ -361,12 +403,12 @@ __device__ void gather_chunk(R* col, unsigned c0, unsigned c1, const uint4* ent,
 // The in-memory path of the SIC stream: the sum of block step k over a
 // sorted bucket of at most kBucketCap entries, or the whole in-row.
 template <class R, class RowF>
-__device__ R memory_sum(const uint4* ent, unsigned cnt, unsigned k, const std::uint32_t* words, const R* vals,
-                        RowF row, const index_t* src) {
+__device__ R memory_sum(const AS_GLOBAL uint4* ent, unsigned cnt, unsigned k, const AS_GLOBAL std::uint32_t* words,
+                        const AS_GLOBAL R* vals, RowF row, const AS_GLOBAL index_t* src) {
     R sum = R(0);
     if (cnt <= kBucketCap) {
         for (unsigned r = 0; r < cnt; ++r) {
-            const index_t q = ent[r].y & kSrcMask;
+            const index_t q = ld(ent + r).y & kSrcMask;
             if ((words[q] >> k) & 1u) sum += vals[q];
         }
     } else {
@@ -389,8 +431,8 @@ struct InputStream {
 };
 
 template <class R, int NR>
-__device__ __forceinline__ void input_init(InputStream<R, NR>& s, uint4* ent, unsigned cnt,
-                                           const std::uint32_t* words) {
+__device__ __forceinline__ void input_init(InputStream<R, NR>& s, AS_GLOBAL uint4* ent, unsigned cnt,
+                                           const AS_GLOBAL std::uint32_t* words) {
     s.cnt = cnt;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -417,11 +459,11 @@ __device__ __forceinline__ void input_init(InputStream<R, NR>& s, uint4* ent, un
 
 // Loads the values of block step k from value-ring row `vals`.
 template <class R, int NR>
-__device__ __forceinline__ void input_prefetch(InputStream<R, NR>& s, unsigned k, const R* vals) {
+__device__ __forceinline__ void input_prefetch(InputStream<R, NR>& s, unsigned k, const AS_GLOBAL R* vals) {
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         R x = R(0);
-        if ((s.word[r] >> k) & 1u) x = at(vals, s.off[r]);
+        if ((s.word[r] >> k) & 1u) x = *reinterpret_cast<const AS_GLOBAL R*>((const AS_GLOBAL char*)vals + s.off[r]);
         s.v[r] = x;
     }
 }
@@ -430,8 +472,8 @@ __device__ __forceinline__ void input_prefetch(InputStream<R, NR>& s, unsigned k
 // in-memory path.
 template <class R, int NR, class EntF, class RowF>
 __device__ __forceinline__ R input_sum(const InputStream<R, NR>& s, unsigned k, EntF ent,
-                                       const std::uint32_t* words, const R* vals, RowF row,
-                                       const index_t* src) {
+                                       const AS_GLOBAL std::uint32_t* words, const AS_GLOBAL R* vals, RowF row,
+                                       const AS_GLOBAL index_t* src) {
     R sum = R(0);
 #pragma unroll
     for (int r = 0; r < NR; ++r) sum += s.v[r];
@@ -461,7 +503,7 @@ __device__ void scatter_emitters(const BatchDev& D, const R* __restrict__ spk_v,
     std::uint32_t tag = src, vlo = 0u, vhi = 0u;
     if (is_neuron && __popc(word) == 1) {
         const unsigned k = __ffs(static_cast<int>(word)) - 1;
-        const R v = spk_v[std::size_t(ring_row(B.bstart + k, B.Rv)) * D.ncap + src];
+        const R v = gp(spk_v)[std::size_t(ring_row(B.bstart + k, B.Rv)) * D.ncap + src];
         tag |= kAttached | (k << kStepShift);
         if constexpr (sizeof(R) == 8) {
             vlo = static_cast<std::uint32_t>(__double2loint(v));
@@ -480,13 +522,15 @@ __device__ void scatter_emitters(const BatchDev& D, const R* __restrict__ spk_v,
     // instead of one round per row. Bucket order does not matter: buckets are
     // sorted by in-CSR slot when they are read.
     const int T0 = is_neuron ? T_EXC : T_SIC;
-    const uint2* ent0 = D.out_ent[T0];
-    unsigned* cnt0 = bucket_counts(D, T0, B.bw);
-    uint4* bkt0 = bucket_entries(D, T0, B.bw);
-    unsigned* cnt1 = bucket_counts(D, T_INH, B.bw);
-    uint4* bkt1 = bucket_entries(D, T_INH, B.bw);
-    unsigned* cnt2 = bucket_counts(D, T_E2A, B.bw);
-    uint4* bkt2 = bucket_entries(D, T_E2A, B.bw);
+    const AS_GLOBAL uint2* ent0 = gp(D.out_ent[T0]);
+    const AS_GLOBAL uint2* ent1 = gp(D.out_ent[T_INH]);
+    const AS_GLOBAL uint2* ent2 = gp(D.out_ent[T_E2A]);
+    AS_GLOBAL unsigned* cnt0 = bucket_counts(D, T0, B.bw);
+    AS_GLOBAL uint4* bkt0 = bucket_entries(D, T0, B.bw);
+    AS_GLOBAL unsigned* cnt1 = bucket_counts(D, T_INH, B.bw);
+    AS_GLOBAL uint4* bkt1 = bucket_entries(D, T_INH, B.bw);
+    AS_GLOBAL unsigned* cnt2 = bucket_counts(D, T_E2A, B.bw);
+    AS_GLOBAL uint4* bkt2 = bucket_entries(D, T_E2A, B.bw);
     while (em != 0ull) {
         const unsigned l = static_cast<unsigned>(__builtin_ctzll(em));
         em &= em - 1ull;
@@ -494,11 +538,11 @@ __device__ void scatter_emitters(const BatchDev& D, const R* __restrict__ spk_v,
         const std::uint32_t et = __builtin_amdgcn_readlane(tag, l);
         const std::uint32_t elo = __builtin_amdgcn_readlane(vlo, l);
         const std::uint32_t ehi = __builtin_amdgcn_readlane(vhi, l);
-        const uint2 r0 = D.out_rs[T0][es];
+        const uint2 r0 = ld(gp(D.out_rs[T0]) + es);
         uint2 r1 = make_uint2(0u, 0u), r2 = make_uint2(0u, 0u);
         if (is_neuron) {
-            r1 = D.out_rs[T_INH][es];
-            r2 = D.out_rs[T_E2A][es];
+            r1 = ld(gp(D.out_rs[T_INH]) + es);
+            r2 = ld(gp(D.out_rs[T_E2A]) + es);
         }
         const unsigned n0 = r0.y - r0.x, n01 = n0 + (r1.y - r1.x), n = n01 + (r2.y - r2.x);
         for (unsigned s0 = lane_id(); s0 < n; s0 += 64u * kScatterUnroll) {
@@ -511,23 +555,23 @@ __device__ void scatter_emitters(const BatchDev& D, const R* __restrict__ spk_v,
                 if (sl >= n) continue;
                 ty[u] = sl < n0 ? 0 : (sl < n01 ? 1 : 2);
                 const std::size_t e = ty[u] == 0 ? r0.x + sl : (ty[u] == 1 ? r1.x + (sl - n0) : r2.x + (sl - n01));
-                const uint2* ent = ty[u] == 0 ? ent0 : (ty[u] == 1 ? D.out_ent[T_INH] : D.out_ent[T_E2A]);
-                te[u] = ent[e];
+                const AS_GLOBAL uint2* ent = ty[u] == 0 ? ent0 : (ty[u] == 1 ? ent1 : ent2);
+                te[u] = ld(ent + e);
             }
 #pragma unroll
             for (unsigned u = 0; u < kScatterUnroll; ++u) {
                 if (s0 + 64u * u >= n) continue;
-                unsigned* cnt = ty[u] == 0 ? cnt0 : (ty[u] == 1 ? cnt1 : cnt2);
-                slot[u] = atomicAdd(&cnt[te[u].x], 1u);
+                AS_GLOBAL unsigned* cnt = ty[u] == 0 ? cnt0 : (ty[u] == 1 ? cnt1 : cnt2);
+                slot[u] = atomic_add(cnt + te[u].x, 1u);
             }
 #pragma unroll
             for (unsigned u = 0; u < kScatterUnroll; ++u) {
                 if (s0 + 64u * u >= n) continue;
-                uint4* bkt = ty[u] == 0 ? bkt0 : (ty[u] == 1 ? bkt1 : bkt2);
+                AS_GLOBAL uint4* bkt = ty[u] == 0 ? bkt0 : (ty[u] == 1 ? bkt1 : bkt2);
                 if (slot[u] < kBucketCap)
-                    bkt[std::size_t(te[u].x) * kBucketCap + slot[u]] = make_uint4(te[u].y, et, elo, ehi);
+                    st(bkt + std::size_t(te[u].x) * kBucketCap + slot[u], make_uint4(te[u].y, et, elo, ehi));
                 else if (slot[u] == kBucketCap)
-                    atomicAdd(D.overflow, 1u);
+                    atomic_add(gp(D.overflow), 1u);
             }
         }
     }
@@ -588,13 +632,13 @@ __device__ std::uint32_t astro_cell(const BatchDev& D, const BlkState<R>& S, con
     const R Ca_tot = at(S.Ca_tot, oa), IP3_0 = at(S.IP3_0, oa), tau = at(S.tau_IP3, oa), dIP3 = at(S.delta_IP3, oa);
     std::uint32_t wout = B.first ? 0u : at(D.sic_w + std::size_t(B.wslot) * D.acap, oa4);
     {
-        const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
-        uint4* ent = bucket_entries(D, T_E2A, B.wslot) + std::size_t(ga) * kBucketCap;
+        const AS_GLOBAL std::uint32_t* nwords = gp(D.spk_w) + std::size_t(B.rslot) * D.ncap;
+        AS_GLOBAL uint4* ent = bucket_entries(D, T_E2A, B.wslot) + std::size_t(ga) * kBucketCap;
         const unsigned cnt = at(bucket_counts(D, T_E2A, B.wslot), oa4);
         sort_bucket(ent, cnt);
-        auto nring = [&](unsigned k) { return S.spk_v + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.ncap; };
+        auto nring = [&](unsigned k) { return gp(S.spk_v) + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.ncap; };
         gather_chunk<KC, R>(col, static_cast<unsigned>(lo - B.bstart), static_cast<unsigned>(hi - B.bstart), ent,
-                            cnt, nwords, nring, [&] { return D.in_rs[T_E2A][ga]; }, D.in_src[T_E2A]);
+                            cnt, nwords, nring, [&] { return ld(gp(D.in_rs[T_E2A]) + ga); }, gp(D.in_src[T_E2A]));
         // random streams tl*kStepStride + a of the member's step tl = t - step0
         // (src/astrosim_fused.cu:393-405): their bases advance by a constant
         // per step; FP32 state draws the noise in FP32 from the same uniforms
@@ -674,8 +718,8 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     const std::uint32_t on = gj * std::uint32_t(sizeof(R)), o4 = gj * 4u;
     const unsigned C = chunk_of<KC>(B);
     const unsigned noise_base = 2u * C;
-    const std::uint32_t* awords = D.sic_w + std::size_t(B.rslot) * D.acap;
-    uint4* const es_set = bucket_entries(D, T_SIC, B.wslot);
+    const AS_GLOBAL std::uint32_t* awords = gp(D.sic_w) + std::size_t(B.rslot) * D.acap;
+    AS_GLOBAL uint4* const es_set = bucket_entries(D, T_SIC, B.wslot);
     auto es = [&] { return es_set + std::size_t(gj) * kBucketCap; };
     const double nstd = exc ? P.noise_exc : P.noise_inh;
     const std::int64_t tl0 = lo - M.step0;  // the member's own step at lo
@@ -690,13 +734,13 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     std::uint32_t wout = B.first ? 0u : at(D.spk_w + std::size_t(B.wslot) * D.ncap, o4);
     const unsigned cs = at(bucket_counts(D, T_SIC, B.wslot), o4);
     {
-        const std::uint32_t* nwords = D.spk_w + std::size_t(B.rslot) * D.ncap;
-        uint4* ee = bucket_entries(D, T_EXC, B.wslot) + std::size_t(gj) * kBucketCap;
-        uint4* ei = bucket_entries(D, T_INH, B.wslot) + std::size_t(gj) * kBucketCap;
+        const AS_GLOBAL std::uint32_t* nwords = gp(D.spk_w) + std::size_t(B.rslot) * D.ncap;
+        AS_GLOBAL uint4* ee = bucket_entries(D, T_EXC, B.wslot) + std::size_t(gj) * kBucketCap;
+        AS_GLOBAL uint4* ei = bucket_entries(D, T_INH, B.wslot) + std::size_t(gj) * kBucketCap;
         const unsigned ce = at(bucket_counts(D, T_EXC, B.wslot), o4), ci = at(bucket_counts(D, T_INH, B.wslot), o4);
         sort_bucket(ee, ce);
         sort_bucket(ei, ci);
-        auto nring = [&](unsigned k) { return S.spk_v + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.ncap; };
+        auto nring = [&](unsigned k) { return gp(S.spk_v) + std::size_t(ring_row(B.bstart + k - B.d, B.Rv)) * D.ncap; };
         const unsigned k0 = static_cast<unsigned>(lo - B.bstart), k1 = static_cast<unsigned>(hi - B.bstart);
         // The Poisson drive of each step (stream tl*kStepStride + j) is folded
         // into its excitatory input as w_e*sum, then + w_poiss*events for
@@ -758,8 +802,8 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         if constexpr (sizeof(R) == 8) {
             // FP64: excitatory sums first, the m0 of pending steps in their
             // (not yet gathered) inhibitory slots, events added at once
-            gather_chunk<KC, R>(col, k0, k1, ee, ce, nwords, nring, [&] { return D.in_rs[T_EXC][gj]; },
-                                D.in_src[T_EXC]);
+            gather_chunk<KC, R>(col, k0, k1, ee, ce, nwords, nring, [&] { return ld(gp(D.in_rs[T_EXC]) + gj); },
+                                gp(D.in_src[T_EXC]));
             for_steps<KC>(n, [&](unsigned i) { col[i] = P.w_e * col[i]; });
             poisson([&](unsigned i, std::uint64_t m0) { col[C + i] = static_cast<double>(m0); },  // exact: < 2^53
                     [&](unsigned i) { return col[C + i]; },
@@ -787,8 +831,8 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
                     col[C + i] = __int_as_float(ev);  // the step's slots are free again
                     evmask |= 1u << i;
                 });
-            gather_chunk<KC, R>(col, k0, k1, ee, ce, nwords, nring, [&] { return D.in_rs[T_EXC][gj]; },
-                                D.in_src[T_EXC]);
+            gather_chunk<KC, R>(col, k0, k1, ee, ce, nwords, nring, [&] { return ld(gp(D.in_rs[T_EXC]) + gj); },
+                                gp(D.in_src[T_EXC]));
             for_steps<KC>(n, [&](unsigned i) {
                 float ein = Q.w_e_abs * col[i];
                 if ((evmask >> i) & 1u)
@@ -798,8 +842,8 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         }
         // inhibitory source ids are local indices used as neuron ids
         // (src/astrosim_fused.cu:459-462); the scatter has already resolved them
-        gather_chunk<KC, R>(col + C, k0, k1, ei, ci, nwords, nring, [&] { return D.in_rs[T_INH][gj]; },
-                            D.in_src[T_INH]);
+        gather_chunk<KC, R>(col + C, k0, k1, ei, ci, nwords, nring, [&] { return ld(gp(D.in_rs[T_INH]) + gj); },
+                            gp(D.in_src[T_INH]));
         for_steps<KC>(n, [&](unsigned i) {
             R& iin = col[C + i];
             if constexpr (sizeof(R) == 8) iin = (-P.w_i) * iin;
@@ -823,7 +867,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     InputStream<R, 1> is;
     std::uint32_t vr = ring_row(lo - B.d, B.Rv);
     input_init(is, es(), cs, awords);
-    input_prefetch(is, static_cast<unsigned>(lo - B.bstart), S.sic_v + std::size_t(vr) * D.acap);
+    input_prefetch(is, static_cast<unsigned>(lo - B.bstart), gp(S.sic_v) + std::size_t(vr) * D.acap);
     std::uint32_t pr = ring_row(lo, B.Rv);
     R noise_val = R(0);
     unsigned nspikes = 0;
@@ -858,14 +902,14 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         const bool sic_step = sic_wait == 0;
         sic_wait = sic_step ? P.sic_interval - 1 : sic_wait - 1;
         if (sic_step) {
-            const R sic_sum = input_sum(is, k, es, awords, S.sic_v + std::size_t(vr) * D.acap,
-                                        [&] { return D.in_rs[T_SIC][gj]; }, D.in_src[T_SIC]);
+            const R sic_sum = input_sum(is, k, es, awords, gp(S.sic_v) + std::size_t(vr) * D.acap,
+                                        [&] { return ld(gp(D.in_rs[T_SIC]) + gj); }, gp(D.in_src[T_SIC]));
             if constexpr (sizeof(R) == 8) I_sic = P.w_a2n * sic_sum;
             else I_sic = Q.w_a2n * sic_sum;
         }
         {
             const std::uint32_t vn = ring_next(vr, B.Rv);
-            if (t + 1 < hi) input_prefetch(is, k + 1u, S.sic_v + std::size_t(vn) * D.acap);
+            if (t + 1 < hi) input_prefetch(is, k + 1u, gp(S.sic_v) + std::size_t(vn) * D.acap);
             vr = vn;
         }
         if (nstd > 0.0 && (t == lo || nrem == 0))  // the noise stream changes every steps_per_noise steps
@@ -1013,7 +1057,7 @@ __global__ __launch_bounds__(MAXT, kIntegrateWaves<R>) void integrate_kernel(con
     unsigned long long c = nspikes;
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
     if (lane_id() == 0 && c != 0ull)
-        atomicAdd(&D.counters[std::size_t(M.slot) * kCountersPerMember + (blockIdx.x % kCountersPerMember)], c);
+        atomic_add(gp(D.counters) + std::size_t(M.slot) * kCountersPerMember + (blockIdx.x % kCountersPerMember), c);
 }
 
 // ---------------------------------------------------------------------------
