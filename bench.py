@@ -25,7 +25,8 @@ FP32c, blocked method) steps/s; every leg's figures are in `legs`:
     the timed window (HIP events around every launch on the simulation's
     stream) per 10 (blocked) or 1 (plain) steps, so that partial launches at
     the window's ends do not change the basis; and, separately, the average
-    of 20 (blocked) or 100 (plain) whole launches right after the window;
+    of 20 (blocked) or 100 (plain) whole launches shortly after the window
+    (after 5 or 50 untimed ones);
   - that kernel's own algorithmic HBM bytes per launch (see
     kernel_bytes_per_launch), the GB/s they give at that launch time and the
     fraction of the 8 TB/s peak, beside the rocprofv3-measured HBM bytes
@@ -137,10 +138,12 @@ def run_leg(A, net, prec: str, method: str, steps: int, warmup: int, block: int,
     if deg is not None:
         d = sim.spike_counts().astype(np.int64) - counts0.astype(np.int64)
         events = int(d @ deg)
-    # full launches right after the window: align to a block, then time whole blocks
+    # full launches right after the window: align to a block and run a few
+    # untimed launches (the host's readbacks above leave the GPU idle, and its
+    # clock ramps down meanwhile), then time whole blocks
     L = 10 if method == "blocked" else 1
     n_full = 20 if method == "blocked" else 100
-    sim.step((-sim.steps) % L)
+    sim.step((-sim.steps) % L + (5 if method == "blocked" else 50) * L)
     sim.set_timing(True)
     sim.step(n_full * L)
     kms, launches = sim.kernel_time()
