@@ -162,6 +162,13 @@ template <class T>
 __device__ __forceinline__ T atomic_add(AS_GLOBAL T* p, T v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// x, hidden from the optimiser: values derived from it are computed where they
+// are used, instead of being hoisted and held in scalar registers across the
+// step loop (where the integrate kernel's scalar registers run short).
+__device__ __forceinline__ std::uint32_t opaque(std::uint32_t x) {
+    asm volatile("" : "+s"(x));
+    return x;
+}
 // The same pointer, typed as device memory.
 template <class T>
 __device__ __forceinline__ AS_GLOBAL T* gp(T* p) {
@@ -361,6 +368,17 @@ __device__ __forceinline__ void for_steps(unsigned n, F f) {
     }
 }
 
+// A lane's staging slots in LDS. Each wave owns lane_words x 64 words, slot
+// by slot: slot i of lane l is word i * 64 + l of its wave's region, so a
+// wave's accesses to one slot are 64 consecutive words (no bank conflicts at
+// any slot count) and every slot is an immediate offset from the lane's base.
+template <class R>
+struct Col {
+    R* p;  // slot 0 of the lane
+    __device__ __forceinline__ R& operator[](unsigned i) const { return p[i * 64u]; }
+    __device__ __forceinline__ Col operator+(unsigned i) const { return Col{p + i * 64u}; }
+};
+
 // Input sums of one target and connection type for block steps [c0, c1),
 // into the lane's LDS slots col[k - c0], in ascending in-CSR slot order: the
 // bucket is sorted; an attached entry adds its value at its step, an
@@ -368,7 +386,7 @@ __device__ __forceinline__ void for_steps(unsigned n, F f) {
 // overflowed bucket walks the whole in-row. ring(k): the source
 // population's value-ring row of block step k.
 template <int KC, class R, class RingF, class RowF>
-__device__ void gather_chunk(R* col, unsigned c0, unsigned c1, const AS_GLOBAL uint4* ent, unsigned cnt,
+__device__ void gather_chunk(Col<R> col, unsigned c0, unsigned c1, const AS_GLOBAL uint4* ent, unsigned cnt,
                              const AS_GLOBAL std::uint32_t* words, RingF ring, RowF row,
                              const AS_GLOBAL index_t* src) {
     for_steps<KC>(c1 - c0, [&](unsigned i) { col[i] = R(0); });
@@ -617,7 +635,7 @@ __device__ __forceinline__ FixedDivisor in_vgpr(const FixedDivisor& d) {
 template <int KC, class R>
 __device__ std::uint32_t astro_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
                            const Member& M, const BlkLaunch& B, index_t ga, index_t a, std::int64_t lo,
-                           std::int64_t hi, R* col) {
+                           std::int64_t hi, Col<R> col) {
     const std::uint32_t oa = ga * std::uint32_t(sizeof(R)), oa4 = ga * 4u;
     const unsigned C = chunk_of<KC>(B);
     const unsigned n = static_cast<unsigned>(hi - lo);
@@ -702,7 +720,7 @@ __device__ std::uint32_t astro_cell(const BatchDev& D, const BlkState<R>& S, con
     }
     if (ever) at(D.ever, ga) = 1;
     at(D.sic_w + std::size_t(B.wslot) * D.acap, oa4) = wout;
-    if (B.last) at(bucket_counts(D, T_E2A, B.wslot), oa4) = 0;
+    if (B.last) at(bucket_counts(D, T_E2A, opaque(B.wslot)), oa4) = 0;
     return B.last ? wout : 0u;
 }
 
@@ -714,13 +732,19 @@ __device__ std::uint32_t astro_cell(const BatchDev& D, const BlkState<R>& S, con
 template <int KC, class R, bool exc>
 __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const RunParams& P, const Params32& Q,
                                 const Member& M, const BlkLaunch& B, index_t gj, index_t j, std::int64_t lo,
-                                std::int64_t hi, R* col, std::uint32_t* emit) {
+                                std::int64_t hi, Col<R> col, std::uint32_t* emit) {
     const std::uint32_t on = gj * std::uint32_t(sizeof(R)), o4 = gj * 4u;
     const unsigned C = chunk_of<KC>(B);
     const unsigned noise_base = 2u * C;
+    // FP64 with at most two noise indices per launch (steps_per_noise >= C,
+    // the usual case) keeps their values in registers and only the inputs in
+    // LDS: 20 words per lane instead of 22 at C = 10 let 4 waves per SIMD share
+    // a CU instead of 3. FP32c keeps them in LDS: its LDS already allows the 7
+    // waves its registers allow, and two more registers measured slower.
+    const bool noise_lds = sizeof(R) == 4 || (C - 1u) / unsigned(P.steps_per_noise) != 0u;
+    R noise_first = R(0), noise_next = R(0);
     const AS_GLOBAL std::uint32_t* awords = gp(D.sic_w) + std::size_t(B.rslot) * D.acap;
-    AS_GLOBAL uint4* const es_set = bucket_entries(D, T_SIC, B.wslot);
-    auto es = [&] { return es_set + std::size_t(gj) * kBucketCap; };
+    auto es = [&] { return bucket_entries(D, T_SIC, opaque(B.wslot)) + std::size_t(gj) * kBucketCap; };
     const double nstd = exc ? P.noise_exc : P.noise_inh;
     const std::int64_t tl0 = lo - M.step0;  // the member's own step at lo
     const std::uint64_t nidx0 = static_cast<std::uint64_t>(tl0 / P.steps_per_noise);
@@ -859,7 +883,10 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
                 R v;
                 if constexpr (sizeof(R) == 8) v = nstd * rng_normal_z(zn);
                 else v = static_cast<float>(nstd) * normal32(zn);
-                col[noise_base + static_cast<unsigned>(idx - nidx0)] = v;
+                const unsigned sl = static_cast<unsigned>(idx - nidx0);
+                if (noise_lds) col[noise_base + sl] = v;
+                else if (sl == 0u) noise_first = v;
+                else noise_next = v;
             }
         }
     }
@@ -913,7 +940,7 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
             vr = vn;
         }
         if (nstd > 0.0 && (t == lo || nrem == 0))  // the noise stream changes every steps_per_noise steps
-            noise_val = col[noise_base + nslot];
+            noise_val = noise_lds ? col[noise_base + nslot] : (nslot == 0u ? noise_first : noise_next);
         bool spiked;
         if constexpr (sizeof(R) == 8) {
             const NeuronConst& p = pc;
@@ -975,9 +1002,10 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     at(S.I_sic, on) = I_sic;
     at(D.spk_w + std::size_t(B.wslot) * D.ncap, o4) = wout;
     if (B.last) {
-        at(bucket_counts(D, T_EXC, B.wslot), o4) = 0;
-        at(bucket_counts(D, T_INH, B.wslot), o4) = 0;
-        at(bucket_counts(D, T_SIC, B.wslot), o4) = 0;
+        const std::uint32_t ws = opaque(B.wslot);
+        at(bucket_counts(D, T_EXC, ws), o4) = 0;
+        at(bucket_counts(D, T_INH, ws), o4) = 0;
+        at(bucket_counts(D, T_SIC, ws), o4) = 0;
     }
     *emit = B.last ? wout : 0u;
     return nspikes;
@@ -995,13 +1023,15 @@ struct IntegrateCtx {
 };
 
 // Minimum waves per SIMD the integrate kernel's register budget targets, per
-// state precision (chosen by measurement: past these, spills cost more than
-// the extra waves hide).
+// state precision. FP32c: the 7 its LDS staging allows at the default 10-step
+// chunk (22 words per lane), reached without spills (68 VGPRs). FP64: 3; it
+// uses 128 VGPRs anyway, which with its 20 LDS words per lane gives 4 waves,
+// and a bound of 4 measured 0.6% slower (a different register assignment).
 #ifndef AS_INTEGRATE_WAVES_F64
 #define AS_INTEGRATE_WAVES_F64 3
 #endif
 #ifndef AS_INTEGRATE_WAVES_F32
-#define AS_INTEGRATE_WAVES_F32 3
+#define AS_INTEGRATE_WAVES_F32 7
 #endif
 template <class R>
 constexpr int kIntegrateWaves = sizeof(R) == 8 ? AS_INTEGRATE_WAVES_F64 : AS_INTEGRATE_WAVES_F32;
@@ -1010,9 +1040,10 @@ constexpr int kIntegrateWaves = sizeof(R) == 8 ? AS_INTEGRATE_WAVES_F64 : AS_INT
 // 10 steps), 0 otherwise.
 template <int MAXT, class R, int KC>
 __global__ __launch_bounds__(MAXT, kIntegrateWaves<R>) void integrate_kernel(const IntegrateCtx<R>* __restrict__ ctx, BlkLaunch B) {
-    // each lane's staging slots: B.lane_words consecutive words of R
+    // each lane's staging slots: B.lane_words slots of R in its wave's region
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    R* col = reinterpret_cast<R*>(lds_raw) + threadIdx.x * unsigned(B.lane_words);
+    const Col<R> col{reinterpret_cast<R*>(lds_raw) + (threadIdx.x >> 6) * (unsigned(B.lane_words) * 64u) +
+                     (threadIdx.x & 63u)};
     const BatchDev& D = ctx->D;
     const BlkState<R>& S = ctx->S;
     const RunParams& P = ctx->P;
@@ -1835,14 +1866,15 @@ private:
 
     // LDS words (of R) per lane that an integrate launch of c steps stages:
     // excitatory and inhibitory inputs per step and the noise values of up to
-    // (c-1)/steps_per_noise + 2 noise indices.
+    // (c-1)/steps_per_noise + 2 noise indices, except where the kernel keeps
+    // those in registers (FP64 with at most two per launch).
     int lds_words(int c) const {
-        return 2 * c + (c - 1) / P_.steps_per_noise + 2;
+        const int extra = (c - 1) / P_.steps_per_noise;
+        return 2 * c + (sizeof(R) == 4 || extra > 0 ? extra + 2 : 0);
     }
-    // A lane's slots in LDS, rounded up to an odd number of 32-bit words
-    // (FP64: of 64-bit words) so that a wave's accesses to one slot hit
-    // distinct banks.
-    int lane_words(int c) const { return lds_words(c) | 1; }
+    // A lane's slots in LDS (slot-major in its wave's region: no padding is
+    // needed against bank conflicts).
+    int lane_words(int c) const { return lds_words(c); }
     // Most steps per integrate launch: the staged words fit 208 bytes per
     // lane (3 resident waves per SIMD share 160 KB) and 64 KB per workgroup.
     int chunk_len() const {
