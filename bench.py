@@ -26,7 +26,7 @@ FP32c, blocked method) steps/s; every leg's figures are in `legs`:
     stream) per 10 (blocked) or 1 (plain) steps, so that partial launches at
     the window's ends do not change the basis; and, separately, the average
     of 20 (blocked) or 100 (plain) whole launches shortly after the window
-    (after 5 or 50 untimed ones);
+    (after 50 or 200 untimed ones);
   - that kernel's own algorithmic HBM bytes per launch (see
     kernel_bytes_per_launch), the GB/s they give at that launch time and the
     fraction of the 8 TB/s peak, beside the rocprofv3-measured HBM bytes
@@ -143,7 +143,7 @@ def run_leg(A, net, prec: str, method: str, steps: int, warmup: int, block: int,
     # clock ramps down meanwhile), then time whole blocks
     L = 10 if method == "blocked" else 1
     n_full = 20 if method == "blocked" else 100
-    sim.step((-sim.steps) % L + (5 if method == "blocked" else 50) * L)
+    sim.step((-sim.steps) % L + (50 if method == "blocked" else 200) * L)
     sim.set_timing(True)
     sim.step(n_full * L)
     kms, launches = sim.kernel_time()
