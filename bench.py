@@ -58,6 +58,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# Blocked-method tile (threads per workgroup) measured fastest at N=10M: FP32c
+# 128 (0.656 ms per launch against 0.676 at 64); FP64 64 (1.59 ms, 128 the same
+# within 1%).
+BLOCKED_TILE = {"fp64": 64, "fp32c": 128}
 SIC_TH = 0.19669
 
 
@@ -501,8 +505,9 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--block", type=int, default=128,
                     help="threads per workgroup of the plain method (the CLI's --block)")
-    ap.add_argument("--tile", type=int, default=64,
-                    help="threads per workgroup (tile) of the blocked method: 64 measured fastest at N=10M")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="threads per workgroup (tile) of the blocked method; 0: the fastest measured at N=10M, "
+                         "64 for FP64 and 128 for FP32c")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
@@ -539,7 +544,7 @@ def main() -> int:
 
     legs = [tuple(x.split(":")) for x in args.legs.split(",")]
     results = [run_leg(A, net, p, m, args.plain_steps if (m == "plain" and args.plain_steps > 0) else args.steps,
-                       args.warmup, args.tile if m == "blocked" else args.block, deg)
+                       args.warmup, (args.tile or BLOCKED_TILE[p]) if m == "blocked" else args.block, deg)
                for p, m in legs]
     workload = f"n{args.n}_seed{args.seed}"
     for leg in results:
