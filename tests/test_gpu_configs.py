@@ -79,6 +79,35 @@ def test_config3_plain_equals_blocked_fp64(net10m):
     sb.close()
 
 
+@pytest.mark.timeout(600)
+def test_config3_plain_equals_blocked_fp64_through_sic_activity(net10m):
+    """12,010 steps: from about step 6,000 astrocytes turn SIC-active (some
+    1,800 by the end), neurons then receive SIC from several astrocytes in a
+    block, and SIC buckets overflow into whole in-row walks. The blocked
+    method (a one-member batch, whose counters report the overflows) stays
+    bit-identical to the plain method through all of it."""
+    steps = 12010
+    print(f"[config3] plain fp64 {steps} steps", flush=True)
+    sp = A.Simulation(net10m, method="plain")
+    sp.step(steps)
+    got = {f: sp.state(f) for f in FIELDS}
+    sps = sp.spike_count()
+    sp.close()
+    print(f"[config3] blocked fp64 {steps} steps", flush=True)
+    b = A.Batch([net10m], method="blocked", block=64)
+    m = b.add(net10m, steps)
+    b.step(steps)
+    b.sync()
+    for f in FIELDS:
+        assert differing(got[f], b.state(m, f)) == 0, f
+    assert b.spike_count(m) == sps
+    overflows = b.info()["bucket_overflows"]
+    active = int(np.count_nonzero((got["Ca"] - SIC_TH) * 1000.0 > 1.0))
+    print(f"[config3] spikes {sps}, bucket overflows {overflows}, SIC-active astrocytes {active}", flush=True)
+    assert overflows > 0 and active > 100
+    b.close()
+
+
 def run_config3(net, precision):
     pre, sim_steps = 1000, 10000  # pre 100 ms, sim 1000 ms at dt 0.1 ms
     sim = A.Simulation(net, precision=precision, method="blocked")
