@@ -92,7 +92,15 @@ struct BatchDev {
     const Tile* tiles;
     std::int64_t* clock;  // block index for graph replays
     std::uint64_t ncap, acap;
+    std::uint64_t sstride;  // row stride of the SIC value ring: acap + ncap (FP32c), acap (FP64)
 };
+
+// Row stride of the SIC value ring for state type R (FP64 reads acap, the
+// field its kernels always held).
+template <class R>
+__device__ __forceinline__ std::uint64_t sic_stride(const BatchDev& D) {
+    return sizeof(R) == 4 ? D.sstride : D.acap;
+}
 
 // State arrays in the precision R (double: FP64; float: FP32c, where
 // Ca/IP3/h carry compensation terms and tau_IP3 holds 1/tau_IP3).
@@ -105,7 +113,7 @@ struct BlkState {
     R *V_reset, *b;
     R *stp_x, *stp_u, *stp_t;
     R* spk_v;  // [Rv][ncap] x*u at the spike step
-    R* sic_v;  // [Rv][acap] SIC factor where nonzero
+    R* sic_v;  // [Rv][acap + ncap] SIC factor where nonzero; columns acap + neuron: see neuron_cell
 };
 
 struct BlkLaunch {
@@ -706,7 +714,7 @@ __device__ std::uint32_t astro_cell(const BatchDev& D, const BlkState<R>& S, con
         if (factor > R(0)) {
             ever = true;
             wout |= 1u << k;
-            at(S.sic_v + std::size_t(pr) * D.acap, oa) = factor;
+            at(S.sic_v + std::size_t(pr) * sic_stride<R>(D), oa) = factor;
         }
         pr = ring_next(pr, B.Rv);
     }
@@ -894,7 +902,60 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
     InputStream<R, 1> is;
     std::uint32_t vr = ring_row(lo - B.d, B.Rv);
     input_init(is, es(), cs, awords);
-    input_prefetch(is, static_cast<unsigned>(lo - B.bstart), gp(S.sic_v) + std::size_t(vr) * D.acap);
+    input_prefetch(is, static_cast<unsigned>(lo - B.bstart), gp(S.sic_v) + std::size_t(vr) * sic_stride<R>(D));
+    // FP32c with KC compiled in: a neuron with several SIC sources in the
+    // block (several active astrocytes in reach) has its per-step SIC sums
+    // computed once per launch: its sources in slot order (the sorted bucket,
+    // or the in-row when it overflowed), each source's values of all the
+    // launch's steps loaded together and added at its emitted steps, so every
+    // step's sum has the in-memory path's terms in the same order. The sums go
+    // to the neuron's own column (acap + neuron) of the SIC value ring's rows
+    // of those steps, and the neuron's register stream then reads that column
+    // as if it were a single source: the step loop loads one value per step,
+    // a step ahead, instead of walking the sources in device memory at every
+    // step (which held waves for hundreds of microseconds once many
+    // astrocytes were active). FP64 has no registers to spare for it.
+    if constexpr (KC > 0 && sizeof(R) == 4) {
+        if (cs > 1u) {
+            const unsigned k0 = static_cast<unsigned>(lo - B.bstart), n = static_cast<unsigned>(hi - lo);
+            const std::uint32_t mask = step_mask(k0, k0 + n);
+            R acc[KC];
+#pragma unroll
+            for (unsigned i = 0; i < unsigned(KC); ++i) acc[i] = R(0);
+            const bool in_bucket = cs <= kBucketCap;
+            index_t e0 = 0u, e1 = cs;
+            if (!in_bucket) {
+                const uint2 rw = ld(gp(D.in_rs[T_SIC]) + gj);
+                e0 = rw.x;
+                e1 = rw.y;
+            }
+            const AS_GLOBAL index_t* isrc = gp(D.in_src[T_SIC]);
+            const AS_GLOBAL uint4* ent = es();
+            for (index_t e = e0; e < e1; ++e) {
+                const index_t q = in_bucket ? (ld(ent + e).y & kSrcMask) : isrc[e];
+                const std::uint32_t w = (awords[q] & mask) >> k0;
+                if (w == 0u) continue;
+                R v[KC];
+#pragma unroll
+                for (unsigned i = 0; i < unsigned(KC); ++i) {
+                    v[i] = R(0);
+                    if ((w >> i) & 1u)
+                        v[i] = gp(S.sic_v)[std::size_t(ring_row(B.bstart + k0 + i - B.d, B.Rv)) * sic_stride<R>(D) + q];
+                }
+#pragma unroll
+                for (unsigned i = 0; i < unsigned(KC); ++i)
+                    if ((w >> i) & 1u) acc[i] += v[i];
+            }
+            const std::size_t col_n = D.acap + gj;
+#pragma unroll
+            for (unsigned i = 0; i < unsigned(KC); ++i)
+                if (i < n) gp(S.sic_v)[std::size_t(ring_row(B.bstart + k0 + i - B.d, B.Rv)) * sic_stride<R>(D) + col_n] = acc[i];
+            is.cnt = 1u;
+            is.off[0] = static_cast<std::uint32_t>(col_n * sizeof(R));
+            is.word[0] = mask;
+            input_prefetch(is, k0, gp(S.sic_v) + std::size_t(vr) * sic_stride<R>(D));
+        }
+    }
     std::uint32_t pr = ring_row(lo, B.Rv);
     R noise_val = R(0);
     unsigned nspikes = 0;
@@ -929,14 +990,14 @@ __device__ unsigned neuron_cell(const BatchDev& D, const BlkState<R>& S, const R
         const bool sic_step = sic_wait == 0;
         sic_wait = sic_step ? P.sic_interval - 1 : sic_wait - 1;
         if (sic_step) {
-            const R sic_sum = input_sum(is, k, es, awords, gp(S.sic_v) + std::size_t(vr) * D.acap,
+            const R sic_sum = input_sum(is, k, es, awords, gp(S.sic_v) + std::size_t(vr) * sic_stride<R>(D),
                                         [&] { return ld(gp(D.in_rs[T_SIC]) + gj); }, gp(D.in_src[T_SIC]));
             if constexpr (sizeof(R) == 8) I_sic = P.w_a2n * sic_sum;
             else I_sic = Q.w_a2n * sic_sum;
         }
         {
             const std::uint32_t vn = ring_next(vr, B.Rv);
-            if (t + 1 < hi) input_prefetch(is, k + 1u, gp(S.sic_v) + std::size_t(vn) * D.acap);
+            if (t + 1 < hi) input_prefetch(is, k + 1u, gp(S.sic_v) + std::size_t(vn) * sic_stride<R>(D));
             vr = vn;
         }
         if (nstd > 0.0 && (t == lo || nrem == 0))  // the noise stream changes every steps_per_noise steps
@@ -1271,7 +1332,7 @@ BatchLayout batch_layout(const BatchCapacity& c, const RunParams& rp) {
     o.spk_w = L.add(std::size_t(g.H) * nn * 4);
     o.spk_v = L.add(std::size_t(g.Rv) * nn * rsize);
     o.sic_w = L.add(std::size_t(g.H) * na * 4);
-    o.sic_v = L.add(std::size_t(g.Rv) * na * rsize);
+    o.sic_v = L.add(std::size_t(g.Rv) * (rsize == 4 ? na + nn : na) * rsize);
     o.counters = L.add(std::size_t(c.members) * kCountersPerMember * sizeof(unsigned long long));
     o.overflow = L.add(sizeof(unsigned));
     o.members = L.add(std::size_t(c.members) * sizeof(Member));
@@ -1364,6 +1425,7 @@ public:
         clock_ = (std::int64_t*)dptr(lay_.clock);
         D_.ncap = nn;
         D_.acap = na;
+        D_.sstride = sizeof(R) == 4 ? na + nn : na;
         scratch0_ = (index_t*)dptr(lay_.scratch0);
         scratch1_ = (index_t*)dptr(lay_.scratch1);
         scan_tmp_ = dptr(lay_.scan_tmp);
